@@ -1,0 +1,193 @@
+"""Benchmark: FP64 FCT tracer advection on one MI355X.
+
+Workload (default): icosahedral sphere level 9 (10*4**9+2 = 2,621,442 nodes before
+~30% land is carved out), 70 levels (nl = 70 interfaces, bottom level 10..70 per
+column from a seeded bathymetry), 2 tracers (T, S), dt = 2700 s, seeded
+streamfunction velocity (CFL <= 0.5), w by continuity. A "step" is one FCT step of
+all tracers; steps are chained (T_old <- T, T <- T_new) with the inputs resident in
+HBM. Metric: node-level updates/s (tracers x wet node-layers per step / step time).
+
+Prints one JSON line (rank 0, N = 1 GPU only).
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+from ctypes import c_void_p
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(level, nl, ntracers, seed, dt):
+    """Time the numpy oracle (one thread) on a smaller mesh of the same recipe."""
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    from fesom_jax_amd.inputs import make_case
+    from fesom_jax_amd.meshgen import icosahedral_mesh
+    from fesom_jax_amd.perfmodel import updates_per_step
+    from oracle import fct_numpy as O
+    mesh = icosahedral_mesh(level, nl, seed=seed)
+    kinds = ("T", "S", "T", "S")[:ntracers]
+    case = make_case(mesh, seed=seed + 1, dt=dt, tracers=kinds)
+    t0 = time.perf_counter()
+    for j in range(ntracers):
+        O.advect_one_fct(mesh, case["uv"], case["w_e"], case["helem"], case["hnode"],
+                         case["hnode_new"], case["T"][j], case["T_old"][j], dt=dt)
+    el = time.perf_counter() - t0
+    u = updates_per_step(mesh, ntracers)
+    return {"value": u / el, "unit": "node-level updates/s", "cores": 1, "kind": "port",
+            "sample": (f"numpy FP64 restatement of the reference FCT (oracle/fct_numpy.py), "
+                       f"1 step x {ntracers} tracers on icosa-{level} ({mesh.nod2D} nodes, "
+                       f"nl={nl}, {u} updates), {el:.2f} s on {cpu_model()}")}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--level", type=int, default=9)
+    ap.add_argument("--nl", type=int, default=70)
+    ap.add_argument("--tracers", type=int, default=2)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--dt", type=float, default=2700.0)
+    ap.add_argument("--cpu-level", type=int, default=6)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from fesom_jax_amd import _lib
+    from fesom_jax_amd.inputs import make_case
+    from fesom_jax_amd.meshgen import icosahedral_mesh
+    from fesom_jax_amd.perfmodel import HBM_PEAK_GBS, algorithmic_bytes, updates_per_step
+    from fesom_jax_amd.tracer_adv import get_plan
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("bench.py needs a ROCm GPU")
+    dev = torch.device("cuda", 0)
+    K = args.tracers
+
+    t0 = time.time()
+    mesh = icosahedral_mesh(args.level, args.nl, seed=args.seed)
+    log(f"mesh icosa-{args.level}: N={mesh.nod2D} E={mesh.elem2D} D={mesh.edge2D} "
+        f"nl={mesh.nl} ({time.time() - t0:.1f}s)")
+    t0 = time.time()
+    kinds = ("T", "S", "T", "S")[:K]
+    case = make_case(mesh, seed=args.seed + 1, dt=args.dt, tracers=kinds)
+    log(f"inputs ({time.time() - t0:.1f}s) cfl={case['cfl']} vel_scale={case['vel_scale']:.4f}")
+
+    up = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    uv, w_e, helem = up(case["uv"]), up(case["w_e"]), up(case["helem"])
+    hnode, hnn = up(case["hnode"]), up(case["hnode_new"])
+    # three rotating buffers per tracer: (T_old, T, T_new)
+    bufs = [[up(case["T_old"][j]), up(case["T"][j]),
+             torch.empty((mesh.nod2D, mesh.nl), dtype=torch.float64, device=dev)]
+            for j in range(K)]
+    del case
+    plan = get_plan(mesh, dev)
+    lib = plan.lib
+    log(f"plan workspace {plan.workspace_bytes / 2**30:.2f} GiB")
+
+    def make_args(rot):
+        olds = (c_void_p * K)(*[bufs[j][rot % 3].data_ptr() for j in range(K)])
+        curs = (c_void_p * K)(*[bufs[j][(rot + 1) % 3].data_ptr() for j in range(K)])
+        news = (c_void_p * K)(*[bufs[j][(rot + 2) % 3].data_ptr() for j in range(K)])
+        a = _lib.FctStepArgs(ntracers=K, dt=args.dt, uv=uv.data_ptr(), w_e=w_e.data_ptr(),
+                             helem=helem.data_ptr(), hnode=hnode.data_ptr(),
+                             hnode_new=hnn.data_ptr(),
+                             T=ctypes.cast(curs, ctypes.POINTER(c_void_p)),
+                             T_old=ctypes.cast(olds, ctypes.POINTER(c_void_p)),
+                             T_new=ctypes.cast(news, ctypes.POINTER(c_void_p)),
+                             w_i=None, Z3d=None, zbar3=None)
+        return a, (olds, curs, news)
+
+    rots = [make_args(r) for r in range(3)]
+    stream = torch.cuda.current_stream(dev)
+    sp = c_void_p(stream.cuda_stream)
+
+    def step(i):
+        a = rots[i % 3][0]
+        _lib.check(lib.fct_forward(plan.ptr, ctypes.byref(a), sp), "fct_forward")
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    log(f"warmup {args.warmup} steps done")
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter()
+    start.record(stream)
+    for i in range(args.steps):
+        step(args.warmup + i)
+    end.record(stream)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter() - t_wall
+    ms = start.elapsed_time(end) / args.steps
+    log(f"timed {args.steps} steps: {ms:.3f} ms/step (wall {1e3 * t_wall / args.steps:.3f})")
+
+    # sanity: the chained field stays finite and bounded
+    last = bufs[0][(args.warmup + args.steps + 1) % 3]
+    m = torch.from_numpy(mesh.node_layer_mask).to(dev)
+    if not bool(torch.isfinite(last[m]).all()):
+        raise RuntimeError("non-finite tracer after the timed steps")
+
+    U = updates_per_step(mesh, K)
+    B = algorithmic_bytes(mesh, K)
+    value = U / (ms * 1e-3)
+    achieved = B / (ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "fct_step (all kernels of one step)",
+                "bytes_per_step": B}
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(args.cpu_level, args.nl, K, args.seed, args.dt)
+        log(f"cpu baseline {cpu['value']:.3e} updates/s")
+    out = {
+        "metric": "FP64 FCT node-level updates/s + %HBM roofline; served p50/p99 vs plain loop",
+        "value": value, "unit": "node-level updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"icosa-{args.level} x nl={mesh.nl}, {K} tracers, plain FCT step",
+                   "nodes": mesh.nod2D, "elements": mesh.elem2D, "edges": mesh.edge2D,
+                   "levels": mesh.nl, "tracers": K, "updates_per_step": U, "dt": args.dt,
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,127 @@
+"""ctypes binding of libfesom_fct.so (declared in include/fesom_fct.h).
+
+Loading is explicit and fails loudly: there is no CPU fallback for the FCT path.
+``torch`` is imported before the library is opened so that the HIP runtime torch
+ships is the one the library binds to (both use the soname libamdhip64.so.7).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_double, c_int32, c_int64, c_void_p
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libfesom_fct.so"
+
+FCT_ABI_VERSION = 1
+FCT_OK = 0
+FCT_E_INVALID = -1
+FCT_E_HIP = -2
+FCT_E_NOMEM = -3
+FCT_E_UNSUPPORTED = -4
+
+FCT_PATH_AUTO = 0
+FCT_PATH_PHASED = 1
+FCT_PATH_FUSED = 2
+
+FCT_TUNE_XCD = 1
+
+FCT_PHASE_LO = 1
+FCT_PHASE_GRAD = 2
+FCT_PHASE_PM = 3
+FCT_PHASE_FINAL = 4
+
+EXPORTED_SYMBOLS = ("fct_abi_version", "fct_last_error", "fct_plan_create",
+                    "fct_plan_get_info", "fct_plan_destroy", "fct_forward",
+                    "fct_forward_ex", "fct_forward_phase", "fct_plan_set_tuning")
+
+
+class FctMeshDesc(ctypes.Structure):
+    _fields_ = [("nod2D", c_int32), ("elem2D", c_int32), ("edge2D", c_int32), ("nl", c_int32),
+                ("edges", c_void_p), ("edge_tri", c_void_p), ("edge_cross_dxdy", c_void_p),
+                ("edge_dxdy", c_void_p), ("edge_up_dn_tri", c_void_p),
+                ("elem_nodes", c_void_p), ("gradient_sca", c_void_p), ("elem_area", c_void_p),
+                ("elem_cos", c_void_p), ("nlevels", c_void_p), ("ulevels", c_void_p),
+                ("nlevels_nod2D", c_void_p), ("nlevels_nod2D_min", c_void_p),
+                ("ulevels_nod2D", c_void_p), ("ulevels_nod2D_max", c_void_p),
+                ("nod_in_elem2D_offsets", c_void_p), ("nod_in_elem2D", c_void_p),
+                ("area", c_void_p), ("areasvol", c_void_p), ("Z", c_void_p), ("zbar", c_void_p),
+                ("max_tracers", c_int32)]
+
+
+class FctStepArgs(ctypes.Structure):
+    _fields_ = [("ntracers", c_int32), ("dt", c_double),
+                ("uv", c_void_p), ("w_e", c_void_p), ("helem", c_void_p),
+                ("hnode", c_void_p), ("hnode_new", c_void_p),
+                ("T", POINTER(c_void_p)), ("T_old", POINTER(c_void_p)),
+                ("T_new", POINTER(c_void_p)),
+                ("w_i", c_void_p), ("Z3d", c_void_p), ("zbar3", c_void_p)]
+
+
+class FctPhaseBufs(ctypes.Structure):
+    _fields_ = [("LO", c_void_p), ("tr_xy", c_void_p), ("fct_plus", c_void_p),
+                ("fct_minus", c_void_p)]
+
+
+class FctPlanInfo(ctypes.Structure):
+    _fields_ = [("nod2D", c_int32), ("elem2D", c_int32), ("edge2D", c_int32), ("nl", c_int32),
+                ("workspace_bytes", c_int64), ("device", c_int32)]
+
+
+_LIB = None
+
+
+def load(path: Path | None = None, *, require_torch: bool = True):
+    """Open the library once and declare the signatures. Raises ``RuntimeError`` if
+    the library has not been built."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    if require_torch:
+        import torch  # noqa: F401  (binds the process to torch's HIP runtime first)
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"{p} is missing: build the HIP library first "
+            "(python -c 'import __graft_entry__ as g; g.build()')")
+    lib = ctypes.CDLL(str(p))
+    lib.fct_abi_version.restype = c_int32
+    lib.fct_abi_version.argtypes = []
+    lib.fct_last_error.restype = ctypes.c_char_p
+    lib.fct_last_error.argtypes = []
+    lib.fct_plan_create.restype = c_int32
+    lib.fct_plan_create.argtypes = [POINTER(FctMeshDesc), POINTER(c_void_p)]
+    lib.fct_plan_get_info.restype = c_int32
+    lib.fct_plan_get_info.argtypes = [c_void_p, POINTER(FctPlanInfo)]
+    lib.fct_plan_destroy.restype = c_int32
+    lib.fct_plan_destroy.argtypes = [c_void_p]
+    lib.fct_forward.restype = c_int32
+    lib.fct_forward.argtypes = [c_void_p, POINTER(FctStepArgs), c_void_p]
+    lib.fct_forward_ex.restype = c_int32
+    lib.fct_forward_ex.argtypes = [c_void_p, POINTER(FctStepArgs), c_int32, c_void_p]
+    lib.fct_plan_set_tuning.restype = c_int32
+    lib.fct_plan_set_tuning.argtypes = [c_void_p, c_int32, c_int32]
+    lib.fct_forward_phase.restype = c_int32
+    lib.fct_forward_phase.argtypes = [c_void_p, POINTER(FctStepArgs), c_int32, c_int32,
+                                      POINTER(FctPhaseBufs), c_void_p]
+    ver = lib.fct_abi_version()
+    if ver != FCT_ABI_VERSION:
+        raise RuntimeError(f"libfesom_fct ABI {ver} != expected {FCT_ABI_VERSION}")
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def check(rc: int, what: str) -> None:
+    """Map a C status to the Python exception the reference's conventions imply."""
+    if rc == FCT_OK:
+        return
+    msg = (_LIB.fct_last_error() or b"").decode("utf-8", "replace") if _LIB else ""
+    text = f"{what}: {msg}"
+    if rc == FCT_E_INVALID:
+        raise ValueError(text)
+    if rc == FCT_E_UNSUPPORTED:
+        raise NotImplementedError(text)
+    if rc == FCT_E_NOMEM:
+        raise MemoryError(text)
+    raise RuntimeError(text)

@@ -1,0 +1,450 @@
+// C ABI of libfesom_fct.so: plan construction (topology, CSR lists in the
+// reference's scatter order, geometry, workspace) and the forward entry points.
+// The declarations and their reference counterparts are in include/fesom_fct.h.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fct_common.h"
+
+namespace fct {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+int check_launch(const char* what) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return set_error(FCT_E_HIP, "%s: launch failed: %s", what, hipGetErrorString(e));
+    return FCT_OK;
+}
+
+}  // namespace fct
+
+using namespace fct;
+
+#define HIPCHK(expr)                                                                  \
+    do {                                                                              \
+        const hipError_t e_ = (expr);                                                 \
+        if (e_ != hipSuccess)                                                         \
+            return set_error(FCT_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+struct fct_plan {
+    int device = -1;
+    DevMesh m{};
+    std::vector<void*> allocs;
+    int64_t bytes = 0;
+    // workspace
+    double *vflux = nullptr, *LO = nullptr, *trxy = nullptr, *gnode = nullptr;
+    double *adfh = nullptr, *adfv = nullptr, *tvmax = nullptr, *tvmin = nullptr;
+    double *fplus = nullptr, *fminus = nullptr, *pf = nullptr, *mf = nullptr;
+    // fused path: per-edge wet level range and per-tracer workspace (kw tracers)
+    const int2* ekr = nullptr;
+    int kw = 0;
+    FusedTuning tune;
+    NodeTab tab;
+    double *fLO = nullptr, *fgnode = nullptr, *fadfh = nullptr, *fpf = nullptr, *fmf = nullptr;
+};
+
+namespace {
+
+template <class T>
+int dev_alloc(fct_plan* p, T** out, size_t count) {
+    *out = nullptr;
+    if (count == 0) count = 1;
+    void* ptr = nullptr;
+    const hipError_t e = hipMalloc(&ptr, count * sizeof(T));
+    if (e != hipSuccess)
+        return set_error(FCT_E_NOMEM, "hipMalloc(%zu bytes) failed: %s", count * sizeof(T),
+                         hipGetErrorString(e));
+    p->allocs.push_back(ptr);
+    p->bytes += (int64_t)(count * sizeof(T));
+    *out = static_cast<T*>(ptr);
+    return FCT_OK;
+}
+
+template <class T>
+int dev_upload(fct_plan* p, const T** out, const T* host, size_t count) {
+    T* d = nullptr;
+    int rc = dev_alloc(p, &d, count);
+    if (rc) return rc;
+    if (count) HIPCHK(hipMemcpy(d, host, count * sizeof(T), hipMemcpyHostToDevice));
+    *out = d;
+    return FCT_OK;
+}
+
+void free_plan(fct_plan* p) {
+    if (!p) return;
+    int cur = -1;
+    if (p->device >= 0 && hipGetDevice(&cur) == hipSuccess && cur != p->device)
+        (void)hipSetDevice(p->device);
+    for (void* ptr : p->allocs) (void)hipFree(ptr);
+    if (cur >= 0 && cur != p->device) (void)hipSetDevice(cur);
+    delete p;
+}
+
+int validate_mesh(const fct_mesh_desc* d) {
+    if (!d) return set_error(FCT_E_INVALID, "mesh descriptor is NULL");
+    if (d->nod2D <= 0 || d->elem2D <= 0 || d->edge2D <= 0 || d->nl < 2)
+        return set_error(FCT_E_INVALID, "bad mesh sizes nod2D=%d elem2D=%d edge2D=%d nl=%d",
+                         d->nod2D, d->elem2D, d->edge2D, d->nl);
+    const int64_t big = (int64_t)d->edge2D * d->nl * 2;
+    if ((int64_t)d->elem2D * d->nl * 2 > big || big > (int64_t)INT32_MAX * 2)
+        return set_error(FCT_E_UNSUPPORTED, "mesh too large for this build");
+    const void* ptrs[] = {d->edges, d->edge_tri, d->edge_cross_dxdy, d->edge_dxdy,
+                          d->edge_up_dn_tri, d->elem_nodes, d->gradient_sca, d->elem_area,
+                          d->elem_cos, d->nlevels, d->ulevels, d->nlevels_nod2D,
+                          d->nlevels_nod2D_min, d->ulevels_nod2D, d->ulevels_nod2D_max,
+                          d->area, d->areasvol, d->Z, d->zbar};
+    for (const void* q : ptrs)
+        if (!q) return set_error(FCT_E_INVALID, "mesh descriptor has a NULL array");
+    const int N = d->nod2D, E = d->elem2D, D = d->edge2D, L = d->nl;
+    for (int e = 0; e < D; ++e) {
+        const int a = d->edges[2 * e], b = d->edges[2 * e + 1];
+        if (a < 0 || a >= N || b < 0 || b >= N || a == b)
+            return set_error(FCT_E_INVALID, "edge %d has bad nodes (%d, %d)", e, a, b);
+        const int t1 = d->edge_tri[2 * e], t2 = d->edge_tri[2 * e + 1];
+        if (t1 < -1 || t1 >= E || t2 < -1 || t2 >= E)
+            return set_error(FCT_E_INVALID, "edge %d has bad edge_tri (%d, %d)", e, t1, t2);
+        const int u = d->edge_up_dn_tri[2 * e], w = d->edge_up_dn_tri[2 * e + 1];
+        if (u < -1 || u >= E || w < -1 || w >= E)
+            return set_error(FCT_E_INVALID, "edge %d has bad edge_up_dn_tri (%d, %d)", e, u, w);
+    }
+    for (int e = 0; e < E; ++e) {
+        for (int j = 0; j < 3; ++j) {
+            const int v = d->elem_nodes[3 * e + j];
+            if (v < 0 || v >= N)
+                return set_error(FCT_E_INVALID, "element %d has bad node %d", e, v);
+        }
+        if (d->ulevels[e] < 1 || d->nlevels[e] > L)
+            return set_error(FCT_E_INVALID, "element %d has bad levels (%d, %d)", e,
+                             d->ulevels[e], d->nlevels[e]);
+    }
+    for (int n = 0; n < N; ++n) {
+        if (d->ulevels_nod2D[n] < 1 || d->nlevels_nod2D[n] > L)
+            return set_error(FCT_E_INVALID, "node %d has bad levels (%d, %d)", n,
+                             d->ulevels_nod2D[n], d->nlevels_nod2D[n]);
+    }
+    return FCT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fct_abi_version(void) { return FCT_ABI_VERSION; }
+
+const char* fct_last_error(void) { return g_last_error.c_str(); }
+
+int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
+    if (!out) return set_error(FCT_E_INVALID, "out is NULL");
+    *out = nullptr;
+    int rc = validate_mesh(d);
+    if (rc) return rc;
+    fct_plan* p = new (std::nothrow) fct_plan();
+    if (!p) return set_error(FCT_E_NOMEM, "host allocation failed");
+    if (hipGetDevice(&p->device) != hipSuccess) {
+        delete p;
+        return set_error(FCT_E_HIP, "hipGetDevice failed");
+    }
+    const int N = d->nod2D, E = d->elem2D, D = d->edge2D, L = d->nl;
+    DevMesh& m = p->m;
+    m.N = N; m.E = E; m.D = D; m.L = L;
+
+    // --- edges ---
+    std::vector<int2> h_edges(D), h_et(D), h_updn(D), h_rng(D);
+    std::vector<double4> h_cross(D);
+    std::vector<double2> h_ec(D);
+    for (int e = 0; e < D; ++e) {
+        const int n1 = d->edges[2 * e], n2 = d->edges[2 * e + 1];
+        const int el1 = d->edge_tri[2 * e], el2 = d->edge_tri[2 * e + 1];
+        const int up = d->edge_up_dn_tri[2 * e], dn = d->edge_up_dn_tri[2 * e + 1];
+        h_edges[e] = make_int2(n1, n2);
+        h_et[e] = make_int2(el1, el2);
+        h_updn[e] = make_int2(up >= 0 ? up : 0, dn >= 0 ? dn : 0);
+        // fill_up_dn_grad (tracer_adv.py:351-363): interior edges use the up/dn
+        // gradients on [max(ulevels_nod2D_max)-1, min(nlevels_nod2D_min)-1)
+        if (up >= 0 && dn >= 0) {
+            const int lo = std::max(d->ulevels_nod2D_max[n1], d->ulevels_nod2D_max[n2]) - 1;
+            const int hi = std::min(d->nlevels_nod2D_min[n1], d->nlevels_nod2D_min[n2]) - 1;
+            h_rng[e] = make_int2(lo, hi);
+        } else {
+            h_rng[e] = make_int2(0, 0);
+        }
+        const double* c = d->edge_cross_dxdy + 4 * (size_t)e;
+        h_cross[e] = make_double4(c[0], c[1], c[2], c[3]);
+        // a = R cos(el1), averaged with el2 when present (tracer_adv.py:211-218)
+        double a = R_EARTH * d->elem_cos[el1 >= 0 ? el1 : 0];
+        if (el2 >= 0) a = 0.5 * (a + R_EARTH * d->elem_cos[el2]);
+        h_ec[e] = make_double2(d->edge_dxdy[2 * e] * a, d->edge_dxdy[2 * e + 1] * R_EARTH);
+    }
+    // wet level range of each edge: union of its triangles' layer ranges
+    std::vector<int2> h_ekr(D);
+    for (int e = 0; e < D; ++e) {
+        int lo = L, hi = 0;
+        for (int side = 0; side < 2; ++side) {
+            const int el = d->edge_tri[2 * e + side];
+            if (el < 0) continue;
+            lo = std::min(lo, d->ulevels[el] - 1);
+            hi = std::max(hi, d->nlevels[el] - 1);
+        }
+        h_ekr[e] = lo < hi ? make_int2(lo, hi) : make_int2(0, 0);
+    }
+    // --- elements ---
+    std::vector<int2> h_elev(E);
+    for (int e = 0; e < E; ++e) h_elev[e] = make_int2(d->ulevels[e] - 1, d->nlevels[e] - 1);
+    // --- nodes ---
+    std::vector<int2> h_nlev(N);
+    for (int n = 0; n < N; ++n)
+        h_nlev[n] = make_int2(d->ulevels_nod2D[n] - 1, d->nlevels_nod2D[n] - 1);
+    // node -> (edge, side), increasing flattened index 2*e+side (the order of the
+    // reference's segment sum over stack([f, -f], axis=1), ops.py:56-81)
+    std::vector<int> ecnt(N + 1, 0), h_eoff(N + 1, 0), h_ecsr(2 * (size_t)D);
+    for (int e = 0; e < D; ++e) { ecnt[h_edges[e].x]++; ecnt[h_edges[e].y]++; }
+    for (int n = 0; n < N; ++n) h_eoff[n + 1] = h_eoff[n] + ecnt[n];
+    {
+        std::vector<int> pos(h_eoff.begin(), h_eoff.end() - 1);
+        for (int e = 0; e < D; ++e) {
+            h_ecsr[pos[h_edges[e].x]++] = 2 * e;
+            h_ecsr[pos[h_edges[e].y]++] = 2 * e + 1;
+        }
+    }
+    // node -> elements, increasing element id (the order of the segment sums over
+    // elem_nodes, tracer_adv.py:339-340, 404-405)
+    std::vector<int> ncnt(N, 0), h_noff(N + 1, 0), h_ncsr(3 * (size_t)E);
+    for (int e = 0; e < E; ++e)
+        for (int j = 0; j < 3; ++j) ncnt[d->elem_nodes[3 * e + j]]++;
+    for (int n = 0; n < N; ++n) h_noff[n + 1] = h_noff[n] + ncnt[n];
+    {
+        std::vector<int> pos(h_noff.begin(), h_noff.end() - 1);
+        for (int e = 0; e < E; ++e)
+            for (int j = 0; j < 3; ++j) h_ncsr[pos[d->elem_nodes[3 * e + j]]++] = e;
+    }
+    // per-node stencil table of the fused path: edges in the order above with their
+    // other end, elements in increasing id with their vertices
+    int maxdeg = 0;
+    for (int n = 0; n < N; ++n)
+        maxdeg = std::max(maxdeg, std::max(h_eoff[n + 1] - h_eoff[n], h_noff[n + 1] - h_noff[n]));
+    const int dm = maxdeg <= 6 ? 6 : (maxdeg <= 8 ? 8 : 0);
+    std::vector<int2> h_tcnt, h_tedge;
+    std::vector<int4> h_telem;
+    if (dm) {
+        h_tcnt.resize(N);
+        h_tedge.assign((size_t)N * dm, make_int2(0, 0));
+        h_telem.assign((size_t)N * dm, make_int4(0, 0, 0, 0));
+        for (int n = 0; n < N; ++n) {
+            const int de = h_eoff[n + 1] - h_eoff[n], ne = h_noff[n + 1] - h_noff[n];
+            h_tcnt[n] = make_int2(de, ne);
+            for (int q = 0; q < de; ++q) {
+                const int ent = h_ecsr[h_eoff[n] + q];
+                const int2 nd = h_edges[ent >> 1];
+                h_tedge[(size_t)n * dm + q] = make_int2(ent, (ent & 1) ? nd.x : nd.y);
+            }
+            for (int q = 0; q < ne; ++q) {
+                const int el = h_ncsr[h_noff[n] + q];
+                h_telem[(size_t)n * dm + q] = make_int4(el, d->elem_nodes[3 * el],
+                                                        d->elem_nodes[3 * el + 1],
+                                                        d->elem_nodes[3 * el + 2]);
+            }
+        }
+    }
+    // static QR4C stencil depths (_z_stencil, tracer_adv.py:228-238)
+    std::vector<double> Zp(L), h_Zs(4 * (size_t)L);
+    for (int k = 0; k < L; ++k) Zp[k] = d->Z[k < L - 1 ? k : L - 2];
+    for (int k = 0; k < L; ++k) {
+        h_Zs[k] = Zp[k];
+        h_Zs[L + k] = Zp[k > 0 ? k - 1 : 0];
+        h_Zs[2 * L + k] = k < 2 ? Zp[k] : Zp[k - 2];
+        h_Zs[3 * L + k] = Zp[k + 1 < L ? k + 1 : L - 1];
+    }
+
+    const size_t NL = (size_t)N * L, DL = (size_t)D * L, EL = (size_t)E * L;
+    if ((rc = dev_upload(p, &m.edges, h_edges.data(), D)) ||
+        (rc = dev_upload(p, &m.edge_tri, h_et.data(), D)) ||
+        (rc = dev_upload(p, &m.updn, h_updn.data(), D)) ||
+        (rc = dev_upload(p, &m.utri_rng, h_rng.data(), D)) ||
+        (rc = dev_upload(p, &m.cross, h_cross.data(), D)) ||
+        (rc = dev_upload(p, &m.edge_c, h_ec.data(), D)) ||
+        (rc = dev_upload(p, &m.en, d->elem_nodes, 3 * (size_t)E)) ||
+        (rc = dev_upload(p, &m.gs, d->gradient_sca, 6 * (size_t)E)) ||
+        (rc = dev_upload(p, &m.earea, d->elem_area, (size_t)E)) ||
+        (rc = dev_upload(p, &m.elev, h_elev.data(), E)) ||
+        (rc = dev_upload(p, &m.nlev, h_nlev.data(), N)) ||
+        (rc = dev_upload(p, &m.ecsr_off, h_eoff.data(), (size_t)N + 1)) ||
+        (rc = dev_upload(p, &m.ecsr, h_ecsr.data(), h_ecsr.size())) ||
+        (rc = dev_upload(p, &m.ncsr_off, h_noff.data(), (size_t)N + 1)) ||
+        (rc = dev_upload(p, &m.ncsr, h_ncsr.data(), h_ncsr.size())) ||
+        (rc = dev_upload(p, &m.area, d->area, NL)) ||
+        (rc = dev_upload(p, &m.areasvol, d->areasvol, NL)) ||
+        (rc = dev_upload(p, &m.Zs, h_Zs.data(), h_Zs.size())) ||
+        (rc = dev_upload(p, &m.zbar, d->zbar, (size_t)L)) ||
+        (rc = dev_alloc(p, &p->vflux, DL)) || (rc = dev_alloc(p, &p->LO, NL)) ||
+        (rc = dev_alloc(p, &p->trxy, 2 * EL)) || (rc = dev_alloc(p, &p->gnode, 2 * NL)) ||
+        (rc = dev_alloc(p, &p->adfh, DL)) || (rc = dev_alloc(p, &p->adfv, NL)) ||
+        (rc = dev_alloc(p, &p->tvmax, NL)) || (rc = dev_alloc(p, &p->tvmin, NL)) ||
+        (rc = dev_alloc(p, &p->fplus, NL)) || (rc = dev_alloc(p, &p->fminus, NL)) ||
+        (rc = dev_alloc(p, &p->pf, NL)) || (rc = dev_alloc(p, &p->mf, NL)) ||
+        (rc = dev_upload(p, &p->ekr, h_ekr.data(), D))) {
+        free_plan(p);
+        return rc;
+    }
+    p->kw = std::max(1, std::min((int)FCT_KCHUNK, d->max_tracers > 0 ? d->max_tracers : 2));
+    p->tab.dm = dm;
+    if (dm && ((rc = dev_upload(p, &p->tab.cnt, h_tcnt.data(), h_tcnt.size())) ||
+               (rc = dev_upload(p, &p->tab.edge, h_tedge.data(), h_tedge.size())) ||
+               (rc = dev_upload(p, &p->tab.elem, h_telem.data(), h_telem.size())))) {
+        free_plan(p);
+        return rc;
+    }
+    if ((rc = dev_alloc(p, &p->fLO, NL * p->kw)) || (rc = dev_alloc(p, &p->fgnode, 2 * NL * p->kw)) ||
+        (rc = dev_alloc(p, &p->fadfh, DL * p->kw)) ||
+        (rc = dev_alloc(p, &p->fpf, NL * p->kw)) || (rc = dev_alloc(p, &p->fmf, NL * p->kw))) {
+        free_plan(p);
+        return rc;
+    }
+    // the fused edge passes write only each edge's wet levels: the rest stays zero,
+    // which is the value the node passes must read there
+    if (hipMemset(p->vflux, 0, DL * sizeof(double)) != hipSuccess ||
+        hipMemset(p->fadfh, 0, DL * p->kw * sizeof(double)) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess) {
+        free_plan(p);
+        return set_error(FCT_E_HIP, "workspace initialisation failed");
+    }
+    *out = p;
+    return FCT_OK;
+}
+
+int fct_plan_get_info(const fct_plan* p, fct_plan_info* info) {
+    if (!p || !info) return set_error(FCT_E_INVALID, "NULL argument");
+    info->nod2D = p->m.N; info->elem2D = p->m.E; info->edge2D = p->m.D; info->nl = p->m.L;
+    info->workspace_bytes = p->bytes;
+    info->device = p->device;
+    return FCT_OK;
+}
+
+int fct_plan_set_tuning(fct_plan* p, int32_t knob, int32_t value) {
+    if (!p) return set_error(FCT_E_INVALID, "NULL plan");
+    switch (knob) {
+        case FCT_TUNE_XCD:
+            if (value < 1 || value > 64) return set_error(FCT_E_INVALID, "xcd %d out of range", value);
+            p->tune.xcd = value;
+            return FCT_OK;
+        default:
+            return set_error(FCT_E_INVALID, "unknown tuning knob %d", knob);
+    }
+}
+
+int fct_plan_destroy(fct_plan* p) {
+    free_plan(p);
+    return FCT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int check_args(const fct_plan* p, const fct_step_args* a) {
+    if (!p || !a) return set_error(FCT_E_INVALID, "NULL plan or args");
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != p->device)
+        return set_error(FCT_E_INVALID, "current device %d is not the plan's device %d", cur,
+                         p->device);
+    if (a->ntracers <= 0 || !a->T || !a->T_old || !a->T_new)
+        return set_error(FCT_E_INVALID, "need ntracers > 0 and T, T_old, T_new arrays");
+    if (!a->uv || !a->w_e || !a->helem || !a->hnode || !a->hnode_new)
+        return set_error(FCT_E_INVALID, "NULL flow or thickness field");
+    if ((a->Z3d == nullptr) != (a->zbar3 == nullptr))
+        return set_error(FCT_E_INVALID, "Z3d and zbar3 must be given together");
+    for (int j = 0; j < a->ntracers; ++j) {
+        if (!a->T[j] || !a->T_old[j] || !a->T_new[j])
+            return set_error(FCT_E_INVALID, "tracer %d has a NULL field", j);
+        if ((const double*)a->T_new[j] == a->T[j] || (const double*)a->T_new[j] == a->T_old[j])
+            return set_error(FCT_E_INVALID, "tracer %d: T_new aliases T or T_old", j);
+    }
+    return FCT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+static int forward_phased(fct_plan* p, const fct_step_args* a, hipStream_t s) {
+    int rc;
+    const DevMesh& m = p->m;
+    if ((rc = launch_vflux(m, a, p->vflux, s))) return rc;
+    for (int j = 0; j < a->ntracers; ++j) {
+        if ((rc = launch_lo(m, a, j, p->vflux, p->LO, p->tvmax, p->tvmin, s)) ||
+            (rc = launch_grad(m, a, j, p->trxy, s)) ||
+            (rc = launch_pm(m, a, j, p->vflux, p->LO, p->trxy, p->gnode, p->adfh, p->adfv,
+                            p->tvmax, p->tvmin, p->fplus, p->fminus, s)) ||
+            (rc = launch_final(m, a, j, p->LO, p->adfh, p->adfv, p->tvmax, p->tvmin,
+                               p->fplus, p->fminus, p->pf, p->mf, s)))
+            return rc;
+    }
+    return FCT_OK;
+}
+
+int fct_forward_ex(fct_plan* p, const fct_step_args* a, int32_t path, void* stream) {
+    int rc = check_args(p, a);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const bool fused_ok = a->w_i == nullptr && p->m.L <= 128 && p->tab.dm > 0;
+    if (path == FCT_PATH_FUSED && !fused_ok)
+        return set_error(FCT_E_UNSUPPORTED,
+                         "fused path needs w_i == NULL, nl <= 128 and node degree <= 8");
+    if (path == FCT_PATH_PHASED || (path == FCT_PATH_AUTO && !fused_ok))
+        return forward_phased(p, a, s);
+    if (path != FCT_PATH_AUTO && path != FCT_PATH_FUSED)
+        return set_error(FCT_E_INVALID, "unknown path %d", path);
+    return launch_fused(p->m, p->tab, p->ekr, a, p->kw, p->tune, p->vflux, p->fLO, p->fgnode,
+                        p->fadfh, p->fpf, p->fmf, s);
+}
+
+int fct_forward(fct_plan* p, const fct_step_args* a, void* stream) {
+    return fct_forward_ex(p, a, FCT_PATH_AUTO, stream);
+}
+
+int fct_forward_phase(fct_plan* p, const fct_step_args* a, int32_t j, int32_t phase,
+                      const fct_phase_bufs* b, void* stream) {
+    int rc = check_args(p, a);
+    if (rc) return rc;
+    if (j < 0 || j >= a->ntracers)
+        return set_error(FCT_E_INVALID, "tracer index %d out of range", j);
+    if (!b || !b->LO || !b->tr_xy || !b->fct_plus || !b->fct_minus)
+        return set_error(FCT_E_INVALID, "phase buffers must all be given");
+    hipStream_t s = (hipStream_t)stream;
+    const DevMesh& m = p->m;
+    switch (phase) {
+        case FCT_PHASE_LO:
+            if ((rc = launch_vflux(m, a, p->vflux, s))) return rc;
+            return launch_lo(m, a, j, p->vflux, b->LO, p->tvmax, p->tvmin, s);
+        case FCT_PHASE_GRAD:
+            return launch_grad(m, a, j, b->tr_xy, s);
+        case FCT_PHASE_PM:
+            return launch_pm(m, a, j, p->vflux, b->LO, b->tr_xy, p->gnode, p->adfh, p->adfv,
+                             p->tvmax, p->tvmin, b->fct_plus, b->fct_minus, s);
+        case FCT_PHASE_FINAL:
+            return launch_final(m, a, j, b->LO, p->adfh, p->adfv, p->tvmax, p->tvmin,
+                                b->fct_plus, b->fct_minus, p->pf, p->mf, s);
+        default:
+            return set_error(FCT_E_INVALID, "unknown phase %d", phase);
+    }
+}
+
+}  // extern "C"

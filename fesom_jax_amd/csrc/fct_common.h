@@ -1,0 +1,83 @@
+// Shared device-side definitions of the FCT library: the device mesh view and the
+// small helpers every kernel uses. Constants are those of
+// fesom_jax/tracer_adv.py:44-46, 192-193 and fesom_jax/config.py:41.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fesom_fct.h"
+
+namespace fct {
+
+constexpr double R_EARTH = 6367500.0;
+constexpr double AB_OLD = -(0.5 + 0.1);
+constexpr double AB_NEW = 1.5 + 0.1;
+constexpr double FCT_FLUX_EPS = 1e-16;
+constexpr double FCT_BIG = 1e3;
+constexpr int FCT_KCHUNK = 8;   // tracers per fused launch (kernel-argument pointer slots)
+
+// Device-resident view of the mesh, passed to every kernel by value.
+struct DevMesh {
+    int N, E, D, L;
+    const int2* edges;        // [D] (n1, n2)
+    const int2* edge_tri;     // [D] (el1, el2), -1 = none
+    const int2* updn;         // [D] (up, dn) triangles, clamped to >= 0
+    const int2* utri_rng;     // [D] level range [lo, hi) where the up/dn gradients apply
+    const double4* cross;     // [D] (dx1, dy1, dx2, dy2) metres
+    const double2* edge_c;    // [D] (edx * a, edy * R_EARTH), a = R cos averaged
+    const int* en;            // [E*3]
+    const double* gs;         // [E*6]
+    const double* earea;      // [E]
+    const int2* elev;         // [E] layer range [ulevels-1, nlevels-1)
+    const int2* nlev;         // [N] (nzmin, nzmax) = (ulevels_nod2D-1, nlevels_nod2D-1)
+    const int* ecsr_off;      // [N+1]
+    const int* ecsr;          // 2*edge + side, increasing per node
+    const int* ncsr_off;      // [N+1]
+    const int* ncsr;          // element ids, increasing per node
+    const double* area;       // [N*L]
+    const double* areasvol;   // [N*L]
+    const double* Zs;         // [4*L] static QR4C stencil depths Z[k], Z[k-1], Z[k-2], Z[k+1]
+    const double* zbar;       // [L]
+};
+
+// Per-node stencil table of the fused path (dm slots per node).
+struct NodeTab {
+    int dm = 0;               // slots per node (6 or 8); 0 = mesh not supported
+    const int2* cnt;          // [N] (incident edges, incident elements)
+    const int2* edge;         // [N*dm] (2*edge + side, other node), reference order
+    const int4* elem;         // [N*dm] (element, v0, v1, v2), increasing element id
+};
+
+__device__ __forceinline__ bool in_layer(int2 r, int k) { return k >= r.x && k < r.y; }
+__device__ __forceinline__ bool in_iface(int2 r, int k) { return k >= r.x && k <= r.y; }
+__device__ __forceinline__ double nz_safe(double x) { return x == 0.0 ? 1.0 : x; }
+
+// Host helpers (defined in fct_api.hip).
+int set_error(int code, const char* fmt, ...);
+int check_launch(const char* what);
+
+// Phased launchers (fct_phased.hip).
+int launch_vflux(const DevMesh& m, const fct_step_args* a, double* vflux, hipStream_t s);
+int launch_lo(const DevMesh& m, const fct_step_args* a, int j, const double* vflux,
+              double* LO, double* scr0, double* scr1, hipStream_t s);
+int launch_grad(const DevMesh& m, const fct_step_args* a, int j, double* tr, hipStream_t s);
+int launch_pm(const DevMesh& m, const fct_step_args* a, int j, const double* vflux,
+              const double* LO, const double* tr, double* gnode, double* adfh, double* adfv,
+              double* tvmax, double* tvmin, double* fplus, double* fminus, hipStream_t s);
+int launch_final(const DevMesh& m, const fct_step_args* a, int j, const double* LO,
+                 const double* adfh, const double* adfv, const double* tvmax,
+                 const double* tvmin, const double* fplus, const double* fminus, double* pf,
+                 double* mf, hipStream_t s);
+
+// Launch-shape knobs of the fused path (set through fct_plan_set_tuning).
+struct FusedTuning {
+    int xcd = 8;   // remap blocks so each of `xcd` XCDs sweeps one contiguous tile range
+};
+
+// Fused launcher (fct_fused.hip): kw = tracers per chunk (workspace slots).
+int launch_fused(const DevMesh& m, const NodeTab& tab, const int2* ekr, const fct_step_args* a,
+                 int kw, const FusedTuning& tune, double* vflux, double* LO, double* gnode,
+                 double* adfh, double* pf, double* mf, hipStream_t s);
+
+}  // namespace fct

@@ -1,0 +1,697 @@
+// Fused FP64 FCT path for gfx950: two edge passes and three node-column passes.
+//
+// Mapping: one 64-lane wavefront per task (a node or an edge, times a tracer); lane l of
+// slot s holds level k = 64*s + l of the task's column (NS = 1 for nl <= 64, 2 for
+// nl <= 128), so every load of a node or edge column is one coalesced row segment, and
+// vertical neighbours (k-2 .. k+1) move between lanes with cross-lane shuffles. Task
+// indices are wave-uniform (scalar registers); each node reads its stencil (incident
+// edges in the reference's summation order, incident triangles with their vertices)
+// from a per-node table with a few scalar loads, and then issues the loads of all its
+// neighbours' columns back to back, so a wavefront waits on memory about once per
+// phase instead of once per edge. Only wet lanes are loaded and computed; T_new is
+// written on every lane (dry lanes: T + 0, as the reference's ale_reconstruct does).
+//
+//   kf_vflux  (edge) : vflux on the edge's wet levels (edge_vflux, tracer_adv.py:69-93)
+//   kf_pass1  (node) : LO and the node-averaged gradient gnode
+//                      (compute_fct_lo :298-313, tracer_gradient_elements :316-326,
+//                       _node_avg_grad :329-341)
+//   kf_adfh   (edge) : adf_h = MFCT high order - upwind low order, once per edge-level
+//                      (:96-105, :196-225, fill_up_dn_grad :344-368)
+//   kf_pass2  (node) : cluster bounds, P+/P-, limiter factors (zalesak_limit :389-451)
+//   kf_pass3  (node) : limited fluxes, flux2dtracer_fct, ale_reconstruct
+//                      (:453-466, :469-484, :142-151)
+//
+// Operation order and per-node summation order are those of fct_phased.hip (and of the
+// reference's jnp expressions). Terms the reference adds as exact zeros (edges or
+// elements dry at level k, where the reference multiplies a finite value by 0) are
+// skipped, which leaves every sum bit-identical while the inputs are finite on the
+// lanes the reference multiplies by zero.
+
+#include "fct_common.h"
+
+namespace fct {
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+// value at level k-1 of a column held as NS slots; `at0` is used at k = 0
+template <int NS>
+__device__ __forceinline__ void col_prev(const double (&v)[NS], double (&o)[NS], double at0) {
+    const int l = lane_id();
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const double up = __shfl_up(v[s], 1);
+        const double wrap = (s > 0) ? __shfl(v[s > 0 ? s - 1 : 0], 63) : at0;
+        o[s] = (l == 0) ? wrap : up;
+    }
+}
+
+// value at level k+1; `atend` is used past the last slot lane
+template <int NS>
+__device__ __forceinline__ void col_next(const double (&v)[NS], double (&o)[NS], double atend) {
+    const int l = lane_id();
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const double dn = __shfl_down(v[s], 1);
+        const double wrap = (s + 1 < NS) ? __shfl(v[s + 1 < NS ? s + 1 : s], 0) : atend;
+        o[s] = (l == 63) ? wrap : dn;
+    }
+}
+
+template <int NS>
+__device__ __forceinline__ double col_bcast0(const double (&v)[NS]) { return __shfl(v[0], 0); }
+
+__device__ __forceinline__ double ab2(double told, double t) { return AB_OLD * told + AB_NEW * t; }
+
+// Block -> tile remap so that each XCD works through one contiguous range of tiles
+// (workgroups are dealt to the XCDs round-robin; node and edge order is spatially
+// local, so a contiguous range keeps neighbours in the same L2).
+__device__ __forceinline__ unsigned block_tile(int xcd) {
+    const unsigned b = blockIdx.x, G = gridDim.x;
+    if (xcd <= 1) return b;
+    const unsigned x = b % (unsigned)xcd, i = b / (unsigned)xcd;
+    const unsigned q = G / (unsigned)xcd, r = G % (unsigned)xcd;
+    return x * q + (x < r ? x : r) + i;
+}
+
+struct FusedArgs {
+    DevMesh m;
+    NodeTab tab;
+    double* vflux;         // [D][L]
+    const int2* ekr;       // [D] wet level range of the edge [lo, hi)
+    const double* uv;
+    const double* helem;
+    const double* w_e;
+    const double* hnode;
+    const double* hnn;
+    const double* Z3d;     // optional
+    const double* zbar3;   // optional
+    double dt;
+    int xcd;               // XCD count for the block -> task remap (1 = identity)
+    int K;                 // tracers in this launch (<= FCT_KCHUNK)
+    const double* T[FCT_KCHUNK];
+    const double* Told[FCT_KCHUNK];
+    double* Tnew[FCT_KCHUNK];
+    double* LO;            // [K][N][L]
+    double2* gnode;        // [K][N][L]
+    double* adfh;          // [K][D][L]
+    double* pf;            // [K][N][L]
+    double* mf;            // [K][N][L]
+};
+
+struct Task {
+    int i, j;   // entity (node or edge), tracer
+};
+
+__device__ __forceinline__ bool get_task(int count, int K, int xcd, Task& t) {
+    int w = (int)((block_tile(xcd) * (size_t)blockDim.x + threadIdx.x) >> 6);
+    if (w >= count * K) return false;
+    w = __builtin_amdgcn_readfirstlane(w);
+    t.i = w / K;
+    t.j = w - t.i * K;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// kf_vflux: vflux on [kmin_e, kmax_e); EPW edges per wavefront, loads issued together
+// ---------------------------------------------------------------------------
+constexpr int VFLUX_EPW = 4;
+
+template <int NS>
+__global__ void __launch_bounds__(256)
+kf_vflux(FusedArgs a) {
+    const DevMesh& m = a.m;
+    Task t;
+    const int groups = (m.D + VFLUX_EPW - 1) / VFLUX_EPW;
+    if (!get_task(groups, 1, a.xcd, t)) return;
+    const int L = m.L;
+    int e[VFLUX_EPW];
+    int2 et[VFLUX_EPW], kr[VFLUX_EPW], r1[VFLUX_EPW], r2[VFLUX_EPW];
+    double4 c[VFLUX_EPW];
+    int kmax = 0;
+#pragma unroll
+    for (int q = 0; q < VFLUX_EPW; ++q) {
+        const int eq = t.i * VFLUX_EPW + q;
+        e[q] = eq < m.D ? eq : m.D - 1;
+        et[q] = m.edge_tri[e[q]];
+        kr[q] = eq < m.D ? a.ekr[e[q]] : make_int2(0, 0);
+        c[q] = m.cross[e[q]];
+        r1[q] = et[q].x >= 0 ? m.elev[et[q].x] : make_int2(0, 0);
+        r2[q] = et[q].y >= 0 ? m.elev[et[q].y] : make_int2(0, 0);
+        kmax = kr[q].y > kmax ? kr[q].y : kmax;
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        if (64 * s >= kmax) break;
+        const int k = 64 * s + lane_id();
+        double2 u1[VFLUX_EPW], u2[VFLUX_EPW];
+        double h1[VFLUX_EPW], h2[VFLUX_EPW];
+        if (k < kmax) {
+#pragma unroll
+            for (int q = 0; q < VFLUX_EPW; ++q) {
+                const size_t i1 = (size_t)(et[q].x >= 0 ? et[q].x : 0) * L + k;
+                const size_t i2 = (size_t)(et[q].y >= 0 ? et[q].y : 0) * L + k;
+                u1[q] = reinterpret_cast<const double2*>(a.uv)[i1];
+                u2[q] = reinterpret_cast<const double2*>(a.uv)[i2];
+                h1[q] = a.helem[i1];
+                h2[q] = a.helem[i2];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < VFLUX_EPW; ++q) {
+            if (k < kr[q].x || k >= kr[q].y) continue;
+            double t1 = 0.0, t2 = 0.0;
+            if (et[q].x >= 0 && in_layer(r1[q], k))
+                t1 = (u1[q].x * c[q].y - u1[q].y * c[q].x) * h1[q];
+            if (et[q].y >= 0 && in_layer(r2[q], k))
+                t2 = (u2[q].y * c[q].z - u2[q].x * c[q].w) * h2[q];
+            a.vflux[(size_t)e[q] * L + k] = t1 + t2;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kf_pass1: LO and gnode
+// ---------------------------------------------------------------------------
+template <int NS, int DM>
+__global__ void __launch_bounds__(256)
+kf_pass1(FusedArgs a) {
+    const DevMesh& m = a.m;
+    Task t;
+    if (!get_task(m.N, a.K, a.xcd, t)) return;
+    const int n = t.i, L = m.L;
+    const int2 nr = m.nlev[n];
+    const int2 cnt = a.tab.cnt[n];          // (edges, elements)
+    const double* __restrict__ T = a.T[t.j];
+    const size_t b0 = (size_t)n * L;
+    const size_t jo = (size_t)t.j * m.N * L;
+    const int kneed = nr.y + 1 < L ? nr.y + 1 : L;
+    int2 es[DM];
+    int4 el[DM];
+#pragma unroll
+    for (int q = 0; q < DM; ++q) {
+        es[q] = a.tab.edge[(size_t)n * DM + (q < cnt.x ? q : 0)];
+        el[q] = a.tab.elem[(size_t)n * DM + (q < cnt.y ? q : 0)];
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        if (64 * s > nr.y) break;
+        const int k = 64 * s + lane_id();
+        const bool rd = k < kneed;
+        const bool wet = in_layer(nr, k);
+        // own column, k-1 and k+1 of T for the vertical fluxes
+        const double Tk = rd ? T[b0 + k] : 0.0;
+        const double Tm = (rd && k > 0) ? T[b0 + k - 1] : Tk;
+        const double Tp = (k + 1 < kneed) ? T[b0 + k + 1] : 0.0;
+        // all neighbour columns and edge fluxes at once (one basic block: slot indices
+        // past the node's count repeat slot 0 and are ignored when summing)
+        double To[DM], vf[DM], tv[DM][3];
+        if (wet) {
+#pragma unroll
+            for (int q = 0; q < DM; ++q) {
+                vf[q] = a.vflux[(size_t)(es[q].x >> 1) * L + k];
+                To[q] = T[(size_t)es[q].y * L + k];
+                tv[q][0] = T[(size_t)el[q].y * L + k];
+                tv[q][1] = T[(size_t)el[q].z * L + k];
+                tv[q][2] = T[(size_t)el[q].w * L + k];
+            }
+        }
+        if (!wet) continue;
+        // LO: edges in reference order
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < DM; ++q) {
+            if (q >= cnt.x) break;
+            const double v = vf[q];
+            if (v == 0.0) continue;
+            const bool side = es[q].x & 1;
+            const double T1 = side ? To[q] : Tk;
+            const double T2 = side ? Tk : To[q];
+            const double av = fabs(v);
+            const double f = -0.5 * (T1 * (v + av) + T2 * (v - av));
+            acc = acc + (side ? -f : f);
+        }
+        const size_t i = b0 + k;
+        double fv0, fv1 = 0.0;
+        {
+            const double wk = a.w_e[i], aw = fabs(wk);
+            fv0 = (-0.5 * (Tk * (wk + aw) + Tm * (wk - aw))) * m.area[i];
+        }
+        if (in_layer(nr, k + 1)) {
+            const double wk = a.w_e[i + 1], aw = fabs(wk);
+            fv1 = (-0.5 * (Tp * (wk + aw) + Tk * (wk - aw))) * m.area[i + 1];
+        }
+        const double av = m.areasvol[i], hn = a.hnn[i];
+        const double safe_a = av > 0.0 ? av : 1.0, safe_hn = hn > 0.0 ? hn : 1.0;
+        const double numer = Tk * a.hnode[i] + (acc + (fv0 - fv1)) * a.dt / safe_a;
+        const double lo = numer / safe_hn;
+        a.LO[jo + i] = (hn > 0.0 && av > 0.0) ? lo : 0.0;
+        // gnode: elements in increasing id
+        double nx = 0.0, ny = 0.0, den = 0.0;
+#pragma unroll
+        for (int q = 0; q < DM; ++q) {
+            if (q >= cnt.y) break;
+            const int e = el[q].x;
+            if (!in_layer(m.elev[e], k)) continue;
+            const double* gs = m.gs + 6 * (size_t)e;
+            const double gx = gs[0] * tv[q][0] + gs[1] * tv[q][1] + gs[2] * tv[q][2];
+            const double gy = gs[3] * tv[q][0] + gs[4] * tv[q][1] + gs[5] * tv[q][2];
+            const double ea = m.earea[e];
+            nx = nx + gx * ea;
+            ny = ny + gy * ea;
+            den = den + ea;
+        }
+        const double d = den > 0.0 ? den : 1.0;
+        a.gnode[jo + i] = make_double2(nx / d, ny / d);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kf_adfh: adf_h(e, k) once per edge-level
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 grad_of(const double* gs, double t0, double t1, double t2) {
+    double2 g;
+    g.x = gs[0] * t0 + gs[1] * t1 + gs[2] * t2;
+    g.y = gs[3] * t0 + gs[4] * t1 + gs[5] * t2;
+    return g;
+}
+
+constexpr int ADFH_EPW = 2;
+
+template <int NS>
+__global__ void __launch_bounds__(256)
+kf_adfh(FusedArgs a) {
+    const DevMesh& m = a.m;
+    Task t;
+    const int groups = (m.D + ADFH_EPW - 1) / ADFH_EPW;
+    if (!get_task(groups, a.K, a.xcd, t)) return;
+    const int L = m.L;
+    const double* __restrict__ T = a.T[t.j];
+    const double* __restrict__ To = a.Told[t.j];
+    const double2* __restrict__ gn = a.gnode + (size_t)t.j * m.N * L;
+    double* __restrict__ out = a.adfh + (size_t)t.j * m.D * L;
+    int e[ADFH_EPW], u[ADFH_EPW][3], d[ADFH_EPW][3];
+    int2 kr[ADFH_EPW], nd[ADFH_EPW], ur[ADFH_EPW], ud[ADFH_EPW];
+    double2 cc[ADFH_EPW];
+    int kmax = 0;
+#pragma unroll
+    for (int q = 0; q < ADFH_EPW; ++q) {
+        const int eq = t.i * ADFH_EPW + q;
+        e[q] = eq < m.D ? eq : m.D - 1;
+        kr[q] = eq < m.D ? a.ekr[e[q]] : make_int2(0, 0);
+        nd[q] = m.edges[e[q]];
+        ur[q] = m.utri_rng[e[q]];
+        ud[q] = m.updn[e[q]];
+        cc[q] = m.edge_c[e[q]];
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+            u[q][v] = m.en[3 * ud[q].x + v];
+            d[q][v] = m.en[3 * ud[q].y + v];
+        }
+        kmax = kr[q].y > kmax ? kr[q].y : kmax;
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        if (64 * s >= kmax) break;
+        const int k = 64 * s + lane_id();
+        double vf[ADFH_EPW], T1[ADFH_EPW], T2[ADFH_EPW], O1[ADFH_EPW], O2[ADFH_EPW];
+        double tu[ADFH_EPW][3], td[ADFH_EPW][3];
+        double2 ga[ADFH_EPW], gb[ADFH_EPW];
+        if (k < kmax) {
+#pragma unroll
+            for (int q = 0; q < ADFH_EPW; ++q) {
+                const size_t i1 = (size_t)nd[q].x * L + k, i2 = (size_t)nd[q].y * L + k;
+                vf[q] = a.vflux[(size_t)e[q] * L + k];
+                T1[q] = T[i1]; T2[q] = T[i2]; O1[q] = To[i1]; O2[q] = To[i2];
+                ga[q] = gn[i1]; gb[q] = gn[i2];
+#pragma unroll
+                for (int v = 0; v < 3; ++v) {
+                    tu[q][v] = T[(size_t)u[q][v] * L + k];
+                    td[q][v] = T[(size_t)d[q][v] * L + k];
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < ADFH_EPW; ++q) {
+            if (k < kr[q].x || k >= kr[q].y) continue;
+            const bool tri = k >= ur[q].x && k < ur[q].y;
+            const double A1 = ab2(O1[q], T1[q]), A2 = ab2(O2[q], T2[q]);
+            double g0, g1, g2, g3;
+            if (tri) {
+                const double* gsu = m.gs + 6 * (size_t)ud[q].x;
+                const double* gsd = m.gs + 6 * (size_t)ud[q].y;
+                const double2 gu = grad_of(gsu, tu[q][0], tu[q][1], tu[q][2]);
+                const double2 gd = grad_of(gsd, td[q][0], td[q][1], td[q][2]);
+                g0 = gu.x; g1 = gd.x; g2 = gu.y; g3 = gd.y;
+            } else {
+                g0 = ga[q].x; g1 = gb[q].x; g2 = ga[q].y; g3 = gb[q].y;
+            }
+            double r = 0.0;
+            const double v = vf[q];
+            if (v != 0.0) {
+                const double av = fabs(v);
+                const double flo = -0.5 * (T1[q] * (v + av) + T2[q] * (v - av));
+                const double diff = A2 - A1;
+                const double Tm1 = A1 + (2.0 * diff + cc[q].x * g0 + cc[q].y * g2) / 6.0;
+                const double Tm2 = A2 - (2.0 * diff + cc[q].x * g1 + cc[q].y * g3) / 6.0;
+                const double ho = -0.5 * ((v + av) * Tm1 + (v - av) * Tm2);
+                r = ho - flo;
+            }
+            out[(size_t)e[q] * L + k] = r;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// vertical antidiffusive flux (adv_flux_ver_ho - adv_flux_ver, tracer_adv.py:241-295,
+// 108-120) at interface k from the column's ttfAB at k-2..k+1 and T at k-1, k
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double adfv_at(const DevMesh& m, int k, int2 nr, size_t i, size_t b0,
+                                          double A2, double A1, double A, double Ap, double Tm,
+                                          double Tk, double W, double ar, const double* Z3d,
+                                          const double* zbar3) {
+    const int L = m.L;
+    double hi = 0.0;
+    if (in_iface(nr, k)) {
+        if (k == nr.x) {
+            hi = -A * W * ar;
+        } else if (k == nr.y) {
+            hi = 0.0;
+        } else if (k == nr.x + 1 || k == nr.y - 1) {
+            hi = -0.5 * (A1 + A) * W * ar;
+        } else if (k >= nr.x + 2 && k <= nr.y - 2) {
+            double Znz, Zm1, Zm2, Zp1, zb;
+            if (Z3d) {
+                const int km1 = k > 0 ? k - 1 : 0, km2 = k > 1 ? k - 2 : 0;
+                const int kp1 = k + 1 < L ? k + 1 : L - 1;
+                Znz = Z3d[i]; Zm1 = Z3d[b0 + km1]; Zm2 = Z3d[b0 + km2]; Zp1 = Z3d[b0 + kp1];
+                zb = zbar3[i];
+            } else {
+                Znz = m.Zs[k]; Zm1 = m.Zs[L + k]; Zm2 = m.Zs[2 * L + k]; Zp1 = m.Zs[3 * L + k];
+                zb = m.zbar[k];
+            }
+            const double qc = (A1 - A) / nz_safe(Zm1 - Znz);
+            const double qu = (A - Ap) / nz_safe(Znz - Zp1);
+            const double qd = (A2 - A1) / nz_safe(Zm2 - Zm1);
+            const double Tmean1 = A + (2.0 * qc + qu) * (zb - Znz) / 3.0;
+            const double Tmean2 = A1 + (2.0 * qc + qd) * (zb - Zm1) / 3.0;
+            hi = -0.5 * (Tmean1 + Tmean2) * W * ar;
+        }
+    }
+    double lo = 0.0;
+    if (in_layer(nr, k)) {
+        const double aw = fabs(W);
+        lo = (-0.5 * (Tk * (W + aw) + Tm * (W - aw))) * ar;
+    }
+    return hi - lo;
+}
+
+// Column of adf_v for the node: every lane k <= nzmax (0 elsewhere).
+template <int NS>
+__device__ __forceinline__ void adfv_column(const FusedArgs& a, int n, int2 nr,
+                                            const double (&Tc)[NS], const double (&Ac)[NS],
+                                            double (&out)[NS]) {
+    const DevMesh& m = a.m;
+    const int L = m.L;
+    const size_t b0 = (size_t)n * L;
+    double Tm[NS], A1[NS], A2[NS], Ap[NS];
+    col_prev<NS>(Tc, Tm, col_bcast0<NS>(Tc));
+    col_prev<NS>(Ac, A1, col_bcast0<NS>(Ac));
+    col_prev<NS>(A1, A2, col_bcast0<NS>(Ac));
+    col_next<NS>(Ac, Ap, 0.0);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int k = 64 * s + lane_id();
+        out[s] = 0.0;
+        if (k <= nr.y && k < L) {
+            const size_t i = b0 + k;
+            out[s] = adfv_at(m, k, nr, i, b0, A2[s], A1[s], Ac[s], Ap[s], Tm[s], Tc[s],
+                             a.w_e[i], m.area[i], a.Z3d, a.zbar3);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kf_pass2: cluster bounds, P+/P-, limiter factors
+// ---------------------------------------------------------------------------
+template <int NS, int DM>
+__global__ void __launch_bounds__(256)
+kf_pass2(FusedArgs a) {
+    const DevMesh& m = a.m;
+    Task t;
+    if (!get_task(m.N, a.K, a.xcd, t)) return;
+    const int n = t.i, L = m.L;
+    const int2 nr = m.nlev[n];
+    const int2 cnt = a.tab.cnt[n];
+    const double* __restrict__ T = a.T[t.j];
+    const double* __restrict__ To = a.Told[t.j];
+    const size_t jo = (size_t)t.j * m.N * L;
+    const double* __restrict__ LO = a.LO + jo;
+    const double* __restrict__ adfh = a.adfh + (size_t)t.j * m.D * L;
+    const size_t b0 = (size_t)n * L;
+    const int kneed = nr.y + 1 < L ? nr.y + 1 : L;
+    int2 es[DM];
+    int4 el[DM];
+#pragma unroll
+    for (int q = 0; q < DM; ++q) {
+        es[q] = a.tab.edge[(size_t)n * DM + (q < cnt.x ? q : 0)];
+        el[q] = a.tab.elem[(size_t)n * DM + (q < cnt.y ? q : 0)];
+    }
+    double Tc[NS], Ac[NS], LOc[NS], tvx[NS], tvn[NS], ap[NS], am[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        Tc[s] = 0.0; Ac[s] = 0.0; LOc[s] = 0.0;
+        tvx[s] = -INFINITY; tvn[s] = INFINITY; ap[s] = 0.0; am[s] = 0.0;
+        if (64 * s >= kneed) continue;
+        const int k = 64 * s + lane_id();
+        const bool rd = k < kneed;
+        const bool wet = in_layer(nr, k);
+        if (rd) {
+            Tc[s] = T[b0 + k];
+            Ac[s] = ab2(To[b0 + k], Tc[s]);
+            if (wet) LOc[s] = LO[b0 + k];
+        }
+        // a1-a3 and b1 inputs: the triangles' vertex columns of LO and T and the
+        // antidiffusive fluxes of the incident edges, loaded in one basic block
+        double lv[DM][3], tw[DM][3], fh[DM];
+        if (rd) {
+#pragma unroll
+            for (int q = 0; q < DM; ++q) {
+                const size_t j0 = (size_t)el[q].y * L + k, j1 = (size_t)el[q].z * L + k,
+                             j2 = (size_t)el[q].w * L + k;
+                lv[q][0] = LO[j0]; lv[q][1] = LO[j1]; lv[q][2] = LO[j2];
+                tw[q][0] = T[j0]; tw[q][1] = T[j1]; tw[q][2] = T[j2];
+                fh[q] = adfh[(size_t)(es[q].x >> 1) * L + k];
+            }
+        }
+        if (rd) {
+#pragma unroll
+            for (int q = 0; q < DM; ++q) {
+                if (q >= cnt.y) break;
+                double amax = -FCT_BIG, amin = FCT_BIG;
+                if (in_layer(m.elev[el[q].x], k)) {
+                    amax = fmax(fmax(fmax(lv[q][0], tw[q][0]), fmax(lv[q][1], tw[q][1])),
+                                fmax(lv[q][2], tw[q][2]));
+                    amin = fmin(fmin(fmin(lv[q][0], tw[q][0]), fmin(lv[q][1], tw[q][1])),
+                                fmin(lv[q][2], tw[q][2]));
+                }
+                tvx[s] = fmax(tvx[s], amax);
+                tvn[s] = fmin(tvn[s], amin);
+            }
+        }
+        if (wet) {
+#pragma unroll
+            for (int q = 0; q < DM; ++q) {
+                if (q >= cnt.x) break;
+                const double f = (es[q].x & 1) ? -fh[q] : fh[q];
+                ap[s] = ap[s] + fmax(f, 0.0);
+                am[s] = am[s] + fmin(f, 0.0);
+            }
+        }
+    }
+    double tvxm[NS], tvxp[NS], tvnm[NS], tvnp[NS];
+    col_prev<NS>(tvx, tvxm, col_bcast0<NS>(tvx));
+    col_next<NS>(tvx, tvxp, 0.0);
+    col_prev<NS>(tvn, tvnm, col_bcast0<NS>(tvn));
+    col_next<NS>(tvn, tvnp, 0.0);
+    double fvc[NS], fvb[NS];
+    adfv_column<NS>(a, n, nr, Tc, Ac, fvc);
+    col_next<NS>(fvc, fvb, 0.0);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int k = 64 * s + lane_id();
+        if (!in_layer(nr, k)) continue;
+        const size_t i = b0 + k;
+        const bool use_own = (k == nr.x) || (k == nr.y - 1);
+        const double cmax = use_own ? tvx[s] : fmax(fmax(tvxm[s], tvx[s]), tvxp[s]);
+        const double cmin = use_own ? tvn[s] : fmin(fmin(tvnm[s], tvn[s]), tvnp[s]);
+        const double ttf_max = cmax - LOc[s];
+        const double ttf_min = cmin - LOc[s];
+        const double pos_v = fmax(fvc[s], 0.0) + fmax(-fvb[s], 0.0);
+        const double neg_v = fmin(fvc[s], 0.0) + fmin(-fvb[s], 0.0);
+        const double fplus = pos_v + ap[s];
+        const double fminus = neg_v + am[s];
+        const double av = m.areasvol[i], hn = a.hnn[i];
+        const double safe_a = av > 0.0 ? av : 1.0, safe_hn = hn > 0.0 ? hn : 1.0;
+        const double flux_pos = fplus * a.dt / safe_a / safe_hn + FCT_FLUX_EPS;
+        const double flux_neg = fminus * a.dt / safe_a / safe_hn - FCT_FLUX_EPS;
+        const bool valid = av > 0.0 && hn > 0.0;
+        a.pf[jo + i] = valid ? fmin(1.0, ttf_max / flux_pos) : 1.0;
+        a.mf[jo + i] = valid ? fmin(1.0, ttf_min / flux_neg) : 1.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kf_pass3: limited fluxes, divergence, reconstruction
+// ---------------------------------------------------------------------------
+template <int NS, int DM>
+__global__ void __launch_bounds__(256)
+kf_pass3(FusedArgs a) {
+    const DevMesh& m = a.m;
+    Task t;
+    if (!get_task(m.N, a.K, a.xcd, t)) return;
+    const int n = t.i, L = m.L;
+    const int2 nr = m.nlev[n];
+    const int2 cnt = a.tab.cnt[n];
+    const double* __restrict__ T = a.T[t.j];
+    const double* __restrict__ To = a.Told[t.j];
+    const size_t jo = (size_t)t.j * m.N * L;
+    const double* __restrict__ pf = a.pf + jo;
+    const double* __restrict__ mf = a.mf + jo;
+    const double* __restrict__ adfh = a.adfh + (size_t)t.j * m.D * L;
+    const size_t b0 = (size_t)n * L;
+    const int kneed = nr.y + 1 < L ? nr.y + 1 : L;
+    int2 es[DM];
+#pragma unroll
+    for (int q = 0; q < DM; ++q) es[q] = a.tab.edge[(size_t)n * DM + (q < cnt.x ? q : 0)];
+    double Tc[NS], Ac[NS], Pc[NS], Mc[NS], acc[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        Tc[s] = 0.0; Ac[s] = 0.0; Pc[s] = 1.0; Mc[s] = 1.0; acc[s] = 0.0;
+        if (64 * s >= kneed) continue;
+        const int k = 64 * s + lane_id();
+        const bool wet = in_layer(nr, k);
+        if (k < kneed) {
+            Tc[s] = T[b0 + k];
+            Ac[s] = ab2(To[b0 + k], Tc[s]);
+        }
+        double fh[DM], po[DM], mo[DM];
+        if (wet) {
+#pragma unroll
+            for (int q = 0; q < DM; ++q) {
+                fh[q] = adfh[(size_t)(es[q].x >> 1) * L + k];
+                const size_t io = (size_t)es[q].y * L + k;
+                po[q] = pf[io];
+                mo[q] = mf[io];
+            }
+        }
+        if (wet) {
+            Pc[s] = pf[b0 + k];
+            Mc[s] = mf[b0 + k];
+#pragma unroll
+            for (int q = 0; q < DM; ++q) {
+                if (q >= cnt.x) break;
+                const double f = fh[q];
+                if (f == 0.0) continue;
+                if (es[q].x & 1) {
+                    const double ae = f >= 0.0 ? fmin(po[q], Mc[s]) : fmin(mo[q], Pc[s]);
+                    acc[s] = acc[s] + (-(ae * f));
+                } else {
+                    const double ae = f >= 0.0 ? fmin(Pc[s], mo[q]) : fmin(Mc[s], po[q]);
+                    acc[s] = acc[s] + ae * f;
+                }
+            }
+        }
+    }
+    double fvc[NS];
+    adfv_column<NS>(a, n, nr, Tc, Ac, fvc);
+    double Pm[NS], Mm[NS];
+    col_prev<NS>(Pc, Pm, col_bcast0<NS>(Pc));
+    col_prev<NS>(Mc, Mm, col_bcast0<NS>(Mc));
+    double vl[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int k = 64 * s + lane_id();
+        vl[s] = 0.0;
+        if (in_iface(nr, k) && k < L) {
+            const double f = fvc[s];
+            double ae;
+            if (k == nr.x) ae = f >= 0.0 ? Pc[s] : Mc[s];
+            else ae = f >= 0.0 ? fmin(Mm[s], Pc[s]) : fmin(Pm[s], Mc[s]);
+            vl[s] = ae * f;
+        }
+    }
+    double vl1[NS];
+    col_next<NS>(vl, vl1, 0.0);
+    const double* __restrict__ LO = a.LO + jo;
+    double* __restrict__ Tn = a.Tnew[t.j];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int k = 64 * s + lane_id();
+        if (k >= L) continue;
+        const size_t i = b0 + k;
+        if (!in_layer(nr, k)) {
+            const double Ti = (k < kneed) ? Tc[s] : T[i];
+            Tn[i] = Ti + 0.0;
+            continue;
+        }
+        const double av = m.areasvol[i], hn = a.hnn[i], hnd = a.hnode[i];
+        const double safe_a = av > 0.0 ? av : 1.0, safe_hn = hn > 0.0 ? hn : 1.0;
+        const double Ti = Tc[s];
+        const double lo_trans = -Ti * hnd + LO[i] * hn;
+        const double vdiv = av > 0.0 ? (vl[s] - vl1[s]) * a.dt / safe_a : 0.0;
+        const double dttf_v = lo_trans + vdiv;
+        const double dttf_h = acc[s] * a.dt / safe_a;
+        double del = dttf_h + dttf_v;
+        del = del + Ti * (hnd - hn);
+        const double dT = hn > 0.0 ? del / safe_hn : 0.0;
+        Tn[i] = Ti + dT;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host launcher
+// ---------------------------------------------------------------------------
+static inline unsigned blocks_for(int64_t waves) { return (unsigned)((waves + 3) / 4); }
+
+template <int NS, int DM>
+static int launch_fused_t(FusedArgs& fa, const fct_step_args* a, int kw, hipStream_t s) {
+    const DevMesh& m = fa.m;
+    const int64_t vg = (m.D + VFLUX_EPW - 1) / VFLUX_EPW;
+    if (m.D) hipLaunchKernelGGL((kf_vflux<NS>), dim3(blocks_for(vg)), dim3(256), 0, s, fa);
+    int rc = check_launch("kf_vflux");
+    if (rc) return rc;
+    for (int j0 = 0; j0 < a->ntracers; j0 += kw) {
+        const int K = (a->ntracers - j0) < kw ? (a->ntracers - j0) : kw;
+        fa.K = K;
+        for (int j = 0; j < FCT_KCHUNK; ++j) {
+            const int jj = j0 + (j < K ? j : 0);
+            fa.T[j] = a->T[jj]; fa.Told[j] = a->T_old[jj]; fa.Tnew[j] = a->T_new[jj];
+        }
+        const unsigned gn = blocks_for((int64_t)m.N * K);
+        const unsigned gd = blocks_for((int64_t)((m.D + ADFH_EPW - 1) / ADFH_EPW) * K);
+        hipLaunchKernelGGL((kf_pass1<NS, DM>), dim3(gn), dim3(256), 0, s, fa);
+        hipLaunchKernelGGL((kf_adfh<NS>), dim3(gd), dim3(256), 0, s, fa);
+        hipLaunchKernelGGL((kf_pass2<NS, DM>), dim3(gn), dim3(256), 0, s, fa);
+        hipLaunchKernelGGL((kf_pass3<NS, DM>), dim3(gn), dim3(256), 0, s, fa);
+        if ((rc = check_launch("fused passes"))) return rc;
+    }
+    return FCT_OK;
+}
+
+int launch_fused(const DevMesh& m, const NodeTab& tab, const int2* ekr, const fct_step_args* a,
+                 int kw, const FusedTuning& tune, double* vflux, double* LO, double* gnode,
+                 double* adfh, double* pf, double* mf, hipStream_t s) {
+    FusedArgs fa;
+    fa.m = m; fa.tab = tab; fa.vflux = vflux; fa.ekr = ekr; fa.uv = a->uv; fa.helem = a->helem;
+    fa.w_e = a->w_e; fa.hnode = a->hnode; fa.hnn = a->hnode_new; fa.Z3d = a->Z3d;
+    fa.zbar3 = a->zbar3; fa.dt = a->dt; fa.xcd = tune.xcd; fa.K = 0;
+    fa.LO = LO; fa.gnode = (double2*)gnode; fa.adfh = adfh; fa.pf = pf; fa.mf = mf;
+    if (kw > FCT_KCHUNK) kw = FCT_KCHUNK;
+    if (tab.dm == 6 && m.L <= 64) return launch_fused_t<1, 6>(fa, a, kw, s);
+    if (tab.dm == 6 && m.L <= 128) return launch_fused_t<2, 6>(fa, a, kw, s);
+    if (tab.dm == 8 && m.L <= 64) return launch_fused_t<1, 8>(fa, a, kw, s);
+    if (tab.dm == 8 && m.L <= 128) return launch_fused_t<2, 8>(fa, a, kw, s);
+    return set_error(FCT_E_UNSUPPORTED, "fused path supports node degree <= 8 and nl <= 128");
+}
+
+}  // namespace fct

@@ -1,0 +1,282 @@
+"""Drop-in FCT tracer advection on the GPU (the reference's
+fesom_jax/tracer_adv.py:532-578 ``advect_one_fct``).
+
+``advect_one_fct`` has the reference's name, arguments, defaults and return value:
+``(T_new, T_old_new)`` with ``T_old_new`` the very ``T`` passed in
+(fesom_jax/tracer_adv.py:59-66, 549). It accepts torch tensors on a ROCm device,
+objects exporting DLPack (e.g. JAX arrays on the GPU, exchanged zero-copy) and numpy
+arrays (copied to the current device and the result copied back). Every field must be
+float64 with the reference's shapes: node fields ``[nod2D, nl]``, ``uv``
+``[elem2D, nl, 2]``, ``helem`` ``[elem2D, nl]``.
+
+The work runs on torch's current HIP stream of the tensors' device, through the C ABI
+of libfesom_fct.so. ``exch`` (the reference's halo-exchange hook) is honoured: it is
+called on the low-order solution, the element gradients and the limiter sums, in the
+reference's order (tracer_adv.py:559, 562, 438-439), between phases of the kernel
+pipeline; ``exch=None`` runs the single-call path.
+
+There is no CPU fallback: without a ROCm GPU or the built library the call raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import weakref
+from ctypes import c_void_p
+
+import numpy as np
+
+from . import _lib
+from .config import DT_DEFAULT
+
+__all__ = ["advect_one_fct", "advect_fct", "FctPlan", "get_plan", "DT_DEFAULT"]
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("fesom_jax_amd.advect_one_fct needs a ROCm GPU; none is visible")
+    return torch
+
+
+# --------------------------------------------------------------------------
+# plan (device topology) cache
+# --------------------------------------------------------------------------
+_MESH_INT = ("edges", "edge_tri", "edge_up_dn_tri", "elem_nodes", "nlevels", "ulevels",
+             "nlevels_nod2D", "nlevels_nod2D_min", "ulevels_nod2D", "ulevels_nod2D_max",
+             "nod_in_elem2D_offsets", "nod_in_elem2D")
+_MESH_F64 = ("edge_cross_dxdy", "edge_dxdy", "gradient_sca", "elem_area", "elem_cos",
+             "area", "areasvol", "Z", "zbar")
+
+
+class FctPlan:
+    """Device-resident topology + workspace for one mesh on one device."""
+
+    def __init__(self, mesh, device=None, max_tracers: int = 4):
+        torch = _torch()
+        self.lib = _lib.load()
+        dev = torch.device("cuda", torch.cuda.current_device() if device is None
+                           else torch.device(device).index or 0)
+        self.device = dev
+        host = {}
+        for k in _MESH_INT:
+            host[k] = np.ascontiguousarray(np.asarray(getattr(mesh, k)), dtype=np.int32)
+        for k in _MESH_F64:
+            host[k] = np.ascontiguousarray(np.asarray(getattr(mesh, k)), dtype=np.float64)
+        self.nod2D = int(mesh.nod2D)
+        self.elem2D = int(mesh.elem2D)
+        self.edge2D = int(mesh.edge2D)
+        self.nl = int(mesh.nl)
+        desc = _lib.FctMeshDesc(nod2D=self.nod2D, elem2D=self.elem2D, edge2D=self.edge2D,
+                                nl=self.nl, max_tracers=int(max_tracers))
+        for k, a in host.items():
+            setattr(desc, k, a.ctypes.data)
+        ptr = c_void_p()
+        with torch.cuda.device(dev):
+            _lib.check(self.lib.fct_plan_create(ctypes.byref(desc), ctypes.byref(ptr)),
+                       "fct_plan_create")
+        self.ptr = ptr
+        self._finalizer = weakref.finalize(self, self.lib.fct_plan_destroy, ptr)
+        info = _lib.FctPlanInfo()
+        _lib.check(self.lib.fct_plan_get_info(ptr, ctypes.byref(info)), "fct_plan_get_info")
+        self.workspace_bytes = int(info.workspace_bytes)
+
+    def close(self):
+        self._finalizer()
+
+
+_PLANS: dict = {}
+
+
+def get_plan(mesh, device) -> FctPlan:
+    """The cached plan of ``mesh`` on ``device`` (keyed by object identity; dropped
+    when the mesh object is garbage collected)."""
+    key = (id(mesh), str(device))
+    plan = _PLANS.get(key)
+    if plan is None:
+        plan = FctPlan(mesh, device)
+        _PLANS[key] = plan
+        try:
+            weakref.finalize(mesh, _PLANS.pop, key, None)
+        except TypeError:
+            pass
+    return plan
+
+
+# --------------------------------------------------------------------------
+# array interchange
+# --------------------------------------------------------------------------
+class _Kind:
+    TORCH, NUMPY, DLPACK = "torch", "numpy", "dlpack"
+
+
+def _kind_of(x):
+    torch = _torch()
+    if isinstance(x, torch.Tensor):
+        return _Kind.TORCH
+    if isinstance(x, np.ndarray):
+        return _Kind.NUMPY
+    if hasattr(x, "__dlpack__"):
+        return _Kind.DLPACK
+    return _Kind.NUMPY
+
+
+def _to_device(x, name, device, shape):
+    torch = _torch()
+    if isinstance(x, torch.Tensor):
+        t = x
+    elif isinstance(x, np.ndarray) or not hasattr(x, "__dlpack__"):
+        t = torch.from_numpy(np.ascontiguousarray(np.asarray(x), dtype=np.float64))
+    else:
+        t = torch.from_dlpack(x)
+    if t.dtype != torch.float64:
+        raise ValueError(f"{name}: dtype {t.dtype} is not float64 (the FCT path is FP64)")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: shape {tuple(t.shape)} != expected {tuple(shape)}")
+    if t.device != device:
+        if t.device.type == "cpu":
+            t = t.to(device)
+        else:
+            raise ValueError(f"{name}: on {t.device}, expected {device}")
+    return t.contiguous()
+
+
+def _from_device(t, kind, like):
+    if kind == _Kind.TORCH:
+        return t
+    if kind == _Kind.NUMPY:
+        return t.cpu().numpy()
+    mod = type(like).__module__.split(".")[0]
+    if mod in ("jax", "jaxlib"):
+        import jax.dlpack  # type: ignore
+        return jax.dlpack.from_dlpack(t)
+    return t
+
+
+def _pick_device(*xs):
+    torch = _torch()
+    for x in xs:
+        if isinstance(x, torch.Tensor) and x.is_cuda:
+            return x.device
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _ptr_array(ts):
+    arr = (c_void_p * len(ts))()
+    for i, t in enumerate(ts):
+        arr[i] = t.data_ptr()
+    return arr
+
+
+def _run(mesh, uv, w_e, helem, hnode, hnode_new, Ts, T_olds, dt, exch, Z3d, zbar3, w_i,
+         path=_lib.FCT_PATH_AUTO):
+    torch = _torch()
+    device = _pick_device(T_olds[0], Ts[0], uv, w_e)
+    N, E, L = int(mesh.nod2D), int(mesh.elem2D), int(mesh.nl)
+    nshape, eshape = (N, L), (E, L)
+    uv_t = _to_device(uv, "uv", device, (E, L, 2))
+    w_t = _to_device(w_e, "w_e", device, nshape)
+    he_t = _to_device(helem, "helem", device, eshape)
+    hn_t = _to_device(hnode, "hnode", device, nshape)
+    hnn_t = _to_device(hnode_new, "hnode_new", device, nshape)
+    T_t = [_to_device(T, f"T[{j}]", device, nshape) for j, T in enumerate(Ts)]
+    To_t = [_to_device(T, f"T_old[{j}]", device, nshape) for j, T in enumerate(T_olds)]
+    wi_t = None if w_i is None else _to_device(w_i, "w_i", device, nshape)
+    if (Z3d is None) != (zbar3 is None):
+        raise ValueError("Z3d and zbar3 must be given together")
+    z3_t = None if Z3d is None else _to_device(Z3d, "Z3d", device, nshape)
+    zb3_t = None if zbar3 is None else _to_device(zbar3, "zbar3", device, nshape)
+    out = [torch.empty(nshape, dtype=torch.float64, device=device) for _ in Ts]
+
+    plan = get_plan(mesh, device)
+    Tp, Top, Tnp = _ptr_array(T_t), _ptr_array(To_t), _ptr_array(out)
+    args = _lib.FctStepArgs(
+        ntracers=len(Ts), dt=float(dt), uv=uv_t.data_ptr(), w_e=w_t.data_ptr(),
+        helem=he_t.data_ptr(), hnode=hn_t.data_ptr(), hnode_new=hnn_t.data_ptr(),
+        T=ctypes.cast(Tp, ctypes.POINTER(c_void_p)),
+        T_old=ctypes.cast(Top, ctypes.POINTER(c_void_p)),
+        T_new=ctypes.cast(Tnp, ctypes.POINTER(c_void_p)),
+        w_i=None if wi_t is None else wi_t.data_ptr(),
+        Z3d=None if z3_t is None else z3_t.data_ptr(),
+        zbar3=None if zb3_t is None else zb3_t.data_ptr())
+    with torch.cuda.device(device):
+        stream = torch.cuda.current_stream(device).cuda_stream
+        if exch is None:
+            _lib.check(plan.lib.fct_forward_ex(plan.ptr, ctypes.byref(args), int(path),
+                                               c_void_p(stream)), "fct_forward")
+        else:
+            for j in range(len(Ts)):
+                _run_phased(plan, args, j, exch, device, N, E, L, stream)
+    return out
+
+
+def _exch_field(exch, t, kind, shape, device, name):
+    """Apply the caller's exch to an intermediate and bring the result back to a
+    contiguous float64 device tensor of the same shape."""
+    r = exch(t, kind)
+    if r is t:
+        return t
+    return _to_device(r, name, device, shape)
+
+
+def _run_phased(plan, args, j, exch, device, N, E, L, stream):
+    torch = _torch()
+    f64 = dict(dtype=torch.float64, device=device)
+    LO = torch.empty((N, L), **f64)
+    tr = torch.empty((E, L, 2), **f64)
+    fp = torch.empty((N, L), **f64)
+    fm = torch.empty((N, L), **f64)
+    s = c_void_p(stream)
+
+    def bufs():
+        return _lib.FctPhaseBufs(LO=LO.data_ptr(), tr_xy=tr.data_ptr(),
+                                 fct_plus=fp.data_ptr(), fct_minus=fm.data_ptr())
+
+    b = bufs()
+    _lib.check(plan.lib.fct_forward_phase(plan.ptr, ctypes.byref(args), j, _lib.FCT_PHASE_LO,
+                                          ctypes.byref(b), s), "phase LO")
+    LO = _exch_field(exch, LO, "nod", (N, L), device, "exch(LO)")
+    b = bufs()
+    _lib.check(plan.lib.fct_forward_phase(plan.ptr, ctypes.byref(args), j,
+                                          _lib.FCT_PHASE_GRAD, ctypes.byref(b), s), "phase GRAD")
+    tr = _exch_field(exch, tr, "elem", (E, L, 2), device, "exch(tr_xy)")
+    b = bufs()
+    _lib.check(plan.lib.fct_forward_phase(plan.ptr, ctypes.byref(args), j, _lib.FCT_PHASE_PM,
+                                          ctypes.byref(b), s), "phase PM")
+    fp = _exch_field(exch, fp, "nod", (N, L), device, "exch(fct_plus)")
+    fm = _exch_field(exch, fm, "nod", (N, L), device, "exch(fct_minus)")
+    b = bufs()
+    _lib.check(plan.lib.fct_forward_phase(plan.ptr, ctypes.byref(args), j,
+                                          _lib.FCT_PHASE_FINAL, ctypes.byref(b), s),
+               "phase FINAL")
+
+
+# --------------------------------------------------------------------------
+# public entry points
+# --------------------------------------------------------------------------
+def advect_one_fct(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, *,
+                   dt: float = DT_DEFAULT, exch=None, Z3d=None, zbar3=None, w_i=None):
+    """One tracer's FCT advection + ALE reconstruction; returns ``(T_new, T)``
+    (fesom_jax/tracer_adv.py:532-578)."""
+    kind = _kind_of(T)
+    out = _run(mesh, uv, w_e, helem, hnode, hnode_new, [T], [T_old], dt, exch, Z3d, zbar3,
+               w_i)
+    return _from_device(out[0], kind, T), T
+
+
+def advect_fct(mesh, uv, w_e, helem, hnode, hnode_new, Ts, T_olds, *,
+               dt: float = DT_DEFAULT, exch=None, Z3d=None, zbar3=None, w_i=None,
+               path: str = "auto"):
+    """K tracers sharing one flow in one call (the two calls of fesom_jax/step.py:409-416
+    fused): returns ``[(T_new_k, T_k)]``. ``path`` selects the kernel path ("auto",
+    "fused", "phased"; all give the same bits). Extension with no reference
+    counterpart."""
+    if len(Ts) != len(T_olds) or not Ts:
+        raise ValueError("Ts and T_olds must be non-empty and of equal length")
+    codes = {"auto": _lib.FCT_PATH_AUTO, "fused": _lib.FCT_PATH_FUSED,
+             "phased": _lib.FCT_PATH_PHASED}
+    if path not in codes:
+        raise ValueError(f"path must be one of {sorted(codes)}")
+    out = _run(mesh, uv, w_e, helem, hnode, hnode_new, list(Ts), list(T_olds), dt, exch, Z3d,
+               zbar3, w_i, codes[path])
+    return [(_from_device(o, _kind_of(T), T), T) for o, T in zip(out, Ts)]

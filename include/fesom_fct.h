@@ -1,0 +1,165 @@
+/* C ABI of the MI355X FP64 FCT tracer-advection library (libfesom_fct.so).
+ *
+ * Drop-in for the reference's FCT entry point
+ *   fesom_jax/tracer_adv.py:532-578  advect_one_fct(mesh, uv, w_e, helem, hnode,
+ *       hnode_new, T, T_old, *, dt, exch, Z3d, zbar3, w_i) -> (T_new, T_old_new)
+ * The reference is pure Python/JAX and has no native FFI; the entry points below are
+ * what a ctypes/DLPack binding of that function calls (INTEGRATION.md shows the
+ * binding). Plain C types only: host pointers for the mesh, device pointers for
+ * fields, a hipStream_t passed as void*.
+ *
+ * Layout: every 3-D field is row-major [n_entity, nl] float64 (a node's column is
+ * contiguous), uv is [elem2D, nl, 2]; indices are 0-based int32 with -1 sentinels
+ * (fesom_jax/mesh.py:13-20).
+ *
+ * Errors: every call returns 0 on success or a negative FCT_E* code; the message of
+ * the last failure on the calling thread is returned by fct_last_error().
+ * Threading: calls are stream-ordered on the stream passed in. A plan owns one
+ * workspace, so two calls on one plan must not overlap in time (serialise them on
+ * one stream, or create one plan per stream).
+ */
+#ifndef FESOM_FCT_H
+#define FESOM_FCT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FCT_ABI_VERSION 1
+
+#define FCT_OK 0
+#define FCT_E_INVALID (-1)      /* bad argument: null pointer, size, range         */
+#define FCT_E_HIP (-2)          /* a HIP runtime call failed                       */
+#define FCT_E_NOMEM (-3)        /* device or host allocation failed                */
+#define FCT_E_UNSUPPORTED (-4)  /* option the library does not implement           */
+
+typedef struct fct_plan fct_plan;
+
+/* Host-side mesh description: the fields of the reference's Mesh that the FCT reads
+ * (fesom_jax/mesh.py:62-129). All pointers are host memory, read only during
+ * fct_plan_create. */
+typedef struct fct_mesh_desc {
+    int32_t nod2D, elem2D, edge2D, nl;
+    const int32_t* edges;              /* [edge2D][2] */
+    const int32_t* edge_tri;           /* [edge2D][2], -1 = boundary */
+    const double*  edge_cross_dxdy;    /* [edge2D][4] metres */
+    const double*  edge_dxdy;          /* [edge2D][2] radians */
+    const int32_t* edge_up_dn_tri;     /* [edge2D][2], -1 = absent */
+    const int32_t* elem_nodes;         /* [elem2D][3] */
+    const double*  gradient_sca;       /* [elem2D][6] */
+    const double*  elem_area;          /* [elem2D] */
+    const double*  elem_cos;           /* [elem2D] */
+    const int32_t* nlevels;            /* [elem2D] */
+    const int32_t* ulevels;            /* [elem2D] */
+    const int32_t* nlevels_nod2D;      /* [nod2D] */
+    const int32_t* nlevels_nod2D_min;  /* [nod2D] */
+    const int32_t* ulevels_nod2D;      /* [nod2D] */
+    const int32_t* ulevels_nod2D_max;  /* [nod2D] */
+    const int32_t* nod_in_elem2D_offsets; /* [nod2D+1] */
+    const int32_t* nod_in_elem2D;      /* [offsets[nod2D]], element-ascending per node */
+    const double*  area;               /* [nod2D][nl] */
+    const double*  areasvol;           /* [nod2D][nl] */
+    const double*  Z;                  /* [nl-1] mid-layer depths */
+    const double*  zbar;               /* [nl] interface depths */
+    int32_t max_tracers;               /* tracers per fused launch the workspace holds
+                                          (1..8; 0 = 2); more tracers run in chunks */
+} fct_mesh_desc;
+
+/* Per-call arguments. Field pointers are device memory of the plan's device.
+ * T, T_old, T_new are host arrays of ntracers device pointers. w_i, Z3d, zbar3 may be
+ * NULL (the reference's None). T_new[k] must not alias T[k] or T_old[k]. */
+typedef struct fct_step_args {
+    int32_t ntracers;
+    double dt;
+    const double* uv;          /* [elem2D][nl][2] */
+    const double* w_e;         /* [nod2D][nl] */
+    const double* helem;       /* [elem2D][nl] */
+    const double* hnode;       /* [nod2D][nl] */
+    const double* hnode_new;   /* [nod2D][nl] */
+    const double* const* T;    /* ntracers x [nod2D][nl] */
+    const double* const* T_old;
+    double* const* T_new;
+    const double* w_i;         /* optional [nod2D][nl] */
+    const double* Z3d;         /* optional [nod2D][nl] (with zbar3) */
+    const double* zbar3;       /* optional [nod2D][nl] (with Z3d) */
+} fct_step_args;
+
+/* Caller-owned intermediates for the phased path (one tracer). Used when the
+ * reference's `exch` hook must see them: LO after phase 1 (tracer_adv.py:559),
+ * tr_xy after phase 2 (:562), fct_plus / fct_minus after phase 3 (:438-439). */
+typedef struct fct_phase_bufs {
+    double* LO;          /* [nod2D][nl] */
+    double* tr_xy;       /* [elem2D][nl][2] */
+    double* fct_plus;    /* [nod2D][nl] */
+    double* fct_minus;   /* [nod2D][nl] */
+} fct_phase_bufs;
+
+enum {
+    FCT_PHASE_LO = 1,     /* vflux + low-order solution LO (+ implicit w_i solve)    */
+    FCT_PHASE_GRAD = 2,   /* element gradients tr_xy                                 */
+    FCT_PHASE_PM = 3,     /* node averages, antidiffusive fluxes, bounds, P+/P-      */
+    FCT_PHASE_FINAL = 4   /* limiter factors, limited fluxes, T_new                  */
+};
+
+enum {
+    FCT_PATH_AUTO = 0,    /* fused unless w_i is given or nl > 128                  */
+    FCT_PATH_PHASED = 1,  /* one kernel per reference stage                         */
+    FCT_PATH_FUSED = 2    /* edge pass + three node-column passes                   */
+};
+
+enum {
+    FCT_TUNE_XCD = 1      /* blocks remapped so each of N XCDs sweeps one contiguous
+                             range of nodes (1 = hardware order); default 8           */
+};
+
+typedef struct fct_plan_info {
+    int32_t nod2D, elem2D, edge2D, nl;
+    int64_t workspace_bytes;   /* device bytes owned by the plan */
+    int32_t device;            /* HIP device ordinal the plan lives on */
+} fct_plan_info;
+
+/* Library ABI version (FCT_ABI_VERSION). No reference counterpart. */
+int fct_abi_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* fct_last_error(void);
+
+/* Build the device-resident plan for a mesh on the current HIP device: topology,
+ * CSR node->(edge,side) lists in reference summation order, geometry and workspace.
+ * Replaces the per-call mesh handling of fesom_jax/tracer_adv.py:532-578 (the Mesh
+ * pytree, fesom_jax/mesh.py:62-129). Allocates device memory. */
+int fct_plan_create(const fct_mesh_desc* mesh, fct_plan** out);
+
+int fct_plan_get_info(const fct_plan* plan, fct_plan_info* info);
+
+int fct_plan_destroy(fct_plan* plan);
+
+/* Set a launch-shape knob (FCT_TUNE_*) of the fused path; results do not depend on
+ * it. No reference counterpart (XLA chooses its own launch shapes). */
+int fct_plan_set_tuning(fct_plan* plan, int32_t knob, int32_t value);
+
+/* One FCT step for args->ntracers tracers sharing the flow inputs:
+ * T_new[k] = advect_one_fct(mesh, uv, w_e, helem, hnode, hnode_new, T[k], T_old[k],
+ *                           dt=dt, exch=None, Z3d, zbar3, w_i)[0]
+ * (fesom_jax/tracer_adv.py:532-578). T_old_new is T itself and is not written.
+ * No allocation; stream-ordered. */
+int fct_forward(fct_plan* plan, const fct_step_args* args, void* stream);
+
+/* fct_forward with an explicit kernel path (FCT_PATH_*); both paths give the same
+ * bits. No reference counterpart (the reference has one XLA graph). */
+int fct_forward_ex(fct_plan* plan, const fct_step_args* args, int32_t path, void* stream);
+
+/* One phase of the step for tracer `tracer` of args (phased path with caller-owned
+ * intermediates, so the caller can apply the reference's exch(field, kind) between
+ * phases, fesom_jax/tracer_adv.py:559, 562, 438-439). Phases must run in order
+ * 1..4 for a tracer; phase 4 writes args->T_new[tracer]. */
+int fct_forward_phase(fct_plan* plan, const fct_step_args* args, int32_t tracer,
+                      int32_t phase, const fct_phase_bufs* bufs, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FESOM_FCT_H */

@@ -1,0 +1,231 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the same
+seeded inputs. Contract: max relative error <= 1e-12 on every lane (FP64); the
+kernels keep the oracle's operation and summation order, so the tests also record
+how many lanes differ at all (expected: none)."""
+
+import numpy as np
+import pytest
+
+from conftest import rel_err, sharp_case
+from oracle import fct_numpy as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no ROCm GPU visible")
+    return torch.device("cuda", 0)
+
+
+def _gpu_step(mesh, case, dev, *, j=None, path="auto", **kw):
+    import torch
+    from fesom_jax_amd.tracer_adv import advect_fct
+    Ts = case["T"] if j is None else [case["T"][j]]
+    Tos = case["T_old"] if j is None else [case["T_old"][j]]
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    out = advect_fct(mesh, t(case["uv"]), t(case["w_e"]), t(case["helem"]), t(case["hnode"]),
+                     t(case["hnode_new"]), [t(x) for x in Ts], [t(x) for x in Tos],
+                     dt=case["dt"], path=path,
+                     **{k: (None if v is None else t(v)) if k != "exch" else v
+                        for k, v in kw.items()})
+    torch.cuda.synchronize()
+    return [o[0].cpu().numpy() for o in out]
+
+
+def _oracle(mesh, case, j, **kw):
+    return O.advect_one_fct(mesh, case["uv"], case["w_e"], case["helem"], case["hnode"],
+                            case["hnode_new"], case["T"][j], case["T_old"][j], dt=case["dt"],
+                            **kw)[0]
+
+
+def _check(name, gpu, ref, mesh):
+    err = rel_err(gpu, ref)
+    ndiff = int(np.sum(gpu != ref))
+    print(f"[parity] {name}: rel_err={err:.3e} differing_lanes={ndiff}/{gpu.size}", flush=True)
+    assert np.all(np.isfinite(gpu[mesh.node_layer_mask]))
+    assert err <= TOL, f"{name}: relative error {err:.3e} > {TOL}"
+    return ndiff
+
+
+def test_pi_two_tracers(pi_mesh, pi_case, torch_dev):
+    outs = _gpu_step(pi_mesh, pi_case, torch_dev)
+    for j, g in enumerate(outs):
+        _check(f"pi tracer {j}", g, _oracle(pi_mesh, pi_case, j), pi_mesh)
+
+
+def test_pi_limiter_active(pi_mesh, pi_case, torch_dev):
+    case = sharp_case(pi_mesh, pi_case, 2.0)
+    ref, _, parts = O.advect_one_fct(pi_mesh, case["uv"], case["w_e"], case["helem"],
+                                     case["hnode"], case["hnode_new"], case["T"][0],
+                                     case["T_old"][0], dt=case["dt"], return_parts=True)
+    m = pi_mesh.node_layer_mask
+    assert (parts["plus_fac"][m] < 1).mean() > 0.02, "limiter not active enough"
+    g = _gpu_step(pi_mesh, case, torch_dev)[0]
+    _check("pi limiter-active", g, ref, pi_mesh)
+
+
+def test_pi_second_step_and_ale(pi_mesh, pi_case, torch_dev):
+    rng = np.random.default_rng(5)
+    m = pi_mesh.node_layer_mask
+    T0 = pi_case["T"][0]
+    T1 = O.advect_one_fct(pi_mesh, pi_case["uv"], pi_case["w_e"], pi_case["helem"],
+                          pi_case["hnode"], pi_case["hnode_new"], T0, T0, dt=2700.0)[0]
+    hnn = pi_case["hnode"] * np.where(m, 1.0 + 0.01 * rng.standard_normal(m.shape), 1.0)
+    case = dict(pi_case, T=[T1], T_old=[T0], hnode_new=hnn)
+    g = _gpu_step(pi_mesh, case, torch_dev)[0]
+    _check("pi step 2 + hnode_new != hnode", g, _oracle(pi_mesh, case, 0), pi_mesh)
+
+
+def test_pi_wsplit_and_zstar(pi_mesh, pi_case, torch_dev):
+    rng = np.random.default_rng(6)
+    m = pi_mesh.node_iface_mask
+    w_i = np.where(m, 0.3 * pi_case["w_e"] * rng.random(m.shape), 0.0)
+    g = _gpu_step(pi_mesh, pi_case, torch_dev, j=0, w_i=w_i)[0]
+    _check("pi w_i", g, _oracle(pi_mesh, pi_case, 0, w_i=w_i), pi_mesh)
+    zb3 = pi_mesh.zbar_3d_n * (1.0 + 0.001 * rng.random(pi_mesh.zbar_3d_n.shape))
+    Z3d = np.concatenate([0.5 * (zb3[:, :-1] + zb3[:, 1:]), zb3[:, -1:]], axis=1)
+    g = _gpu_step(pi_mesh, pi_case, torch_dev, j=0, Z3d=Z3d, zbar3=zb3)[0]
+    _check("pi Z3d/zbar3", g, _oracle(pi_mesh, pi_case, 0, Z3d=Z3d, zbar3=zb3), pi_mesh)
+
+
+def test_icosa_three_tracers(icosa4_mesh, icosa4_case, torch_dev):
+    outs = _gpu_step(icosa4_mesh, icosa4_case, torch_dev)
+    for j, g in enumerate(outs):
+        _check(f"icosa-4 tracer {j}", g, _oracle(icosa4_mesh, icosa4_case, j), icosa4_mesh)
+
+
+def test_exch_phased_equals_fused(pi_mesh, pi_case, torch_dev):
+    import torch
+    seen = []
+
+    def exch_copy(f, kind):
+        seen.append((kind, tuple(f.shape)))
+        return f.clone()
+
+    fused = _gpu_step(pi_mesh, pi_case, torch_dev, j=0)[0]
+    phased = _gpu_step(pi_mesh, pi_case, torch_dev, j=0, exch=exch_copy)[0]
+    N, E, L = pi_mesh.nod2D, pi_mesh.elem2D, pi_mesh.nl
+    assert seen == [("nod", (N, L)), ("elem", (E, L, 2)), ("nod", (N, L)), ("nod", (N, L))]
+    assert np.array_equal(fused, phased)
+
+
+def test_exch_hook_sees_reference_fields(pi_mesh, pi_case, torch_dev):
+    """The intermediates handed to exch equal the oracle's LO, tr_xy, fct_plus,
+    fct_minus; an exch that modifies LO changes the result as in the oracle."""
+    got = {}
+
+    def grab(f, kind):
+        got.setdefault(kind, []).append(f.detach().cpu().numpy().copy())
+        return f
+
+    _gpu_step(pi_mesh, pi_case, torch_dev, j=0, exch=grab)
+    _, _, parts = O.advect_one_fct(pi_mesh, pi_case["uv"], pi_case["w_e"], pi_case["helem"],
+                                   pi_case["hnode"], pi_case["hnode_new"], pi_case["T"][0],
+                                   pi_case["T_old"][0], dt=2700.0, return_parts=True)
+    assert rel_err(got["nod"][0], parts["LO"]) <= TOL
+    assert rel_err(got["elem"][0], parts["tr_xy"]) <= TOL
+    assert rel_err(got["nod"][1], parts["fct_plus"]) <= TOL
+    assert rel_err(got["nod"][2], parts["fct_minus"]) <= TOL
+
+    gstate = {"n": 0}
+
+    def bump_once(f, kind):
+        gstate["n"] += 1
+        return f * 1.001 if gstate["n"] == 1 else f
+
+    g = _gpu_step(pi_mesh, pi_case, torch_dev, j=0, exch=bump_once)[0]
+    state = {"n": 0}
+
+    def np_bump(f, kind):
+        state["n"] += 1
+        return f * 1.001 if state["n"] == 1 else f
+
+    ref = _oracle(pi_mesh, pi_case, 0, exch=np_bump)
+    _check("pi exch(LO)*1.001", g, ref, pi_mesh)
+
+
+def test_run_twice_bitwise(icosa4_mesh, icosa4_case, torch_dev):
+    a = _gpu_step(icosa4_mesh, icosa4_case, torch_dev)
+    b = _gpu_step(icosa4_mesh, icosa4_case, torch_dev)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_dry_lanes_copy_input(pi_mesh, pi_case, torch_dev):
+    m = pi_mesh.node_layer_mask
+    T = np.where(m, pi_case["T"][0], 7.25)
+    case = dict(pi_case, T=[T], T_old=[T])
+    g = _gpu_step(pi_mesh, case, torch_dev)[0]
+    assert np.array_equal(g[~m], T[~m])
+    _check("pi nonzero dry lanes", g, _oracle(pi_mesh, case, 0), pi_mesh)
+
+
+def test_numpy_in_numpy_out_and_T_old_identity(pi_mesh, pi_case, torch_dev):
+    from fesom_jax_amd.tracer_adv import advect_one_fct
+    c = pi_case
+    T = c["T"][0]
+    Tn, To = advect_one_fct(pi_mesh, c["uv"], c["w_e"], c["helem"], c["hnode"],
+                            c["hnode_new"], T, c["T_old"][0], dt=c["dt"])
+    assert isinstance(Tn, np.ndarray) and To is T
+    _check("pi numpy boundary", Tn, _oracle(pi_mesh, c, 0), pi_mesh)
+
+
+def test_bad_inputs_raise(pi_mesh, pi_case, torch_dev):
+    import torch
+    from fesom_jax_amd.tracer_adv import advect_one_fct
+    c = pi_case
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(torch_dev)
+    with pytest.raises(ValueError):
+        advect_one_fct(pi_mesh, t(c["uv"]), t(c["w_e"]), t(c["helem"]), t(c["hnode"]),
+                       t(c["hnode_new"]), t(c["T"][0]).float(), t(c["T_old"][0]))
+    with pytest.raises(ValueError):
+        advect_one_fct(pi_mesh, t(c["uv"][:, :-1]), t(c["w_e"]), t(c["helem"]), t(c["hnode"]),
+                       t(c["hnode_new"]), t(c["T"][0]), t(c["T_old"][0]))
+
+
+@pytest.mark.parametrize("which", ["pi", "icosa4", "sharp"])
+def test_fused_equals_phased_bitwise(which, pi_mesh, pi_case, icosa4_mesh, icosa4_case,
+                                     torch_dev):
+    mesh, case = {"pi": (pi_mesh, pi_case), "icosa4": (icosa4_mesh, icosa4_case),
+                  "sharp": (pi_mesh, sharp_case(pi_mesh, pi_case, 2.0))}[which]
+    fused = _gpu_step(mesh, case, torch_dev, path="fused")
+    phased = _gpu_step(mesh, case, torch_dev, path="phased")
+    for j, (f, p) in enumerate(zip(fused, phased)):
+        ref = _oracle(mesh, case, j)
+        _check(f"{which} fused tracer {j}", f, ref, mesh)
+        assert np.array_equal(f, p), f"{which} tracer {j}: fused and phased differ"
+
+
+def test_fused_zstar_and_many_tracers(icosa4_mesh, icosa4_case, torch_dev):
+    """Z3d/zbar3 on the fused path, and 6 tracers (more than one workspace chunk)."""
+    rng = np.random.default_rng(9)
+    m = icosa4_mesh
+    zb3 = m.zbar_3d_n * (1.0 + 0.001 * rng.random(m.zbar_3d_n.shape))
+    Z3d = np.concatenate([0.5 * (zb3[:, :-1] + zb3[:, 1:]), zb3[:, -1:]], axis=1)
+    g = _gpu_step(m, icosa4_case, torch_dev, j=0, path="fused", Z3d=Z3d, zbar3=zb3)[0]
+    _check("icosa4 fused Z3d", g, _oracle(m, icosa4_case, 0, Z3d=Z3d, zbar3=zb3), m)
+    Ts = [icosa4_case["T"][j % 3] * (1.0 + 0.01 * j) for j in range(6)]
+    case = dict(icosa4_case, T=Ts, T_old=[x * 0.999 for x in Ts])
+    outs = _gpu_step(m, case, torch_dev, path="fused")
+    for j, g in enumerate(outs):
+        _check(f"icosa4 fused 6 tracers [{j}]", g, _oracle(m, case, j), m)
+
+
+def test_icosa_nl70_two_slots(torch_dev):
+    """nl = 70 needs two 64-lane slots per column on the fused path."""
+    from fesom_jax_amd.inputs import make_case
+    from fesom_jax_amd.meshgen import icosahedral_mesh
+    m = icosahedral_mesh(4, 70, seed=21)
+    case = make_case(m, seed=22, dt=2700.0)
+    assert int(m.nlevels_nod2D.max()) > 65
+    outs = _gpu_step(m, case, torch_dev, path="fused")
+    for j, g in enumerate(outs):
+        _check(f"icosa4 nl70 fused [{j}]", g, _oracle(m, case, j), m)
+    sh = sharp_case(m, case, 2.0)
+    g = _gpu_step(m, sh, torch_dev, path="fused")[0]
+    _check("icosa4 nl70 fused sharp", g, _oracle(m, sh, 0), m)
