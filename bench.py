@@ -86,7 +86,7 @@ def main():
 
     import torch
     from fesom_jax_amd import _lib
-    from fesom_jax_amd.inputs import make_case
+    from fesom_jax_amd.inputs import make_case_device
     from fesom_jax_amd.meshgen import icosahedral_mesh
     from fesom_jax_amd.perfmodel import HBM_PEAK_GBS, algorithmic_bytes, updates_per_step
     from fesom_jax_amd.tracer_adv import get_plan
@@ -102,14 +102,14 @@ def main():
         f"nl={mesh.nl} ({time.time() - t0:.1f}s)")
     t0 = time.time()
     kinds = ("T", "S", "T", "S")[:K]
-    case = make_case(mesh, seed=args.seed + 1, dt=args.dt, tracers=kinds)
+    case = make_case_device(mesh, dev, seed=args.seed + 1, dt=args.dt, tracers=kinds)
+    torch.cuda.synchronize()
     log(f"inputs ({time.time() - t0:.1f}s) cfl={case['cfl']} vel_scale={case['vel_scale']:.4f}")
 
-    up = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-    uv, w_e, helem = up(case["uv"]), up(case["w_e"]), up(case["helem"])
-    hnode, hnn = up(case["hnode"]), up(case["hnode_new"])
+    uv, w_e, helem = case["uv"], case["w_e"], case["helem"]
+    hnode, hnn = case["hnode"], case["hnode_new"]
     # three rotating buffers per tracer: (T_old, T, T_new)
-    bufs = [[up(case["T_old"][j]), up(case["T"][j]),
+    bufs = [[case["T_old"][j], case["T"][j],
              torch.empty((mesh.nod2D, mesh.nl), dtype=torch.float64, device=dev)]
             for j in range(K)]
     del case

@@ -204,7 +204,7 @@ kf_pass1(FusedArgs a) {
         const double Tp = (k + 1 < kneed) ? T[b0 + k + 1] : 0.0;
         // all neighbour columns and edge fluxes at once (one basic block: slot indices
         // past the node's count repeat slot 0 and are ignored when summing)
-        double To[DM], vf[DM], tv[DM][3];
+        double To[DM] = {}, vf[DM] = {}, tv[DM][3] = {};
         if (wet) {
 #pragma unroll
             for (int q = 0; q < DM; ++q) {
@@ -216,19 +216,19 @@ kf_pass1(FusedArgs a) {
             }
         }
         if (!wet) continue;
-        // LO: edges in reference order
+        // LO: edges in reference order. Skipped terms add +0.0, which leaves a sum that
+        // starts at +0.0 unchanged, so selects replace branches bit for bit.
         double acc = 0.0;
 #pragma unroll
         for (int q = 0; q < DM; ++q) {
-            if (q >= cnt.x) break;
             const double v = vf[q];
-            if (v == 0.0) continue;
             const bool side = es[q].x & 1;
             const double T1 = side ? To[q] : Tk;
             const double T2 = side ? Tk : To[q];
             const double av = fabs(v);
             const double f = -0.5 * (T1 * (v + av) + T2 * (v - av));
-            acc = acc + (side ? -f : f);
+            const bool use = q < cnt.x && v != 0.0;
+            acc = acc + (use ? (side ? -f : f) : 0.0);
         }
         const size_t i = b0 + k;
         double fv0, fv1 = 0.0;
@@ -249,16 +249,15 @@ kf_pass1(FusedArgs a) {
         double nx = 0.0, ny = 0.0, den = 0.0;
 #pragma unroll
         for (int q = 0; q < DM; ++q) {
-            if (q >= cnt.y) break;
             const int e = el[q].x;
-            if (!in_layer(m.elev[e], k)) continue;
+            const bool use = q < cnt.y && in_layer(m.elev[e], k);
             const double* gs = m.gs + 6 * (size_t)e;
             const double gx = gs[0] * tv[q][0] + gs[1] * tv[q][1] + gs[2] * tv[q][2];
             const double gy = gs[3] * tv[q][0] + gs[4] * tv[q][1] + gs[5] * tv[q][2];
             const double ea = m.earea[e];
-            nx = nx + gx * ea;
-            ny = ny + gy * ea;
-            den = den + ea;
+            nx = nx + (use ? gx * ea : 0.0);
+            ny = ny + (use ? gy * ea : 0.0);
+            den = den + (use ? ea : 0.0);
         }
         const double d = den > 0.0 ? den : 1.0;
         a.gnode[jo + i] = make_double2(nx / d, ny / d);
@@ -313,9 +312,12 @@ kf_adfh(FusedArgs a) {
     for (int s = 0; s < NS; ++s) {
         if (64 * s >= kmax) break;
         const int k = 64 * s + lane_id();
-        double vf[ADFH_EPW], T1[ADFH_EPW], T2[ADFH_EPW], O1[ADFH_EPW], O2[ADFH_EPW];
-        double tu[ADFH_EPW][3], td[ADFH_EPW][3];
+        double vf[ADFH_EPW] = {}, T1[ADFH_EPW] = {}, T2[ADFH_EPW] = {}, O1[ADFH_EPW] = {},
+               O2[ADFH_EPW] = {};
+        double tu[ADFH_EPW][3] = {}, td[ADFH_EPW][3] = {};
         double2 ga[ADFH_EPW], gb[ADFH_EPW];
+#pragma unroll
+        for (int q = 0; q < ADFH_EPW; ++q) { ga[q] = make_double2(0.0, 0.0); gb[q] = ga[q]; }
         if (k < kmax) {
 #pragma unroll
             for (int q = 0; q < ADFH_EPW; ++q) {
@@ -335,27 +337,20 @@ kf_adfh(FusedArgs a) {
             if (k < kr[q].x || k >= kr[q].y) continue;
             const bool tri = k >= ur[q].x && k < ur[q].y;
             const double A1 = ab2(O1[q], T1[q]), A2 = ab2(O2[q], T2[q]);
-            double g0, g1, g2, g3;
-            if (tri) {
-                const double* gsu = m.gs + 6 * (size_t)ud[q].x;
-                const double* gsd = m.gs + 6 * (size_t)ud[q].y;
-                const double2 gu = grad_of(gsu, tu[q][0], tu[q][1], tu[q][2]);
-                const double2 gd = grad_of(gsd, td[q][0], td[q][1], td[q][2]);
-                g0 = gu.x; g1 = gd.x; g2 = gu.y; g3 = gd.y;
-            } else {
-                g0 = ga[q].x; g1 = gb[q].x; g2 = ga[q].y; g3 = gb[q].y;
-            }
-            double r = 0.0;
+            const double* gsu = m.gs + 6 * (size_t)ud[q].x;
+            const double* gsd = m.gs + 6 * (size_t)ud[q].y;
+            const double2 gu = grad_of(gsu, tu[q][0], tu[q][1], tu[q][2]);
+            const double2 gd = grad_of(gsd, td[q][0], td[q][1], td[q][2]);
+            const double g0 = tri ? gu.x : ga[q].x, g1 = tri ? gd.x : gb[q].x;
+            const double g2 = tri ? gu.y : ga[q].y, g3 = tri ? gd.y : gb[q].y;
             const double v = vf[q];
-            if (v != 0.0) {
-                const double av = fabs(v);
-                const double flo = -0.5 * (T1[q] * (v + av) + T2[q] * (v - av));
-                const double diff = A2 - A1;
-                const double Tm1 = A1 + (2.0 * diff + cc[q].x * g0 + cc[q].y * g2) / 6.0;
-                const double Tm2 = A2 - (2.0 * diff + cc[q].x * g1 + cc[q].y * g3) / 6.0;
-                const double ho = -0.5 * ((v + av) * Tm1 + (v - av) * Tm2);
-                r = ho - flo;
-            }
+            const double av = fabs(v);
+            const double flo = -0.5 * (T1[q] * (v + av) + T2[q] * (v - av));
+            const double diff = A2 - A1;
+            const double Tm1 = A1 + (2.0 * diff + cc[q].x * g0 + cc[q].y * g2) / 6.0;
+            const double Tm2 = A2 - (2.0 * diff + cc[q].x * g1 + cc[q].y * g3) / 6.0;
+            const double ho = -0.5 * ((v + av) * Tm1 + (v - av) * Tm2);
+            const double r = v != 0.0 ? ho - flo : 0.0;
             out[(size_t)e[q] * L + k] = r;
         }
     }
@@ -472,7 +467,7 @@ kf_pass2(FusedArgs a) {
         }
         // a1-a3 and b1 inputs: the triangles' vertex columns of LO and T and the
         // antidiffusive fluxes of the incident edges, loaded in one basic block
-        double lv[DM][3], tw[DM][3], fh[DM];
+        double lv[DM][3] = {}, tw[DM][3] = {}, fh[DM] = {};
         if (rd) {
 #pragma unroll
             for (int q = 0; q < DM; ++q) {
@@ -483,29 +478,25 @@ kf_pass2(FusedArgs a) {
                 fh[q] = adfh[(size_t)(es[q].x >> 1) * L + k];
             }
         }
-        if (rd) {
 #pragma unroll
-            for (int q = 0; q < DM; ++q) {
-                if (q >= cnt.y) break;
-                double amax = -FCT_BIG, amin = FCT_BIG;
-                if (in_layer(m.elev[el[q].x], k)) {
-                    amax = fmax(fmax(fmax(lv[q][0], tw[q][0]), fmax(lv[q][1], tw[q][1])),
-                                fmax(lv[q][2], tw[q][2]));
-                    amin = fmin(fmin(fmin(lv[q][0], tw[q][0]), fmin(lv[q][1], tw[q][1])),
-                                fmin(lv[q][2], tw[q][2]));
-                }
-                tvx[s] = fmax(tvx[s], amax);
-                tvn[s] = fmin(tvn[s], amin);
-            }
+        for (int q = 0; q < DM; ++q) {
+            const bool ew = in_layer(m.elev[el[q].x], k);
+            const double amax = ew ? fmax(fmax(fmax(lv[q][0], tw[q][0]), fmax(lv[q][1], tw[q][1])),
+                                          fmax(lv[q][2], tw[q][2]))
+                                   : -FCT_BIG;
+            const double amin = ew ? fmin(fmin(fmin(lv[q][0], tw[q][0]), fmin(lv[q][1], tw[q][1])),
+                                          fmin(lv[q][2], tw[q][2]))
+                                   : FCT_BIG;
+            const bool use = rd && q < cnt.y;
+            tvx[s] = use ? fmax(tvx[s], amax) : tvx[s];
+            tvn[s] = use ? fmin(tvn[s], amin) : tvn[s];
         }
-        if (wet) {
 #pragma unroll
-            for (int q = 0; q < DM; ++q) {
-                if (q >= cnt.x) break;
-                const double f = (es[q].x & 1) ? -fh[q] : fh[q];
-                ap[s] = ap[s] + fmax(f, 0.0);
-                am[s] = am[s] + fmin(f, 0.0);
-            }
+        for (int q = 0; q < DM; ++q) {
+            const double f = (es[q].x & 1) ? -fh[q] : fh[q];
+            const bool use = wet && q < cnt.x;
+            ap[s] = ap[s] + (use ? fmax(f, 0.0) : 0.0);
+            am[s] = am[s] + (use ? fmin(f, 0.0) : 0.0);
         }
     }
     double tvxm[NS], tvxp[NS], tvnm[NS], tvnp[NS];
@@ -574,7 +565,7 @@ kf_pass3(FusedArgs a) {
             Tc[s] = T[b0 + k];
             Ac[s] = ab2(To[b0 + k], Tc[s]);
         }
-        double fh[DM], po[DM], mo[DM];
+        double fh[DM] = {}, po[DM] = {}, mo[DM] = {};
         if (wet) {
 #pragma unroll
             for (int q = 0; q < DM; ++q) {
@@ -587,19 +578,16 @@ kf_pass3(FusedArgs a) {
         if (wet) {
             Pc[s] = pf[b0 + k];
             Mc[s] = mf[b0 + k];
+        }
 #pragma unroll
-            for (int q = 0; q < DM; ++q) {
-                if (q >= cnt.x) break;
-                const double f = fh[q];
-                if (f == 0.0) continue;
-                if (es[q].x & 1) {
-                    const double ae = f >= 0.0 ? fmin(po[q], Mc[s]) : fmin(mo[q], Pc[s]);
-                    acc[s] = acc[s] + (-(ae * f));
-                } else {
-                    const double ae = f >= 0.0 ? fmin(Pc[s], mo[q]) : fmin(Mc[s], po[q]);
-                    acc[s] = acc[s] + ae * f;
-                }
-            }
+        for (int q = 0; q < DM; ++q) {
+            const double f = fh[q];
+            const bool side = es[q].x & 1;
+            const double ae = side ? (f >= 0.0 ? fmin(po[q], Mc[s]) : fmin(mo[q], Pc[s]))
+                                   : (f >= 0.0 ? fmin(Pc[s], mo[q]) : fmin(Mc[s], po[q]));
+            const double fl = ae * f;
+            const bool use = wet && q < cnt.x && f != 0.0;
+            acc[s] = acc[s] + (use ? (side ? -fl : fl) : 0.0);
         }
     }
     double fvc[NS];

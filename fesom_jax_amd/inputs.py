@@ -165,3 +165,113 @@ def make_case(mesh: Mesh, *, seed=0, tracers=("T", "S"), dt=2700.0, speed=0.5,
     Ts = [make_tracer(mesh, rng, kind) for kind in tracers]
     return dict(uv=uv, w_e=w, helem=helem, hnode=hnode, hnode_new=hnode.copy(),
                 T=Ts, T_old=[t.copy() for t in Ts], dt=dt, cfl=(ch, cv), vel_scale=fac)
+
+
+# --------------------------------------------------------------------------
+# the same recipe with the heavy [entity, level] arrays built on a GPU (torch), for
+# meshes whose inputs are too large to build quickly on the host
+# --------------------------------------------------------------------------
+def _t_csr_sum(torch, off, entries, vals, nrows):
+    """Row sums of a CSR list on the device, each row accumulated in stored order
+    from +0.0 (gathers only: deterministic)."""
+    counts = off[1:] - off[:-1]
+    out = torch.zeros((nrows,) + tuple(vals.shape[1:]), dtype=vals.dtype, device=vals.device)
+    for j in range(int(counts.max().item()) if counts.numel() else 0):
+        rows = torch.nonzero(counts > j).squeeze(1)
+        out[rows] = out[rows] + vals[entries[off[rows] + j]]
+    return out
+
+
+def make_case_device(mesh: Mesh, device, *, seed=0, tracers=("T", "S"), dt=2700.0,
+                     speed=0.5, efold=1000.0, max_cfl=0.5, smooth_passes=3, amp=0.5):
+    """make_case's recipe with the [entity, level] arrays computed by torch on
+    ``device``; returns torch tensors. The random draws are numpy's (same seed), the
+    arithmetic the same up to rounding; ``w_e`` is rescaled with the velocity rather
+    than recomputed when the CFL cap applies."""
+    import torch
+    dev = torch.device(device)
+    f64 = torch.float64
+    rng = np.random.default_rng(seed)
+    up = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    N, E, L = mesh.nod2D, mesh.elem2D, mesh.nl
+    nlm = up(mesh.node_layer_mask)
+    elm = up(mesh.elem_layer_mask)
+    zn = up(mesh.zbar_3d_n)
+    hnode = torch.zeros((N, L), dtype=f64, device=dev)
+    hnode[:, :-1] = zn[:, :-1] - zn[:, 1:]
+    hnode = torch.where(nlm, hnode, torch.zeros((), dtype=f64, device=dev))
+    dz = np.zeros(L)
+    dz[:-1] = mesh.zbar[:-1] - mesh.zbar[1:]
+    helem = torch.where(elm, up(dz)[None, :].expand(E, L), torch.zeros((), dtype=f64, device=dev))
+    # velocity
+    psi = random_sph_field(node_xyz(mesh), 8, rng)
+    gs = mesh.gradient_sca
+    pp = psi[mesh.elem_nodes]
+    dpx = gs[:, 0] * pp[:, 0] + gs[:, 1] * pp[:, 1] + gs[:, 2] * pp[:, 2]
+    dpy = gs[:, 3] * pp[:, 0] + gs[:, 4] * pp[:, 1] + gs[:, 5] * pp[:, 2]
+    u0, v0 = -dpy, dpx
+    smax = np.sqrt(u0 * u0 + v0 * v0).max()
+    scale = speed / smax if smax > 0 else 0.0
+    prof = np.zeros(L)
+    prof[:-1] = np.exp(mesh.Z / efold)
+    pr = up(prof)[None, :]
+    zero = torch.zeros((), dtype=f64, device=dev)
+    uv = torch.empty((E, L, 2), dtype=f64, device=dev)
+    uv[:, :, 0] = torch.where(elm, up(u0 * scale)[:, None] * pr, zero)
+    uv[:, :, 1] = torch.where(elm, up(v0 * scale)[:, None] * pr, zero)
+    # w by continuity
+    et = mesh.edge_tri
+    el1 = up(np.where(et[:, 0] >= 0, et[:, 0], 0).astype(np.int64))
+    el2 = up(np.where(et[:, 1] >= 0, et[:, 1], 0).astype(np.int64))
+    has1 = up(et[:, 0] >= 0)[:, None]
+    has2 = up(et[:, 1] >= 0)[:, None]
+    cr = up(mesh.edge_cross_dxdy)
+    U, V = uv[:, :, 0], uv[:, :, 1]
+    t1 = torch.where(elm[el1] & has1, (V[el1] * cr[:, 0:1] - U[el1] * cr[:, 1:2]) * helem[el1], zero)
+    t2 = torch.where(elm[el2] & has2, (V[el2] * cr[:, 2:3] - U[el2] * cr[:, 3:4]) * helem[el2], zero)
+    c = t1 + (-1.0) * t2
+    del t1, t2
+    vals = torch.stack([c, -c], dim=1).reshape(-1, L)
+    del c
+    off, ent = node_edge_csr(mesh)
+    div = _t_csr_sum(torch, up(off), up(ent), vals, N)
+    del vals
+    w = torch.flip(torch.cumsum(torch.flip(div, [1]), 1), [1])
+    area = up(mesh.area)
+    w = w / torch.where(area > 0, area, torch.ones((), dtype=f64, device=dev))
+    w = torch.where(up(mesh.node_iface_mask), w, zero)
+    # CFL cap
+    sp = torch.sqrt(uv[..., 0] ** 2 + uv[..., 1] ** 2)
+    ch = float((sp * dt / up(np.sqrt(mesh.elem_area))[:, None]).max())
+    hinf = torch.where(hnode > 0, hnode, torch.full((), float("inf"), dtype=f64, device=dev))
+    cv = float((w.abs() * dt / hinf).max())
+    fac = 1.0
+    if max(ch, cv) > max_cfl:
+        fac = max_cfl / max(ch, cv)
+        uv = uv * fac
+        w = w * fac
+        ch, cv = ch * fac, cv * fac
+    # tracers
+    noff = up(mesh.nod_in_elem2D_offsets.astype(np.int64))
+    nie = up(mesh.nod_in_elem2D.astype(np.int64))
+    ea = up(mesh.elem_area)
+    en = up(mesh.elem_nodes.astype(np.int64))
+    den = _t_csr_sum(torch, noff, nie, ea, N)
+    d = np.zeros(L)
+    d[:-1] = -mesh.Z
+    Ts = []
+    for kind in tracers:
+        f = up(rng.standard_normal((N, L)))
+        for _ in range(smooth_passes):
+            em = (f[en[:, 0]] + f[en[:, 1]] + f[en[:, 2]]) / 3.0
+            f = _t_csr_sum(torch, noff, nie, em * ea[:, None], N) / den[:, None]
+        if kind == "T":
+            base = up(28.0 * np.exp(-d / 800.0) - 1.0)
+        elif kind == "S":
+            base = up(np.full(L, 35.0))
+        else:
+            raise ValueError(f"unknown tracer kind {kind!r}")
+        Ts.append(torch.where(nlm, base[None, :] + amp * f, zero))
+    return dict(uv=uv.contiguous(), w_e=w.contiguous(), helem=helem.contiguous(),
+                hnode=hnode.contiguous(), hnode_new=hnode.clone(), T=Ts,
+                T_old=[t.clone() for t in Ts], dt=dt, cfl=(ch, cv), vel_scale=fac)

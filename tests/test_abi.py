@@ -1,0 +1,52 @@
+"""The C-ABI library loads and exports every symbol include/fesom_fct.h declares (CPU
+only: no compute call is made without a GPU)."""
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from fesom_jax_amd import _lib
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "fesom_fct.h"
+
+
+def _declared():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(fct_\w+)\(", text, re.M)))
+
+
+def test_header_and_binding_agree():
+    assert _declared() == sorted(_lib.EXPORTED_SYMBOLS)
+
+
+def test_library_exports_all_symbols():
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("library not built")
+    lib = _lib.load(require_torch=False)
+    for name in _declared():
+        assert hasattr(lib, name), name
+    assert lib.fct_abi_version() == _lib.FCT_ABI_VERSION
+    assert lib.fct_last_error() == b""
+
+
+def test_struct_layouts_match_header():
+    # field counts mirror the C structs; a missing/extra field shifts every pointer
+    text = HEADER.read_text()
+    body = text[text.index("typedef struct fct_mesh_desc"):text.index("} fct_mesh_desc;")]
+    n_ptr = len(re.findall(r"\*\s*\w+;", body)) + len(re.findall(r"\*\s*\w+,", body))
+    n_int = 5  # nod2D, elem2D, edge2D, nl, max_tracers
+    assert len(_lib.FctMeshDesc._fields_) == n_ptr + n_int
+    assert ctypes.sizeof(_lib.FctMeshDesc) == 4 * 4 + 8 * n_ptr + 8
+
+
+def test_plan_create_rejects_bad_mesh_without_gpu():
+    """Argument validation happens before any device call."""
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("library not built")
+    lib = _lib.load(require_torch=False)
+    out = ctypes.c_void_p()
+    rc = lib.fct_plan_create(None, ctypes.byref(out))
+    assert rc == _lib.FCT_E_INVALID
+    assert b"NULL" in lib.fct_last_error()
