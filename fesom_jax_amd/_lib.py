@@ -23,6 +23,8 @@ FCT_E_UNSUPPORTED = -4
 FCT_PATH_AUTO = 0
 FCT_PATH_PHASED = 1
 FCT_PATH_FUSED = 2
+FCT_PATH_BIT_PHASED = 1
+FCT_PATH_BIT_FUSED = 2
 
 FCT_TUNE_XCD = 1
 
@@ -65,7 +67,7 @@ class FctPhaseBufs(ctypes.Structure):
 
 class FctPlanInfo(ctypes.Structure):
     _fields_ = [("nod2D", c_int32), ("elem2D", c_int32), ("edge2D", c_int32), ("nl", c_int32),
-                ("workspace_bytes", c_int64), ("device", c_int32)]
+                ("workspace_bytes", c_int64), ("device", c_int32), ("paths", c_int32)]
 
 
 _LIB = None
@@ -110,6 +112,30 @@ def load(path: Path | None = None, *, require_torch: bool = True):
     if path is None:
         _LIB = lib
     return lib
+
+
+MESH_INT_FIELDS = ("edges", "edge_tri", "edge_up_dn_tri", "elem_nodes", "nlevels", "ulevels",
+                   "nlevels_nod2D", "nlevels_nod2D_min", "ulevels_nod2D", "ulevels_nod2D_max",
+                   "nod_in_elem2D_offsets", "nod_in_elem2D")
+MESH_F64_FIELDS = ("edge_cross_dxdy", "edge_dxdy", "gradient_sca", "elem_area", "elem_cos",
+                   "area", "areasvol", "Z", "zbar")
+
+
+def mesh_desc_from(mesh, max_tracers: int = 4):
+    """``fct_mesh_desc`` over contiguous host copies of the mesh fields the FCT reads
+    (the reference's Mesh names, fesom_jax/mesh.py:62-129). Returns the descriptor and
+    the arrays it points into; keep both alive until ``fct_plan_create`` returns."""
+    import numpy as np
+    host = {}
+    for k in MESH_INT_FIELDS:
+        host[k] = np.ascontiguousarray(np.asarray(getattr(mesh, k)), dtype=np.int32)
+    for k in MESH_F64_FIELDS:
+        host[k] = np.ascontiguousarray(np.asarray(getattr(mesh, k)), dtype=np.float64)
+    desc = FctMeshDesc(nod2D=int(mesh.nod2D), elem2D=int(mesh.elem2D),
+                       edge2D=int(mesh.edge2D), nl=int(mesh.nl), max_tracers=int(max_tracers))
+    for k, a in host.items():
+        setattr(desc, k, a.ctypes.data)
+    return desc, host
 
 
 def check(rc: int, what: str) -> None:

@@ -144,6 +144,51 @@ int validate_mesh(const fct_mesh_desc* d) {
     return FCT_OK;
 }
 
+// Mesh properties the fused node passes rely on (checked once; the phased path is
+// used when they fail):
+//  - every triangle adjacent to an edge (n, p) contains n and p, and every triangle
+//    around n is adjacent to the two edges of n it contains, so the vertices of the
+//    triangles around n are n and the far ends of n's edges;
+//  - a triangle's layer range lies inside each vertex's layer range, so a vertex of a
+//    triangle that is wet at level k is itself wet at k.
+bool fused_mesh_ok(const fct_mesh_desc* d, const std::vector<int>& eoff,
+                   const std::vector<int>& ecsr, const std::vector<int>& noff,
+                   const std::vector<int>& ncsr) {
+    const int N = d->nod2D;
+    auto has = [&](int el, int v) {
+        return d->elem_nodes[3 * el] == v || d->elem_nodes[3 * el + 1] == v ||
+               d->elem_nodes[3 * el + 2] == v;
+    };
+    for (int n = 0; n < N; ++n) {
+        for (int q = eoff[n]; q < eoff[n + 1]; ++q) {
+            const int e = ecsr[q] >> 1;
+            const int p = (ecsr[q] & 1) ? d->edges[2 * e] : d->edges[2 * e + 1];
+            for (int side = 0; side < 2; ++side) {
+                const int el = d->edge_tri[2 * e + side];
+                if (el >= 0 && !(has(el, n) && has(el, p))) return false;
+            }
+        }
+        for (int q = noff[n]; q < noff[n + 1]; ++q) {
+            const int el = ncsr[q];
+            for (int j = 0; j < 3; ++j) {
+                const int v = d->elem_nodes[3 * el + j];
+                if (v == n) continue;
+                bool found = false;
+                for (int r = eoff[n]; r < eoff[n + 1] && !found; ++r) {
+                    const int e = ecsr[r] >> 1;
+                    const int p = (ecsr[r] & 1) ? d->edges[2 * e] : d->edges[2 * e + 1];
+                    found = p == v && (d->edge_tri[2 * e] == el || d->edge_tri[2 * e + 1] == el);
+                }
+                if (!found) return false;
+            }
+            if (d->nlevels[el] > d->ulevels[el] &&
+                (d->ulevels_nod2D[n] > d->ulevels[el] || d->nlevels_nod2D[n] < d->nlevels[el]))
+                return false;
+        }
+    }
+    return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -241,23 +286,41 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
     int maxdeg = 0;
     for (int n = 0; n < N; ++n)
         maxdeg = std::max(maxdeg, std::max(h_eoff[n + 1] - h_eoff[n], h_noff[n + 1] - h_noff[n]));
-    const int dm = maxdeg <= 6 ? 6 : (maxdeg <= 8 ? 8 : 0);
-    std::vector<int2> h_tcnt, h_tedge;
-    std::vector<int4> h_telem;
+    int dm = maxdeg <= 6 ? 6 : (maxdeg <= 8 ? 8 : 0);
+    if (dm && (L > 128 || !fused_mesh_ok(d, h_eoff, h_ecsr, h_noff, h_ncsr))) dm = 0;
+    std::vector<int2> h_tcnt;
+    std::vector<int4> h_tedge, h_telem;
     if (dm) {
         h_tcnt.resize(N);
-        h_tedge.assign((size_t)N * dm, make_int2(0, 0));
+        h_tedge.assign((size_t)N * dm, make_int4(0, 0, 0, 0));
         h_telem.assign((size_t)N * dm, make_int4(0, 0, 0, 0));
         for (int n = 0; n < N; ++n) {
             const int de = h_eoff[n + 1] - h_eoff[n], ne = h_noff[n + 1] - h_noff[n];
             h_tcnt[n] = make_int2(de, ne);
             for (int q = 0; q < de; ++q) {
                 const int ent = h_ecsr[h_eoff[n] + q];
-                const int2 nd = h_edges[ent >> 1];
-                h_tedge[(size_t)n * dm + q] = make_int2(ent, (ent & 1) ? nd.x : nd.y);
+                const int e = ent >> 1;
+                const int2 nd = h_edges[e];
+                // layer ranges of the edge's triangles, one byte each (lo1, hi1, lo2, hi2);
+                // a missing triangle gets the empty range [0, 0)
+                unsigned packed = 0;
+                for (int side = 0; side < 2; ++side) {
+                    const int el = d->edge_tri[2 * e + side];
+                    if (el < 0) continue;
+                    const unsigned lo = (unsigned)(d->ulevels[el] - 1);
+                    const unsigned hi = (unsigned)std::max(d->nlevels[el] - 1, d->ulevels[el] - 1);
+                    packed |= (lo | (hi << 8)) << (16 * side);
+                }
+                h_tedge[(size_t)n * dm + q].x = ent;
+                h_tedge[(size_t)n * dm + q].y = (ent & 1) ? nd.x : nd.y;
+                h_tedge[(size_t)n * dm + q].z = (int)packed;
             }
             for (int q = 0; q < ne; ++q) {
                 const int el = h_ncsr[h_noff[n] + q];
+                // layer range of element slot q, lo|hi bytes
+                const unsigned lo = (unsigned)(d->ulevels[el] - 1);
+                const unsigned hi = (unsigned)std::max(d->nlevels[el] - 1, d->ulevels[el] - 1);
+                h_tedge[(size_t)n * dm + q].w = (int)(lo | (hi << 8));
                 h_telem[(size_t)n * dm + q] = make_int4(el, d->elem_nodes[3 * el],
                                                         d->elem_nodes[3 * el + 1],
                                                         d->elem_nodes[3 * el + 2]);
@@ -335,6 +398,7 @@ int fct_plan_get_info(const fct_plan* p, fct_plan_info* info) {
     info->nod2D = p->m.N; info->elem2D = p->m.E; info->edge2D = p->m.D; info->nl = p->m.L;
     info->workspace_bytes = p->bytes;
     info->device = p->device;
+    info->paths = FCT_PATH_BIT_PHASED | (p->tab.dm > 0 ? FCT_PATH_BIT_FUSED : 0);
     return FCT_OK;
 }
 

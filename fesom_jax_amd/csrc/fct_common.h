@@ -45,7 +45,9 @@ struct DevMesh {
 struct NodeTab {
     int dm = 0;               // slots per node (6 or 8); 0 = mesh not supported
     const int2* cnt;          // [N] (incident edges, incident elements)
-    const int2* edge;         // [N*dm] (2*edge + side, other node), reference order
+    const int4* edge;         // [N*dm] (2*edge + side, other node, layer ranges of the
+                              //  edge's triangles as bytes lo1|hi1|lo2|hi2, layer range of
+                              //  element slot q as bytes lo|hi), edges in reference order
     const int4* elem;         // [N*dm] (element, v0, v1, v2), increasing element id
 };
 

@@ -1,15 +1,16 @@
 // Fused FP64 FCT path for gfx950: two edge passes and three node-column passes.
 //
 // Mapping: one 64-lane wavefront per task (a node or an edge, times a tracer); lane l of
-// slot s holds level k = 64*s + l of the task's column (NS = 1 for nl <= 64, 2 for
-// nl <= 128), so every load of a node or edge column is one coalesced row segment, and
-// vertical neighbours (k-2 .. k+1) move between lanes with cross-lane shuffles. Task
-// indices are wave-uniform (scalar registers); each node reads its stencil (incident
-// edges in the reference's summation order, incident triangles with their vertices)
-// from a per-node table with a few scalar loads, and then issues the loads of all its
-// neighbours' columns back to back, so a wavefront waits on memory about once per
-// phase instead of once per edge. Only wet lanes are loaded and computed; T_new is
-// written on every lane (dry lanes: T + 0, as the reference's ale_reconstruct does).
+// slot s holds level k = 64*s + l of the task's column, so every load of a node or edge
+// column is one coalesced row segment, and vertical neighbours move between lanes with
+// DPP wavefront shifts. A node whose column fits in 64 lanes takes the one-slot body
+// even when nl > 64 (a wave-uniform branch), so deep columns alone pay for the second
+// slot. Task indices are wave-uniform (scalar registers); each node reads its stencil
+// (incident edges in the reference's summation order with the layer ranges of their
+// triangles, incident triangles with their vertices) from a per-node table with a few
+// scalar loads, and then issues the loads of all its neighbours' columns back to back.
+// Only wet lanes are computed; T_new is written on every lane (dry lanes: T + 0, as the
+// reference's ale_reconstruct does).
 //
 //   kf_vflux  (edge) : vflux on the edge's wet levels (edge_vflux, tracer_adv.py:69-93)
 //   kf_pass1  (node) : LO and the node-averaged gradient gnode
@@ -25,7 +26,10 @@
 // reference's jnp expressions). Terms the reference adds as exact zeros (edges or
 // elements dry at level k, where the reference multiplies a finite value by 0) are
 // skipped, which leaves every sum bit-identical while the inputs are finite on the
-// lanes the reference multiplies by zero.
+// lanes the reference multiplies by zero. The cluster bounds use the fact that the
+// vertices of the triangles around a node are the node and the far ends of its edges
+// (checked on the host when the plan is built): a maximum over the triangles' vertex
+// maxima equals the maximum over that vertex set, whatever the grouping.
 
 #include "fct_common.h"
 
@@ -33,14 +37,35 @@ namespace fct {
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 
+// value of lane l-1 (wave_shr:1) or l+1 (wave_shl:1) of a 64-bit register; lane 0 resp.
+// 63 receives 0. Called with all 64 lanes active.
+__device__ __forceinline__ double wave_shr1(double v) {
+    int2 x = __builtin_bit_cast(int2, v);
+    x.x = __builtin_amdgcn_update_dpp(0, x.x, 0x138, 0xf, 0xf, false);
+    x.y = __builtin_amdgcn_update_dpp(0, x.y, 0x138, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, x);
+}
+__device__ __forceinline__ double wave_shl1(double v) {
+    int2 x = __builtin_bit_cast(int2, v);
+    x.x = __builtin_amdgcn_update_dpp(0, x.x, 0x130, 0xf, 0xf, false);
+    x.y = __builtin_amdgcn_update_dpp(0, x.y, 0x130, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, x);
+}
+__device__ __forceinline__ double lane_value(double v, int l) {
+    int2 x = __builtin_bit_cast(int2, v);
+    x.x = __builtin_amdgcn_readlane(x.x, l);
+    x.y = __builtin_amdgcn_readlane(x.y, l);
+    return __builtin_bit_cast(double, x);
+}
+
 // value at level k-1 of a column held as NS slots; `at0` is used at k = 0
 template <int NS>
 __device__ __forceinline__ void col_prev(const double (&v)[NS], double (&o)[NS], double at0) {
     const int l = lane_id();
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-        const double up = __shfl_up(v[s], 1);
-        const double wrap = (s > 0) ? __shfl(v[s > 0 ? s - 1 : 0], 63) : at0;
+        const double up = wave_shr1(v[s]);
+        const double wrap = (s > 0) ? lane_value(v[s > 0 ? s - 1 : 0], 63) : at0;
         o[s] = (l == 0) ? wrap : up;
     }
 }
@@ -51,16 +76,21 @@ __device__ __forceinline__ void col_next(const double (&v)[NS], double (&o)[NS],
     const int l = lane_id();
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-        const double dn = __shfl_down(v[s], 1);
-        const double wrap = (s + 1 < NS) ? __shfl(v[s + 1 < NS ? s + 1 : s], 0) : atend;
+        const double dn = wave_shl1(v[s]);
+        const double wrap = (s + 1 < NS) ? lane_value(v[s + 1 < NS ? s + 1 : s], 0) : atend;
         o[s] = (l == 63) ? wrap : dn;
     }
 }
 
 template <int NS>
-__device__ __forceinline__ double col_bcast0(const double (&v)[NS]) { return __shfl(v[0], 0); }
+__device__ __forceinline__ double col_bcast0(const double (&v)[NS]) { return lane_value(v[0], 0); }
 
 __device__ __forceinline__ double ab2(double told, double t) { return AB_OLD * told + AB_NEW * t; }
+
+__device__ __forceinline__ bool in_bytes(int packed, int shift, int k) {
+    const int lo = (packed >> shift) & 0xff, hi = (packed >> (shift + 8)) & 0xff;
+    return k >= lo && k < hi;
+}
 
 // Block -> tile remap so that each XCD works through one contiguous range of tiles
 // (workgroups are dealt to the XCDs round-robin; node and edge order is spatially
@@ -185,7 +215,7 @@ kf_pass1(FusedArgs a) {
     const size_t b0 = (size_t)n * L;
     const size_t jo = (size_t)t.j * m.N * L;
     const int kneed = nr.y + 1 < L ? nr.y + 1 : L;
-    int2 es[DM];
+    int4 es[DM];
     int4 el[DM];
 #pragma unroll
     for (int q = 0; q < DM; ++q) {
@@ -250,7 +280,7 @@ kf_pass1(FusedArgs a) {
 #pragma unroll
         for (int q = 0; q < DM; ++q) {
             const int e = el[q].x;
-            const bool use = q < cnt.y && in_layer(m.elev[e], k);
+            const bool use = q < cnt.y && in_bytes(es[q].w, 0, k);
             const double* gs = m.gs + 6 * (size_t)e;
             const double gx = gs[0] * tv[q][0] + gs[1] * tv[q][1] + gs[2] * tv[q][2];
             const double gy = gs[3] * tv[q][0] + gs[4] * tv[q][1] + gs[5] * tv[q][2];
@@ -400,7 +430,7 @@ __device__ __forceinline__ double adfv_at(const DevMesh& m, int k, int2 nr, size
     return hi - lo;
 }
 
-// Column of adf_v for the node: every lane k <= nzmax (0 elsewhere).
+// Column of adf_v for the node: every lane k <= nzmax (0 elsewhere). All lanes active.
 template <int NS>
 __device__ __forceinline__ void adfv_column(const FusedArgs& a, int n, int2 nr,
                                             const double (&Tc)[NS], const double (&Ac)[NS],
@@ -429,28 +459,22 @@ __device__ __forceinline__ void adfv_column(const FusedArgs& a, int n, int2 nr,
 // kf_pass2: cluster bounds, P+/P-, limiter factors
 // ---------------------------------------------------------------------------
 template <int NS, int DM>
-__global__ void __launch_bounds__(256)
-kf_pass2(FusedArgs a) {
+__device__ __forceinline__ void pass2_body(const FusedArgs& a, const Task& t, int2 nr) {
     const DevMesh& m = a.m;
-    Task t;
-    if (!get_task(m.N, a.K, a.xcd, t)) return;
     const int n = t.i, L = m.L;
-    const int2 nr = m.nlev[n];
     const int2 cnt = a.tab.cnt[n];
-    const double* __restrict__ T = a.T[t.j];
-    const double* __restrict__ To = a.Told[t.j];
     const size_t jo = (size_t)t.j * m.N * L;
-    const double* __restrict__ LO = a.LO + jo;
-    const double* __restrict__ adfh = a.adfh + (size_t)t.j * m.D * L;
     const size_t b0 = (size_t)n * L;
+    const double* __restrict__ Tb = a.T[t.j];
+    const double* __restrict__ LOb = a.LO + jo;
+    const double* __restrict__ Tr = Tb + b0;
+    const double* __restrict__ Or = a.Told[t.j] + b0;
+    const double* __restrict__ LOr = LOb + b0;
+    const double* __restrict__ adfh = a.adfh + (size_t)t.j * m.D * L;
     const int kneed = nr.y + 1 < L ? nr.y + 1 : L;
-    int2 es[DM];
-    int4 el[DM];
+    int4 es[DM];
 #pragma unroll
-    for (int q = 0; q < DM; ++q) {
-        es[q] = a.tab.edge[(size_t)n * DM + (q < cnt.x ? q : 0)];
-        el[q] = a.tab.elem[(size_t)n * DM + (q < cnt.y ? q : 0)];
-    }
+    for (int q = 0; q < DM; ++q) es[q] = a.tab.edge[(size_t)n * DM + (q < cnt.x ? q : 0)];
     double Tc[NS], Ac[NS], LOc[NS], tvx[NS], tvn[NS], ap[NS], am[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -460,37 +484,45 @@ kf_pass2(FusedArgs a) {
         const int k = 64 * s + lane_id();
         const bool rd = k < kneed;
         const bool wet = in_layer(nr, k);
+        // own column and the neighbours' LO, T and the edges' antidiffusive fluxes,
+        // loaded in one basic block
+        double lnb[DM] = {}, tnb[DM] = {}, fh[DM] = {};
+        double lo_own = 0.0;
         if (rd) {
-            Tc[s] = T[b0 + k];
-            Ac[s] = ab2(To[b0 + k], Tc[s]);
-            if (wet) LOc[s] = LO[b0 + k];
-        }
-        // a1-a3 and b1 inputs: the triangles' vertex columns of LO and T and the
-        // antidiffusive fluxes of the incident edges, loaded in one basic block
-        double lv[DM][3] = {}, tw[DM][3] = {}, fh[DM] = {};
-        if (rd) {
+            Tc[s] = Tr[k];
+            Ac[s] = ab2(Or[k], Tc[s]);
+            lo_own = LOr[k];
 #pragma unroll
             for (int q = 0; q < DM; ++q) {
-                const size_t j0 = (size_t)el[q].y * L + k, j1 = (size_t)el[q].z * L + k,
-                             j2 = (size_t)el[q].w * L + k;
-                lv[q][0] = LO[j0]; lv[q][1] = LO[j1]; lv[q][2] = LO[j2];
-                tw[q][0] = T[j0]; tw[q][1] = T[j1]; tw[q][2] = T[j2];
+                const size_t r = (size_t)es[q].y * L + k;
+                lnb[q] = LOb[r];
+                tnb[q] = Tb[r];
                 fh[q] = adfh[(size_t)(es[q].x >> 1) * L + k];
             }
         }
+        LOc[s] = wet ? lo_own : 0.0;
+        // a1-a3: max/min of (LO, T) over the vertices of the node's triangles that are
+        // wet at k, and the -+1e3 pad when one of them is dry at k
+        bool act = false, inact = false;
 #pragma unroll
         for (int q = 0; q < DM; ++q) {
-            const bool ew = in_layer(m.elev[el[q].x], k);
-            const double amax = ew ? fmax(fmax(fmax(lv[q][0], tw[q][0]), fmax(lv[q][1], tw[q][1])),
-                                          fmax(lv[q][2], tw[q][2]))
-                                   : -FCT_BIG;
-            const double amin = ew ? fmin(fmin(fmin(lv[q][0], tw[q][0]), fmin(lv[q][1], tw[q][1])),
-                                          fmin(lv[q][2], tw[q][2]))
-                                   : FCT_BIG;
-            const bool use = rd && q < cnt.y;
-            tvx[s] = use ? fmax(tvx[s], amax) : tvx[s];
-            tvn[s] = use ? fmin(tvn[s], amin) : tvn[s];
+            const bool in = in_bytes(es[q].w, 0, k);
+            act = act || (q < cnt.y && in);
+            inact = inact || (q < cnt.y && !in);
         }
+        double mx = act ? fmax(LOc[s], Tc[s]) : -INFINITY;
+        double mn = act ? fmin(LOc[s], Tc[s]) : INFINITY;
+#pragma unroll
+        for (int q = 0; q < DM; ++q) {
+            const bool inc = q < cnt.x && (in_bytes(es[q].z, 0, k) || in_bytes(es[q].z, 16, k));
+            mx = inc ? fmax(mx, fmax(lnb[q], tnb[q])) : mx;
+            mn = inc ? fmin(mn, fmin(lnb[q], tnb[q])) : mn;
+        }
+        mx = inact ? fmax(mx, -FCT_BIG) : mx;
+        mn = inact ? fmin(mn, FCT_BIG) : mn;
+        tvx[s] = rd ? mx : -INFINITY;
+        tvn[s] = rd ? mn : INFINITY;
+        // b1: P+/P- from the incident edges in reference order
 #pragma unroll
         for (int q = 0; q < DM; ++q) {
             const double f = (es[q].x & 1) ? -fh[q] : fh[q];
@@ -531,17 +563,23 @@ kf_pass2(FusedArgs a) {
     }
 }
 
+template <int NS, int DM>
+__global__ void __launch_bounds__(256)
+kf_pass2(FusedArgs a) {
+    Task t;
+    if (!get_task(a.m.N, a.K, a.xcd, t)) return;
+    const int2 nr = a.m.nlev[t.i];
+    if (NS == 1 || nr.y < 64) pass2_body<1, DM>(a, t, nr);
+    else pass2_body<NS, DM>(a, t, nr);
+}
+
 // ---------------------------------------------------------------------------
 // kf_pass3: limited fluxes, divergence, reconstruction
 // ---------------------------------------------------------------------------
 template <int NS, int DM>
-__global__ void __launch_bounds__(256)
-kf_pass3(FusedArgs a) {
+__device__ __forceinline__ void pass3_body(const FusedArgs& a, const Task& t, int2 nr) {
     const DevMesh& m = a.m;
-    Task t;
-    if (!get_task(m.N, a.K, a.xcd, t)) return;
     const int n = t.i, L = m.L;
-    const int2 nr = m.nlev[n];
     const int2 cnt = a.tab.cnt[n];
     const double* __restrict__ T = a.T[t.j];
     const double* __restrict__ To = a.Told[t.j];
@@ -551,7 +589,7 @@ kf_pass3(FusedArgs a) {
     const double* __restrict__ adfh = a.adfh + (size_t)t.j * m.D * L;
     const size_t b0 = (size_t)n * L;
     const int kneed = nr.y + 1 < L ? nr.y + 1 : L;
-    int2 es[DM];
+    int4 es[DM];
 #pragma unroll
     for (int q = 0; q < DM; ++q) es[q] = a.tab.edge[(size_t)n * DM + (q < cnt.x ? q : 0)];
     double Tc[NS], Ac[NS], Pc[NS], Mc[NS], acc[NS];
@@ -634,6 +672,18 @@ kf_pass3(FusedArgs a) {
         const double dT = hn > 0.0 ? del / safe_hn : 0.0;
         Tn[i] = Ti + dT;
     }
+    // levels past the slots (a shallow column taking the one-slot body): dry copies
+    for (int k = 64 * NS + lane_id(); k < L; k += 64) Tn[b0 + k] = T[b0 + k] + 0.0;
+}
+
+template <int NS, int DM>
+__global__ void __launch_bounds__(256)
+kf_pass3(FusedArgs a) {
+    Task t;
+    if (!get_task(a.m.N, a.K, a.xcd, t)) return;
+    const int2 nr = a.m.nlev[t.i];
+    if (NS == 1 || nr.y < 64) pass3_body<1, DM>(a, t, nr);
+    else pass3_body<NS, DM>(a, t, nr);
 }
 
 // ---------------------------------------------------------------------------

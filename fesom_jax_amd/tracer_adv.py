@@ -42,11 +42,6 @@ def _torch():
 # --------------------------------------------------------------------------
 # plan (device topology) cache
 # --------------------------------------------------------------------------
-_MESH_INT = ("edges", "edge_tri", "edge_up_dn_tri", "elem_nodes", "nlevels", "ulevels",
-             "nlevels_nod2D", "nlevels_nod2D_min", "ulevels_nod2D", "ulevels_nod2D_max",
-             "nod_in_elem2D_offsets", "nod_in_elem2D")
-_MESH_F64 = ("edge_cross_dxdy", "edge_dxdy", "gradient_sca", "elem_area", "elem_cos",
-             "area", "areasvol", "Z", "zbar")
 
 
 class FctPlan:
@@ -58,19 +53,11 @@ class FctPlan:
         dev = torch.device("cuda", torch.cuda.current_device() if device is None
                            else torch.device(device).index or 0)
         self.device = dev
-        host = {}
-        for k in _MESH_INT:
-            host[k] = np.ascontiguousarray(np.asarray(getattr(mesh, k)), dtype=np.int32)
-        for k in _MESH_F64:
-            host[k] = np.ascontiguousarray(np.asarray(getattr(mesh, k)), dtype=np.float64)
         self.nod2D = int(mesh.nod2D)
         self.elem2D = int(mesh.elem2D)
         self.edge2D = int(mesh.edge2D)
         self.nl = int(mesh.nl)
-        desc = _lib.FctMeshDesc(nod2D=self.nod2D, elem2D=self.elem2D, edge2D=self.edge2D,
-                                nl=self.nl, max_tracers=int(max_tracers))
-        for k, a in host.items():
-            setattr(desc, k, a.ctypes.data)
+        desc, _host = _lib.mesh_desc_from(mesh, max_tracers)
         ptr = c_void_p()
         with torch.cuda.device(dev):
             _lib.check(self.lib.fct_plan_create(ctypes.byref(desc), ctypes.byref(ptr)),
@@ -80,6 +67,7 @@ class FctPlan:
         info = _lib.FctPlanInfo()
         _lib.check(self.lib.fct_plan_get_info(ptr, ctypes.byref(info)), "fct_plan_get_info")
         self.workspace_bytes = int(info.workspace_bytes)
+        self.fused_available = bool(info.paths & _lib.FCT_PATH_BIT_FUSED)
 
     def close(self):
         self._finalizer()

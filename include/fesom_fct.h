@@ -118,7 +118,11 @@ typedef struct fct_plan_info {
     int32_t nod2D, elem2D, edge2D, nl;
     int64_t workspace_bytes;   /* device bytes owned by the plan */
     int32_t device;            /* HIP device ordinal the plan lives on */
+    int32_t paths;             /* FCT_PATH_BIT_* of the paths this mesh can take */
 } fct_plan_info;
+
+#define FCT_PATH_BIT_PHASED 1
+#define FCT_PATH_BIT_FUSED 2
 
 /* Library ABI version (FCT_ABI_VERSION). No reference counterpart. */
 int fct_abi_version(void);
