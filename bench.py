@@ -76,11 +76,15 @@ def main():
     ap.add_argument("--dt", type=float, default=2700.0)
     ap.add_argument("--cpu-level", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=("plain", "batched"), default="plain",
+                    help="plain: one icosa-9 instance (the metric); batched: 64 uneven "
+                         "instances in one batched launch vs 64 sequential plain launches")
+    ap.add_argument("--instances", type=int, default=64)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md). Run with --gpus 1.",
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(instances and meshes would shard across GPUs by process). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 
@@ -94,6 +98,8 @@ def main():
     if not torch.cuda.is_available():
         raise RuntimeError("bench.py needs a ROCm GPU")
     dev = torch.device("cuda", 0)
+    if args.workload == "batched":
+        return run_batched(args, dev)
     K = args.tracers
 
     t0 = time.time()
@@ -186,6 +192,83 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out), flush=True)
+
+
+def run_batched(args, dev):
+    """64 independent instances: icosahedral levels 5-8 (equal odds, i.e. log-uniform
+    over the four node counts), nl 47 or 70, 2-4 tracers, per-instance seeded fields.
+    Instances with the same (level, nl) share the generated mesh geometry but own their
+    plan, workspace, flow and tracers. Times S steps of (a) every instance through its
+    own plain fct_forward launches, one after the other, and (b) one batched launch set
+    per step replayed as a HIP graph, with the launch shape picked by the autotuner."""
+    import torch
+    from fesom_jax_amd import autotune
+    from fesom_jax_amd.batch import FctBatch, Instance
+    from fesom_jax_amd.inputs import make_case_device
+    from fesom_jax_amd.meshgen import icosahedral_mesh
+    from fesom_jax_amd.perfmodel import HBM_PEAK_GBS, algorithmic_bytes
+
+    rng = np.random.default_rng(args.seed)
+    specs = [(int(rng.integers(5, 9)), int(rng.choice([47, 70])), int(rng.integers(2, 5)))
+             for _ in range(args.instances)]
+    meshes, insts, B = {}, [], 0
+    t0 = time.time()
+    for i, (lv, nl, K) in enumerate(specs):
+        if (lv, nl) not in meshes:
+            meshes[(lv, nl)] = icosahedral_mesh(lv, nl, seed=args.seed + 10 * lv + nl)
+        mesh = meshes[(lv, nl)]
+        case = make_case_device(mesh, dev, seed=args.seed + 1000 + i, dt=args.dt,
+                                tracers=("T", "S", "T", "S")[:K])
+        insts.append(Instance.from_case(mesh, case, dev))
+        B += algorithmic_bytes(mesh, K)
+        del case
+    torch.cuda.synchronize()
+    U = sum(inst.updates for inst in insts)
+    log(f"{len(insts)} instances ({time.time() - t0:.1f}s): levels "
+        f"{sorted(set(s[0] for s in specs))}, {U} updates/step, "
+        f"device memory {torch.cuda.memory_allocated(dev) / 2**30:.1f} GiB (torch) ")
+    stream = torch.cuda.current_stream(dev)
+
+    def seq_step():
+        for inst in insts:
+            inst.step_alone(stream)
+
+    batch = FctBatch(insts)
+    best, table = autotune.tune_batch(batch, stream, reps=2, warmup=1)
+    log(f"autotuner: {autotune.describe(best)} "
+        f"({min(ms for _, ms in table):.3f} .. {max(ms for _, ms in table):.3f} ms/step)")
+    batch.prepare()
+    ms_seq = autotune.time_ms(seq_step, stream, reps=args.steps, warmup=args.warmup)
+    batch.prepare()
+    ms_bat = autotune.time_ms(lambda: batch.step(graph=True, stream=stream), stream,
+                              reps=args.steps, warmup=args.warmup)
+    log(f"sequential plain {ms_seq:.3f} ms/step, batched graph {ms_bat:.3f} ms/step")
+    for inst in insts:
+        for t in inst.T():
+            if not bool(torch.isfinite(t).all()):
+                raise RuntimeError("non-finite tracer after the batched steps")
+    achieved = B / (ms_bat * 1e-3) / 1e9
+    out = {
+        "metric": "FP64 FCT node-level updates/s, 64 uneven instances in one batched "
+                  "launch (vs 64 sequential plain launches)",
+        "value": U / (ms_bat * 1e-3), "unit": "node-level updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_bat,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{len(insts)} instances, icosa 5-8, nl 47/70, 2-4 tracers",
+                   "instances": [{"level": lv, "nl": nl, "tracers": K} for lv, nl, K in specs],
+                   "updates_per_step": U, "dt": args.dt, "parallelism": "single GPU",
+                   "tuning": autotune.describe(best)},
+        "sequential_plain": {"value": U / (ms_seq * 1e-3), "ms_per_step": ms_seq},
+        "speedup_vs_sequential": ms_seq / ms_bat,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": None, "kernel": "batched fct step (all launches of one step)",
+                     "bytes_per_step": B},
+        "cpu_baseline": None,
+    }
+    batch.close()
     print(json.dumps(out), flush=True)
 
 

@@ -27,6 +27,10 @@ FCT_PATH_BIT_PHASED = 1
 FCT_PATH_BIT_FUSED = 2
 
 FCT_TUNE_XCD = 1
+FCT_TUNE_WAVES_PER_GROUP = 2
+FCT_TUNE_VFLUX_EDGES = 3
+FCT_TUNE_ADFH_EDGES = 4
+FCT_BATCH_SLOTS = 4
 
 FCT_PHASE_LO = 1
 FCT_PHASE_GRAD = 2
@@ -35,7 +39,9 @@ FCT_PHASE_FINAL = 4
 
 EXPORTED_SYMBOLS = ("fct_abi_version", "fct_last_error", "fct_plan_create",
                     "fct_plan_get_info", "fct_plan_destroy", "fct_forward",
-                    "fct_forward_ex", "fct_forward_phase", "fct_plan_set_tuning")
+                    "fct_forward_ex", "fct_forward_phase", "fct_plan_set_tuning",
+                    "fct_batch_create", "fct_batch_set_tuning", "fct_batch_bind",
+                    "fct_batch_launch", "fct_batch_graph_launch", "fct_batch_destroy")
 
 
 class FctMeshDesc(ctypes.Structure):
@@ -106,6 +112,18 @@ def load(path: Path | None = None, *, require_torch: bool = True):
     lib.fct_forward_phase.restype = c_int32
     lib.fct_forward_phase.argtypes = [c_void_p, POINTER(FctStepArgs), c_int32, c_int32,
                                       POINTER(FctPhaseBufs), c_void_p]
+    lib.fct_batch_create.restype = c_int32
+    lib.fct_batch_create.argtypes = [POINTER(c_void_p), c_int32, POINTER(c_void_p)]
+    lib.fct_batch_set_tuning.restype = c_int32
+    lib.fct_batch_set_tuning.argtypes = [c_void_p, c_int32, c_int32]
+    lib.fct_batch_bind.restype = c_int32
+    lib.fct_batch_bind.argtypes = [c_void_p, c_int32, POINTER(FctStepArgs)]
+    lib.fct_batch_launch.restype = c_int32
+    lib.fct_batch_launch.argtypes = [c_void_p, c_int32, c_void_p]
+    lib.fct_batch_graph_launch.restype = c_int32
+    lib.fct_batch_graph_launch.argtypes = [c_void_p, c_int32, c_void_p]
+    lib.fct_batch_destroy.restype = c_int32
+    lib.fct_batch_destroy.argtypes = [c_void_p]
     ver = lib.fct_abi_version()
     if ver != FCT_ABI_VERSION:
         raise RuntimeError(f"libfesom_fct ABI {ver} != expected {FCT_ABI_VERSION}")

@@ -189,6 +189,49 @@ bool fused_mesh_ok(const fct_mesh_desc* d, const std::vector<int>& eoff,
     return true;
 }
 
+// Workspace of the phased path, allocated on the first call that takes it (the fused
+// path, which batches and most callers use, never needs it).
+int ensure_phased(fct_plan* p) {
+    if (p->LO) return FCT_OK;
+    const size_t NL = (size_t)p->m.N * p->m.L, DL = (size_t)p->m.D * p->m.L,
+                 EL = (size_t)p->m.E * p->m.L;
+    int rc;
+    if ((rc = dev_alloc(p, &p->trxy, 2 * EL)) || (rc = dev_alloc(p, &p->gnode, 2 * NL)) ||
+        (rc = dev_alloc(p, &p->adfh, DL)) || (rc = dev_alloc(p, &p->adfv, NL)) ||
+        (rc = dev_alloc(p, &p->tvmax, NL)) || (rc = dev_alloc(p, &p->tvmin, NL)) ||
+        (rc = dev_alloc(p, &p->fplus, NL)) || (rc = dev_alloc(p, &p->fminus, NL)) ||
+        (rc = dev_alloc(p, &p->pf, NL)) || (rc = dev_alloc(p, &p->mf, NL)) ||
+        (rc = dev_alloc(p, &p->LO, NL)))
+        return rc;
+    return FCT_OK;
+}
+
+int apply_tuning(FusedTuning& t, int knob, int value) {
+    switch (knob) {
+        case FCT_TUNE_XCD:
+            if (value < 1 || value > 64) return set_error(FCT_E_INVALID, "xcd %d out of range", value);
+            t.xcd = value;
+            return FCT_OK;
+        case FCT_TUNE_WAVES_PER_GROUP:
+            if (value != 1 && value != 2 && value != 4)
+                return set_error(FCT_E_INVALID, "waves per workgroup %d not in {1, 2, 4}", value);
+            t.wpb = value;
+            return FCT_OK;
+        case FCT_TUNE_VFLUX_EDGES:
+            if (value != 2 && value != 4)
+                return set_error(FCT_E_INVALID, "vflux edges per wave %d not in {2, 4}", value);
+            t.vflux_epw = value;
+            return FCT_OK;
+        case FCT_TUNE_ADFH_EDGES:
+            if (value != 1 && value != 2)
+                return set_error(FCT_E_INVALID, "adf_h edges per wave %d not in {1, 2}", value);
+            t.adfh_epw = value;
+            return FCT_OK;
+        default:
+            return set_error(FCT_E_INVALID, "unknown tuning knob %d", knob);
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -357,12 +400,7 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
         (rc = dev_upload(p, &m.areasvol, d->areasvol, NL)) ||
         (rc = dev_upload(p, &m.Zs, h_Zs.data(), h_Zs.size())) ||
         (rc = dev_upload(p, &m.zbar, d->zbar, (size_t)L)) ||
-        (rc = dev_alloc(p, &p->vflux, DL)) || (rc = dev_alloc(p, &p->LO, NL)) ||
-        (rc = dev_alloc(p, &p->trxy, 2 * EL)) || (rc = dev_alloc(p, &p->gnode, 2 * NL)) ||
-        (rc = dev_alloc(p, &p->adfh, DL)) || (rc = dev_alloc(p, &p->adfv, NL)) ||
-        (rc = dev_alloc(p, &p->tvmax, NL)) || (rc = dev_alloc(p, &p->tvmin, NL)) ||
-        (rc = dev_alloc(p, &p->fplus, NL)) || (rc = dev_alloc(p, &p->fminus, NL)) ||
-        (rc = dev_alloc(p, &p->pf, NL)) || (rc = dev_alloc(p, &p->mf, NL)) ||
+        (rc = dev_alloc(p, &p->vflux, DL)) ||
         (rc = dev_upload(p, &p->ekr, h_ekr.data(), D))) {
         free_plan(p);
         return rc;
@@ -404,14 +442,7 @@ int fct_plan_get_info(const fct_plan* p, fct_plan_info* info) {
 
 int fct_plan_set_tuning(fct_plan* p, int32_t knob, int32_t value) {
     if (!p) return set_error(FCT_E_INVALID, "NULL plan");
-    switch (knob) {
-        case FCT_TUNE_XCD:
-            if (value < 1 || value > 64) return set_error(FCT_E_INVALID, "xcd %d out of range", value);
-            p->tune.xcd = value;
-            return FCT_OK;
-        default:
-            return set_error(FCT_E_INVALID, "unknown tuning knob %d", knob);
-    }
+    return apply_tuning(p->tune, knob, value);
 }
 
 int fct_plan_destroy(fct_plan* p) {
@@ -449,7 +480,8 @@ int check_args(const fct_plan* p, const fct_step_args* a) {
 extern "C" {
 
 static int forward_phased(fct_plan* p, const fct_step_args* a, hipStream_t s) {
-    int rc;
+    int rc = ensure_phased(p);
+    if (rc) return rc;
     const DevMesh& m = p->m;
     if ((rc = launch_vflux(m, a, p->vflux, s))) return rc;
     for (int j = 0; j < a->ntracers; ++j) {
@@ -492,6 +524,7 @@ int fct_forward_phase(fct_plan* p, const fct_step_args* a, int32_t j, int32_t ph
         return set_error(FCT_E_INVALID, "tracer index %d out of range", j);
     if (!b || !b->LO || !b->tr_xy || !b->fct_plus || !b->fct_minus)
         return set_error(FCT_E_INVALID, "phase buffers must all be given");
+    if ((rc = ensure_phased(p))) return rc;
     hipStream_t s = (hipStream_t)stream;
     const DevMesh& m = p->m;
     switch (phase) {
@@ -509,6 +542,254 @@ int fct_forward_phase(fct_plan* p, const fct_step_args* a, int32_t j, int32_t ph
         default:
             return set_error(FCT_E_INVALID, "unknown phase %d", phase);
     }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batches: several plans stepped by one launch per pass and (slots, degree) class
+// ---------------------------------------------------------------------------
+namespace {
+
+struct BatchSlot {
+    bool bound = false;
+    std::vector<fct_step_args> args;        // host copies pointing into `ptrs`
+    std::vector<std::vector<double*>> ptrs;   // per instance: T, T_old, T_new arrays
+    std::vector<int> K;
+    std::vector<BatchGroup> groups;
+    std::vector<void*> allocs;              // device memory of descriptors and tables
+    hipGraphExec_t exec = nullptr;
+};
+
+}  // namespace
+
+struct fct_batch {
+    int device = -1;
+    std::vector<fct_plan*> plans;
+    FusedTuning tune;
+    BatchSlot slots[FCT_BATCH_SLOTS];
+    hipStream_t capture = nullptr;
+};
+
+namespace {
+
+void release_slot(BatchSlot& s) {
+    if (s.exec) (void)hipGraphExecDestroy(s.exec);
+    s.exec = nullptr;
+    for (void* p : s.allocs) (void)hipFree(p);
+    s.allocs.clear();
+    s.groups.clear();
+    s.bound = false;
+}
+
+template <class T>
+int slot_upload(BatchSlot& s, const T** out, const std::vector<T>& host) {
+    void* d = nullptr;
+    const size_t bytes = std::max<size_t>(1, host.size()) * sizeof(T);
+    if (hipMalloc(&d, bytes) != hipSuccess)
+        return set_error(FCT_E_NOMEM, "hipMalloc(%zu bytes) for a batch table failed", bytes);
+    s.allocs.push_back(d);
+    if (!host.empty())
+        HIPCHK(hipMemcpy(d, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
+    *out = static_cast<const T*>(d);
+    return FCT_OK;
+}
+
+// per-workgroup (instance, first wavefront) table for one pass kind
+void block_table(std::vector<int2>& blk, int inst, int64_t waves, int wpb) {
+    for (int64_t w = 0; w < waves; w += wpb) blk.push_back(make_int2(inst, (int)w));
+}
+
+int build_slot(fct_batch* b, BatchSlot& s, const fct_step_args* args) {
+    const int n = (int)b->plans.size();
+    const FusedTuning& t = b->tune;
+    const size_t dsz = fused_args_size();
+    // (slots, degree) classes in first-appearance order
+    std::vector<std::pair<int, int>> keys;
+    std::vector<int> cls(n);
+    for (int i = 0; i < n; ++i) {
+        const fct_plan* p = b->plans[i];
+        const std::pair<int, int> key(p->m.L <= 64 ? 1 : 2, p->tab.dm);
+        auto it = std::find(keys.begin(), keys.end(), key);
+        cls[i] = (int)(it - keys.begin());
+        if (it == keys.end()) keys.push_back(key);
+    }
+    for (size_t c = 0; c < keys.size(); ++c) {
+        BatchGroup g;
+        g.ns = keys[c].first;
+        g.dm = keys[c].second;
+        std::vector<unsigned char> desc;
+        std::vector<int2> bv, bn, ba;
+        for (int i = 0; i < n; ++i) {
+            if (cls[i] != (int)c) continue;
+            fct_plan* p = b->plans[i];
+            const int inst = g.count++;
+            desc.resize(desc.size() + dsz);
+            int rc = fill_fused_args(desc.data() + inst * dsz, p->m, p->tab, p->ekr, &args[i], t,
+                                     p->vflux, p->fLO, p->fgnode, p->fadfh, p->fpf, p->fmf);
+            if (rc) return rc;
+            block_table(bv, inst, vflux_waves(p->m.D, t), t.wpb);
+            block_table(bn, inst, node_waves(p->m.N, args[i].ntracers), t.wpb);
+            block_table(ba, inst, adfh_waves(p->m.D, args[i].ntracers, t), t.wpb);
+        }
+        std::vector<unsigned char> dcopy(desc);
+        const unsigned char* dd = nullptr;
+        int rc;
+        if ((rc = slot_upload(s, &dd, dcopy)) || (rc = slot_upload(s, &g.blk_v, bv)) ||
+            (rc = slot_upload(s, &g.blk_n, bn)) || (rc = slot_upload(s, &g.blk_a, ba)))
+            return rc;
+        g.desc = reinterpret_cast<const FusedArgs*>(dd);
+        g.nb_v = (unsigned)bv.size(); g.nb_n = (unsigned)bn.size(); g.nb_a = (unsigned)ba.size();
+        s.groups.push_back(g);
+    }
+    s.bound = true;
+    return FCT_OK;
+}
+
+int check_batch(fct_batch* b, int slot) {
+    if (!b) return set_error(FCT_E_INVALID, "NULL batch");
+    if (slot < 0 || slot >= FCT_BATCH_SLOTS)
+        return set_error(FCT_E_INVALID, "batch slot %d out of range", slot);
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != b->device)
+        return set_error(FCT_E_INVALID, "current device %d is not the batch's device %d", cur,
+                         b->device);
+    return FCT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fct_batch_create(fct_plan* const* plans, int32_t count, fct_batch** out) {
+    if (!out) return set_error(FCT_E_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!plans || count <= 0) return set_error(FCT_E_INVALID, "need count > 0 plans");
+    fct_batch* b = new (std::nothrow) fct_batch();
+    if (!b) return set_error(FCT_E_NOMEM, "host allocation failed");
+    for (int i = 0; i < count; ++i) {
+        fct_plan* p = plans[i];
+        if (!p) { delete b; return set_error(FCT_E_INVALID, "plan %d is NULL", i); }
+        if (p->tab.dm == 0) {
+            delete b;
+            return set_error(FCT_E_UNSUPPORTED, "plan %d cannot take the fused path", i);
+        }
+        for (int k = 0; k < i; ++k)
+            if (plans[k] == p) {
+                delete b;
+                return set_error(FCT_E_INVALID, "plan %d appears twice (instances share a workspace)", i);
+            }
+        if (i == 0) b->device = p->device;
+        else if (p->device != b->device) {
+            delete b;
+            return set_error(FCT_E_INVALID, "plans live on different devices");
+        }
+        b->plans.push_back(p);
+    }
+    if (hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return set_error(FCT_E_HIP, "hipStreamCreateWithFlags failed");
+    }
+    *out = b;
+    return FCT_OK;
+}
+
+int fct_batch_set_tuning(fct_batch* b, int32_t knob, int32_t value) {
+    if (!b) return set_error(FCT_E_INVALID, "NULL batch");
+    FusedTuning t = b->tune;
+    int rc = apply_tuning(t, knob, value);
+    if (rc) return rc;
+    b->tune = t;
+    for (int s = 0; s < FCT_BATCH_SLOTS; ++s) {
+        BatchSlot& sl = b->slots[s];
+        if (!sl.bound) continue;
+        release_slot(sl);
+        if ((rc = build_slot(b, sl, sl.args.data()))) return rc;
+    }
+    return FCT_OK;
+}
+
+int fct_batch_bind(fct_batch* b, int32_t slot, const fct_step_args* args) {
+    int rc = check_batch(b, slot);
+    if (rc) return rc;
+    if (!args) return set_error(FCT_E_INVALID, "NULL args");
+    const int n = (int)b->plans.size();
+    for (int i = 0; i < n; ++i) {
+        if ((rc = check_args(b->plans[i], &args[i]))) return rc;
+        if (args[i].w_i)
+            return set_error(FCT_E_UNSUPPORTED, "instance %d: w_i is not supported in batches", i);
+        if (args[i].ntracers > b->plans[i]->kw)
+            return set_error(FCT_E_UNSUPPORTED, "instance %d: %d tracers > plan's max_tracers %d",
+                             i, args[i].ntracers, b->plans[i]->kw);
+    }
+    BatchSlot& s = b->slots[slot];
+    release_slot(s);
+    if ((rc = build_slot(b, s, args))) { release_slot(s); return rc; }
+    // keep the arguments (and the caller's pointer arrays' contents) for re-tuning
+    s.args.assign(args, args + n);
+    s.ptrs.assign(n, {});
+    for (int i = 0; i < n; ++i) {
+        const int K = args[i].ntracers;
+        std::vector<double*>& v = s.ptrs[i];
+        v.resize(3 * (size_t)K);
+        for (int j = 0; j < K; ++j) {
+            v[j] = const_cast<double*>(args[i].T[j]);
+            v[K + j] = const_cast<double*>(args[i].T_old[j]);
+            v[2 * K + j] = args[i].T_new[j];
+        }
+        s.args[i].T = v.data();
+        s.args[i].T_old = v.data() + K;
+        s.args[i].T_new = v.data() + 2 * K;
+    }
+    return FCT_OK;
+}
+
+int fct_batch_launch(fct_batch* b, int32_t slot, void* stream) {
+    int rc = check_batch(b, slot);
+    if (rc) return rc;
+    const BatchSlot& s = b->slots[slot];
+    if (!s.bound) return set_error(FCT_E_INVALID, "batch slot %d is not bound", slot);
+    for (const BatchGroup& g : s.groups)
+        if ((rc = launch_batch_group(g, b->tune, (hipStream_t)stream))) return rc;
+    return FCT_OK;
+}
+
+int fct_batch_graph_launch(fct_batch* b, int32_t slot, void* stream) {
+    int rc = check_batch(b, slot);
+    if (rc) return rc;
+    BatchSlot& s = b->slots[slot];
+    if (!s.bound) return set_error(FCT_E_INVALID, "batch slot %d is not bound", slot);
+    if (!s.exec) {
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
+        for (const BatchGroup& g : s.groups) {
+            if ((rc = launch_batch_group(g, b->tune, b->capture))) {
+                (void)hipStreamEndCapture(b->capture, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+        }
+        HIPCHK(hipStreamEndCapture(b->capture, &graph));
+        const hipError_t e = hipGraphInstantiate(&s.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            s.exec = nullptr;
+            return set_error(FCT_E_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+        }
+    }
+    HIPCHK(hipGraphLaunch(s.exec, (hipStream_t)stream));
+    return FCT_OK;
+}
+
+int fct_batch_destroy(fct_batch* b) {
+    if (!b) return FCT_OK;
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && cur != b->device) (void)hipSetDevice(b->device);
+    for (BatchSlot& s : b->slots) release_slot(s);
+    if (b->capture) (void)hipStreamDestroy(b->capture);
+    if (cur >= 0 && cur != b->device) (void)hipSetDevice(cur);
+    delete b;
+    return FCT_OK;
 }
 
 }  // extern "C"

@@ -72,14 +72,45 @@ int launch_final(const DevMesh& m, const fct_step_args* a, int j, const double* 
                  const double* tvmin, const double* fplus, const double* fminus, double* pf,
                  double* mf, hipStream_t s);
 
-// Launch-shape knobs of the fused path (set through fct_plan_set_tuning).
+// Launch-shape knobs of the fused path (fct_plan_set_tuning / fct_batch_set_tuning);
+// results do not depend on them.
 struct FusedTuning {
-    int xcd = 8;   // remap blocks so each of `xcd` XCDs sweeps one contiguous tile range
+    int xcd = 8;        // remap blocks so each of `xcd` XCDs sweeps one contiguous tile range
+    int wpb = 4;        // wavefronts per workgroup (1, 2 or 4)
+    int vflux_epw = 4;  // edges per wavefront in kf_vflux (2 or 4)
+    int adfh_epw = 2;   // edges per wavefront in kf_adfh (1 or 2)
+};
+
+struct FusedArgs;
+
+// Batched launch of several plans' fused steps: for every pass kind, one launch over
+// the concatenated task ranges of the instances of one (slots, degree) class.
+struct BatchGroup {
+    int ns = 1, dm = 6;
+    int count = 0;                 // instances in the group
+    const FusedArgs* desc = nullptr;   // [count] device descriptors
+    const int2* blk_v = nullptr;   // per workgroup: (instance, first wave) of the vflux pass
+    const int2* blk_n = nullptr;   // ... of the node passes
+    const int2* blk_a = nullptr;   // ... of the adfh pass
+    unsigned nb_v = 0, nb_n = 0, nb_a = 0;
 };
 
 // Fused launcher (fct_fused.hip): kw = tracers per chunk (workspace slots).
 int launch_fused(const DevMesh& m, const NodeTab& tab, const int2* ekr, const fct_step_args* a,
                  int kw, const FusedTuning& tune, double* vflux, double* LO, double* gnode,
                  double* adfh, double* pf, double* mf, hipStream_t s);
+
+// Host-side descriptor of one instance's fused step (K <= kw tracers, one chunk).
+size_t fused_args_size();
+int fill_fused_args(void* out, const DevMesh& m, const NodeTab& tab, const int2* ekr,
+                    const fct_step_args* a, const FusedTuning& tune, double* vflux,
+                    double* LO, double* gnode, double* adfh, double* pf, double* mf);
+int launch_batch_group(const BatchGroup& g, const FusedTuning& tune, hipStream_t s);
+// tasks (wavefronts) of instance with D edges, N nodes, K tracers for each pass kind
+inline int64_t vflux_waves(int D, const FusedTuning& t) { return (D + t.vflux_epw - 1) / t.vflux_epw; }
+inline int64_t node_waves(int N, int K) { return (int64_t)N * K; }
+inline int64_t adfh_waves(int D, int K, const FusedTuning& t) {
+    return (int64_t)((D + t.adfh_epw - 1) / t.adfh_epw) * K;
+}
 
 }  // namespace fct

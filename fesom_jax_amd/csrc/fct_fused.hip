@@ -31,6 +31,8 @@
 // (checked on the host when the plan is built): a maximum over the triangles' vertex
 // maxima equals the maximum over that vertex set, whatever the grouping.
 
+#include <cstring>
+
 #include "fct_common.h"
 
 namespace fct {
@@ -128,39 +130,35 @@ struct FusedArgs {
     double* mf;            // [K][N][L]
 };
 
-struct Task {
-    int i, j;   // entity (node or edge), tracer
-};
-
-__device__ __forceinline__ bool get_task(int count, int K, int xcd, Task& t) {
-    int w = (int)((block_tile(xcd) * (size_t)blockDim.x + threadIdx.x) >> 6);
-    if (w >= count * K) return false;
-    w = __builtin_amdgcn_readfirstlane(w);
-    t.i = w / K;
-    t.j = w - t.i * K;
+// Wavefront -> task index. Single-instance launches number the wavefronts of the
+// (remapped) grid; batched launches look up (instance, first wavefront) per workgroup.
+__device__ __forceinline__ bool single_wave(int64_t total, int xcd, int& w) {
+    const int64_t ww = ((int64_t)block_tile(xcd) * blockDim.x + threadIdx.x) >> 6;
+    if (ww >= total) return false;
+    w = __builtin_amdgcn_readfirstlane((int)ww);
     return true;
 }
 
-// ---------------------------------------------------------------------------
-// kf_vflux: vflux on [kmin_e, kmax_e); EPW edges per wavefront, loads issued together
-// ---------------------------------------------------------------------------
-constexpr int VFLUX_EPW = 4;
+__device__ __forceinline__ int batch_wave(const int2* __restrict__ blk, int xcd, int& inst) {
+    const int2 bi = blk[block_tile(xcd)];
+    inst = __builtin_amdgcn_readfirstlane(bi.x);
+    return __builtin_amdgcn_readfirstlane(bi.y + (int)(threadIdx.x >> 6));
+}
 
-template <int NS>
-__global__ void __launch_bounds__(256)
-kf_vflux(FusedArgs a) {
+// ---------------------------------------------------------------------------
+// vflux on [kmin_e, kmax_e) for EPW consecutive edges, loads issued together
+// ---------------------------------------------------------------------------
+template <int NS, int EPW>
+__device__ __forceinline__ void vflux_group(const FusedArgs& a, int g) {
     const DevMesh& m = a.m;
-    Task t;
-    const int groups = (m.D + VFLUX_EPW - 1) / VFLUX_EPW;
-    if (!get_task(groups, 1, a.xcd, t)) return;
     const int L = m.L;
-    int e[VFLUX_EPW];
-    int2 et[VFLUX_EPW], kr[VFLUX_EPW], r1[VFLUX_EPW], r2[VFLUX_EPW];
-    double4 c[VFLUX_EPW];
+    int e[EPW];
+    int2 et[EPW], kr[EPW], r1[EPW], r2[EPW];
+    double4 c[EPW];
     int kmax = 0;
 #pragma unroll
-    for (int q = 0; q < VFLUX_EPW; ++q) {
-        const int eq = t.i * VFLUX_EPW + q;
+    for (int q = 0; q < EPW; ++q) {
+        const int eq = g * EPW + q;
         e[q] = eq < m.D ? eq : m.D - 1;
         et[q] = m.edge_tri[e[q]];
         kr[q] = eq < m.D ? a.ekr[e[q]] : make_int2(0, 0);
@@ -173,11 +171,11 @@ kf_vflux(FusedArgs a) {
     for (int s = 0; s < NS; ++s) {
         if (64 * s >= kmax) break;
         const int k = 64 * s + lane_id();
-        double2 u1[VFLUX_EPW], u2[VFLUX_EPW];
-        double h1[VFLUX_EPW], h2[VFLUX_EPW];
+        double2 u1[EPW], u2[EPW];
+        double h1[EPW], h2[EPW];
         if (k < kmax) {
 #pragma unroll
-            for (int q = 0; q < VFLUX_EPW; ++q) {
+            for (int q = 0; q < EPW; ++q) {
                 const size_t i1 = (size_t)(et[q].x >= 0 ? et[q].x : 0) * L + k;
                 const size_t i2 = (size_t)(et[q].y >= 0 ? et[q].y : 0) * L + k;
                 u1[q] = reinterpret_cast<const double2*>(a.uv)[i1];
@@ -187,7 +185,7 @@ kf_vflux(FusedArgs a) {
             }
         }
 #pragma unroll
-        for (int q = 0; q < VFLUX_EPW; ++q) {
+        for (int q = 0; q < EPW; ++q) {
             if (k < kr[q].x || k >= kr[q].y) continue;
             double t1 = 0.0, t2 = 0.0;
             if (et[q].x >= 0 && in_layer(r1[q], k))
@@ -200,20 +198,17 @@ kf_vflux(FusedArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// kf_pass1: LO and gnode
+// pass 1 of node n, tracer j: LO and gnode
 // ---------------------------------------------------------------------------
 template <int NS, int DM>
-__global__ void __launch_bounds__(256)
-kf_pass1(FusedArgs a) {
+__device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
     const DevMesh& m = a.m;
-    Task t;
-    if (!get_task(m.N, a.K, a.xcd, t)) return;
-    const int n = t.i, L = m.L;
+    const int L = m.L;
     const int2 nr = m.nlev[n];
     const int2 cnt = a.tab.cnt[n];          // (edges, elements)
-    const double* __restrict__ T = a.T[t.j];
+    const double* __restrict__ T = a.T[j];
     const size_t b0 = (size_t)n * L;
-    const size_t jo = (size_t)t.j * m.N * L;
+    const size_t jo = (size_t)j * m.N * L;
     const int kneed = nr.y + 1 < L ? nr.y + 1 : L;
     int4 es[DM];
     int4 el[DM];
@@ -295,7 +290,7 @@ kf_pass1(FusedArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// kf_adfh: adf_h(e, k) once per edge-level
+// adf_h(e, k) for EPW consecutive edges, tracer j
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double2 grad_of(const double* gs, double t0, double t1, double t2) {
     double2 g;
@@ -304,27 +299,21 @@ __device__ __forceinline__ double2 grad_of(const double* gs, double t0, double t
     return g;
 }
 
-constexpr int ADFH_EPW = 2;
-
-template <int NS>
-__global__ void __launch_bounds__(256)
-kf_adfh(FusedArgs a) {
+template <int NS, int EPW>
+__device__ __forceinline__ void adfh_group(const FusedArgs& a, int g, int j) {
     const DevMesh& m = a.m;
-    Task t;
-    const int groups = (m.D + ADFH_EPW - 1) / ADFH_EPW;
-    if (!get_task(groups, a.K, a.xcd, t)) return;
     const int L = m.L;
-    const double* __restrict__ T = a.T[t.j];
-    const double* __restrict__ To = a.Told[t.j];
-    const double2* __restrict__ gn = a.gnode + (size_t)t.j * m.N * L;
-    double* __restrict__ out = a.adfh + (size_t)t.j * m.D * L;
-    int e[ADFH_EPW], u[ADFH_EPW][3], d[ADFH_EPW][3];
-    int2 kr[ADFH_EPW], nd[ADFH_EPW], ur[ADFH_EPW], ud[ADFH_EPW];
-    double2 cc[ADFH_EPW];
+    const double* __restrict__ T = a.T[j];
+    const double* __restrict__ To = a.Told[j];
+    const double2* __restrict__ gn = a.gnode + (size_t)j * m.N * L;
+    double* __restrict__ out = a.adfh + (size_t)j * m.D * L;
+    int e[EPW], u[EPW][3], d[EPW][3];
+    int2 kr[EPW], nd[EPW], ur[EPW], ud[EPW];
+    double2 cc[EPW];
     int kmax = 0;
 #pragma unroll
-    for (int q = 0; q < ADFH_EPW; ++q) {
-        const int eq = t.i * ADFH_EPW + q;
+    for (int q = 0; q < EPW; ++q) {
+        const int eq = g * EPW + q;
         e[q] = eq < m.D ? eq : m.D - 1;
         kr[q] = eq < m.D ? a.ekr[e[q]] : make_int2(0, 0);
         nd[q] = m.edges[e[q]];
@@ -342,15 +331,14 @@ kf_adfh(FusedArgs a) {
     for (int s = 0; s < NS; ++s) {
         if (64 * s >= kmax) break;
         const int k = 64 * s + lane_id();
-        double vf[ADFH_EPW] = {}, T1[ADFH_EPW] = {}, T2[ADFH_EPW] = {}, O1[ADFH_EPW] = {},
-               O2[ADFH_EPW] = {};
-        double tu[ADFH_EPW][3] = {}, td[ADFH_EPW][3] = {};
-        double2 ga[ADFH_EPW], gb[ADFH_EPW];
+        double vf[EPW] = {}, T1[EPW] = {}, T2[EPW] = {}, O1[EPW] = {}, O2[EPW] = {};
+        double tu[EPW][3] = {}, td[EPW][3] = {};
+        double2 ga[EPW], gb[EPW];
 #pragma unroll
-        for (int q = 0; q < ADFH_EPW; ++q) { ga[q] = make_double2(0.0, 0.0); gb[q] = ga[q]; }
+        for (int q = 0; q < EPW; ++q) { ga[q] = make_double2(0.0, 0.0); gb[q] = ga[q]; }
         if (k < kmax) {
 #pragma unroll
-            for (int q = 0; q < ADFH_EPW; ++q) {
+            for (int q = 0; q < EPW; ++q) {
                 const size_t i1 = (size_t)nd[q].x * L + k, i2 = (size_t)nd[q].y * L + k;
                 vf[q] = a.vflux[(size_t)e[q] * L + k];
                 T1[q] = T[i1]; T2[q] = T[i2]; O1[q] = To[i1]; O2[q] = To[i2];
@@ -363,7 +351,7 @@ kf_adfh(FusedArgs a) {
             }
         }
 #pragma unroll
-        for (int q = 0; q < ADFH_EPW; ++q) {
+        for (int q = 0; q < EPW; ++q) {
             if (k < kr[q].x || k >= kr[q].y) continue;
             const bool tri = k >= ur[q].x && k < ur[q].y;
             const double A1 = ab2(O1[q], T1[q]), A2 = ab2(O2[q], T2[q]);
@@ -456,21 +444,21 @@ __device__ __forceinline__ void adfv_column(const FusedArgs& a, int n, int2 nr,
 }
 
 // ---------------------------------------------------------------------------
-// kf_pass2: cluster bounds, P+/P-, limiter factors
+// pass 2 of node n, tracer j: cluster bounds, P+/P-, limiter factors
 // ---------------------------------------------------------------------------
 template <int NS, int DM>
-__device__ __forceinline__ void pass2_body(const FusedArgs& a, const Task& t, int2 nr) {
+__device__ __forceinline__ void pass2_body(const FusedArgs& a, int n, int j, int2 nr) {
     const DevMesh& m = a.m;
-    const int n = t.i, L = m.L;
+    const int L = m.L;
     const int2 cnt = a.tab.cnt[n];
-    const size_t jo = (size_t)t.j * m.N * L;
+    const size_t jo = (size_t)j * m.N * L;
     const size_t b0 = (size_t)n * L;
-    const double* __restrict__ Tb = a.T[t.j];
+    const double* __restrict__ Tb = a.T[j];
     const double* __restrict__ LOb = a.LO + jo;
     const double* __restrict__ Tr = Tb + b0;
-    const double* __restrict__ Or = a.Told[t.j] + b0;
+    const double* __restrict__ Or = a.Told[j] + b0;
     const double* __restrict__ LOr = LOb + b0;
-    const double* __restrict__ adfh = a.adfh + (size_t)t.j * m.D * L;
+    const double* __restrict__ adfh = a.adfh + (size_t)j * m.D * L;
     const int kneed = nr.y + 1 < L ? nr.y + 1 : L;
     int4 es[DM];
 #pragma unroll
@@ -564,29 +552,26 @@ __device__ __forceinline__ void pass2_body(const FusedArgs& a, const Task& t, in
 }
 
 template <int NS, int DM>
-__global__ void __launch_bounds__(256)
-kf_pass2(FusedArgs a) {
-    Task t;
-    if (!get_task(a.m.N, a.K, a.xcd, t)) return;
-    const int2 nr = a.m.nlev[t.i];
-    if (NS == 1 || nr.y < 64) pass2_body<1, DM>(a, t, nr);
-    else pass2_body<NS, DM>(a, t, nr);
+__device__ __forceinline__ void pass2_task(const FusedArgs& a, int n, int j) {
+    const int2 nr = a.m.nlev[n];
+    if (NS == 1 || nr.y < 64) pass2_body<1, DM>(a, n, j, nr);
+    else pass2_body<NS, DM>(a, n, j, nr);
 }
 
 // ---------------------------------------------------------------------------
-// kf_pass3: limited fluxes, divergence, reconstruction
+// pass 3 of node n, tracer j: limited fluxes, divergence, reconstruction
 // ---------------------------------------------------------------------------
 template <int NS, int DM>
-__device__ __forceinline__ void pass3_body(const FusedArgs& a, const Task& t, int2 nr) {
+__device__ __forceinline__ void pass3_body(const FusedArgs& a, int n, int j, int2 nr) {
     const DevMesh& m = a.m;
-    const int n = t.i, L = m.L;
+    const int L = m.L;
     const int2 cnt = a.tab.cnt[n];
-    const double* __restrict__ T = a.T[t.j];
-    const double* __restrict__ To = a.Told[t.j];
-    const size_t jo = (size_t)t.j * m.N * L;
+    const double* __restrict__ T = a.T[j];
+    const double* __restrict__ To = a.Told[j];
+    const size_t jo = (size_t)j * m.N * L;
     const double* __restrict__ pf = a.pf + jo;
     const double* __restrict__ mf = a.mf + jo;
-    const double* __restrict__ adfh = a.adfh + (size_t)t.j * m.D * L;
+    const double* __restrict__ adfh = a.adfh + (size_t)j * m.D * L;
     const size_t b0 = (size_t)n * L;
     const int kneed = nr.y + 1 < L ? nr.y + 1 : L;
     int4 es[DM];
@@ -649,7 +634,7 @@ __device__ __forceinline__ void pass3_body(const FusedArgs& a, const Task& t, in
     double vl1[NS];
     col_next<NS>(vl, vl1, 0.0);
     const double* __restrict__ LO = a.LO + jo;
-    double* __restrict__ Tn = a.Tnew[t.j];
+    double* __restrict__ Tn = a.Tnew[j];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         const int k = 64 * s + lane_id();
@@ -677,25 +662,108 @@ __device__ __forceinline__ void pass3_body(const FusedArgs& a, const Task& t, in
 }
 
 template <int NS, int DM>
-__global__ void __launch_bounds__(256)
-kf_pass3(FusedArgs a) {
-    Task t;
-    if (!get_task(a.m.N, a.K, a.xcd, t)) return;
-    const int2 nr = a.m.nlev[t.i];
-    if (NS == 1 || nr.y < 64) pass3_body<1, DM>(a, t, nr);
-    else pass3_body<NS, DM>(a, t, nr);
+__device__ __forceinline__ void pass3_task(const FusedArgs& a, int n, int j) {
+    const int2 nr = a.m.nlev[n];
+    if (NS == 1 || nr.y < 64) pass3_body<1, DM>(a, n, j, nr);
+    else pass3_body<NS, DM>(a, n, j, nr);
 }
 
 // ---------------------------------------------------------------------------
-// host launcher
+// kernels: single instance (descriptor by value) and batched (descriptor array)
 // ---------------------------------------------------------------------------
-static inline unsigned blocks_for(int64_t waves) { return (unsigned)((waves + 3) / 4); }
+template <int NS, int EPW>
+__global__ void __launch_bounds__(256) kf_vflux(FusedArgs a) {
+    int w;
+    if (single_wave((a.m.D + EPW - 1) / EPW, a.xcd, w)) vflux_group<NS, EPW>(a, w);
+}
+template <int NS, int EPW>
+__global__ void __launch_bounds__(256) kb_vflux(const FusedArgs* __restrict__ d,
+                                                const int2* __restrict__ blk, int xcd) {
+    int inst;
+    const int w = batch_wave(blk, xcd, inst);
+    const FusedArgs& a = d[inst];
+    if (w < (a.m.D + EPW - 1) / EPW) vflux_group<NS, EPW>(a, w);
+}
+
+template <int NS, int EPW>
+__global__ void __launch_bounds__(256) kf_adfh(FusedArgs a) {
+    int w;
+    if (!single_wave((int64_t)((a.m.D + EPW - 1) / EPW) * a.K, a.xcd, w)) return;
+    const int g = w / a.K;
+    adfh_group<NS, EPW>(a, g, w - g * a.K);
+}
+template <int NS, int EPW>
+__global__ void __launch_bounds__(256) kb_adfh(const FusedArgs* __restrict__ d,
+                                               const int2* __restrict__ blk, int xcd) {
+    int inst;
+    const int w = batch_wave(blk, xcd, inst);
+    const FusedArgs& a = d[inst];
+    if (w >= ((a.m.D + EPW - 1) / EPW) * a.K) return;
+    const int g = w / a.K;
+    adfh_group<NS, EPW>(a, g, w - g * a.K);
+}
+
+#define FCT_NODE_KERNELS(PASS)                                                          \
+    template <int NS, int DM>                                                           \
+    __global__ void __launch_bounds__(256) kf_##PASS(FusedArgs a) {                     \
+        int w;                                                                          \
+        if (!single_wave((int64_t)a.m.N * a.K, a.xcd, w)) return;                       \
+        const int n = w / a.K;                                                          \
+        PASS##_task<NS, DM>(a, n, w - n * a.K);                                         \
+    }                                                                                   \
+    template <int NS, int DM>                                                           \
+    __global__ void __launch_bounds__(256) kb_##PASS(const FusedArgs* __restrict__ d,   \
+                                                     const int2* __restrict__ blk,      \
+                                                     int xcd) {                         \
+        int inst;                                                                       \
+        const int w = batch_wave(blk, xcd, inst);                                       \
+        const FusedArgs& a = d[inst];                                                   \
+        if (w >= a.m.N * a.K) return;                                                   \
+        const int n = w / a.K;                                                          \
+        PASS##_task<NS, DM>(a, n, w - n * a.K);                                         \
+    }
+
+FCT_NODE_KERNELS(pass1)
+FCT_NODE_KERNELS(pass2)
+FCT_NODE_KERNELS(pass3)
+#undef FCT_NODE_KERNELS
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+static inline unsigned blocks_for(int64_t waves, int wpb) {
+    return (unsigned)((waves + wpb - 1) / wpb);
+}
 
 template <int NS, int DM>
-static int launch_fused_t(FusedArgs& fa, const fct_step_args* a, int kw, hipStream_t s) {
+static void launch_single(const FusedArgs& fa, const FusedTuning& t, hipStream_t s, int stage) {
     const DevMesh& m = fa.m;
-    const int64_t vg = (m.D + VFLUX_EPW - 1) / VFLUX_EPW;
-    if (m.D) hipLaunchKernelGGL((kf_vflux<NS>), dim3(blocks_for(vg)), dim3(256), 0, s, fa);
+    const dim3 blk(64 * t.wpb);
+    const unsigned gn = blocks_for(node_waves(m.N, fa.K), t.wpb);
+    switch (stage) {
+        case 0:
+            if (t.vflux_epw == 2)
+                hipLaunchKernelGGL((kf_vflux<NS, 2>), dim3(blocks_for(vflux_waves(m.D, t), t.wpb)), blk, 0, s, fa);
+            else
+                hipLaunchKernelGGL((kf_vflux<NS, 4>), dim3(blocks_for(vflux_waves(m.D, t), t.wpb)), blk, 0, s, fa);
+            break;
+        case 1: hipLaunchKernelGGL((kf_pass1<NS, DM>), dim3(gn), blk, 0, s, fa); break;
+        case 2:
+            if (t.adfh_epw == 1)
+                hipLaunchKernelGGL((kf_adfh<NS, 1>), dim3(blocks_for(adfh_waves(m.D, fa.K, t), t.wpb)), blk, 0, s, fa);
+            else
+                hipLaunchKernelGGL((kf_adfh<NS, 2>), dim3(blocks_for(adfh_waves(m.D, fa.K, t), t.wpb)), blk, 0, s, fa);
+            break;
+        case 3: hipLaunchKernelGGL((kf_pass2<NS, DM>), dim3(gn), blk, 0, s, fa); break;
+        case 4: hipLaunchKernelGGL((kf_pass3<NS, DM>), dim3(gn), blk, 0, s, fa); break;
+    }
+}
+
+template <int NS, int DM>
+static int launch_fused_t(FusedArgs& fa, const fct_step_args* a, int kw, const FusedTuning& t,
+                          hipStream_t s) {
+    fa.K = 1;
+    if (fa.m.D) launch_single<NS, DM>(fa, t, s, 0);
     int rc = check_launch("kf_vflux");
     if (rc) return rc;
     for (int j0 = 0; j0 < a->ntracers; j0 += kw) {
@@ -705,31 +773,88 @@ static int launch_fused_t(FusedArgs& fa, const fct_step_args* a, int kw, hipStre
             const int jj = j0 + (j < K ? j : 0);
             fa.T[j] = a->T[jj]; fa.Told[j] = a->T_old[jj]; fa.Tnew[j] = a->T_new[jj];
         }
-        const unsigned gn = blocks_for((int64_t)m.N * K);
-        const unsigned gd = blocks_for((int64_t)((m.D + ADFH_EPW - 1) / ADFH_EPW) * K);
-        hipLaunchKernelGGL((kf_pass1<NS, DM>), dim3(gn), dim3(256), 0, s, fa);
-        hipLaunchKernelGGL((kf_adfh<NS>), dim3(gd), dim3(256), 0, s, fa);
-        hipLaunchKernelGGL((kf_pass2<NS, DM>), dim3(gn), dim3(256), 0, s, fa);
-        hipLaunchKernelGGL((kf_pass3<NS, DM>), dim3(gn), dim3(256), 0, s, fa);
+        for (int stage = 1; stage <= 4; ++stage) launch_single<NS, DM>(fa, t, s, stage);
         if ((rc = check_launch("fused passes"))) return rc;
     }
     return FCT_OK;
 }
 
-int launch_fused(const DevMesh& m, const NodeTab& tab, const int2* ekr, const fct_step_args* a,
-                 int kw, const FusedTuning& tune, double* vflux, double* LO, double* gnode,
-                 double* adfh, double* pf, double* mf, hipStream_t s) {
-    FusedArgs fa;
+static bool tuning_ok(const FusedTuning& t) {
+    return (t.wpb == 1 || t.wpb == 2 || t.wpb == 4) && (t.vflux_epw == 2 || t.vflux_epw == 4) &&
+           (t.adfh_epw == 1 || t.adfh_epw == 2) && t.xcd >= 1;
+}
+
+static void base_args(FusedArgs& fa, const DevMesh& m, const NodeTab& tab, const int2* ekr,
+                      const fct_step_args* a, const FusedTuning& tune, double* vflux, double* LO,
+                      double* gnode, double* adfh, double* pf, double* mf) {
     fa.m = m; fa.tab = tab; fa.vflux = vflux; fa.ekr = ekr; fa.uv = a->uv; fa.helem = a->helem;
     fa.w_e = a->w_e; fa.hnode = a->hnode; fa.hnn = a->hnode_new; fa.Z3d = a->Z3d;
     fa.zbar3 = a->zbar3; fa.dt = a->dt; fa.xcd = tune.xcd; fa.K = 0;
     fa.LO = LO; fa.gnode = (double2*)gnode; fa.adfh = adfh; fa.pf = pf; fa.mf = mf;
+}
+
+int launch_fused(const DevMesh& m, const NodeTab& tab, const int2* ekr, const fct_step_args* a,
+                 int kw, const FusedTuning& tune, double* vflux, double* LO, double* gnode,
+                 double* adfh, double* pf, double* mf, hipStream_t s) {
+    if (!tuning_ok(tune)) return set_error(FCT_E_INVALID, "bad launch-shape knobs");
+    FusedArgs fa;
+    base_args(fa, m, tab, ekr, a, tune, vflux, LO, gnode, adfh, pf, mf);
     if (kw > FCT_KCHUNK) kw = FCT_KCHUNK;
-    if (tab.dm == 6 && m.L <= 64) return launch_fused_t<1, 6>(fa, a, kw, s);
-    if (tab.dm == 6 && m.L <= 128) return launch_fused_t<2, 6>(fa, a, kw, s);
-    if (tab.dm == 8 && m.L <= 64) return launch_fused_t<1, 8>(fa, a, kw, s);
-    if (tab.dm == 8 && m.L <= 128) return launch_fused_t<2, 8>(fa, a, kw, s);
+    if (tab.dm == 6 && m.L <= 64) return launch_fused_t<1, 6>(fa, a, kw, tune, s);
+    if (tab.dm == 6 && m.L <= 128) return launch_fused_t<2, 6>(fa, a, kw, tune, s);
+    if (tab.dm == 8 && m.L <= 64) return launch_fused_t<1, 8>(fa, a, kw, tune, s);
+    if (tab.dm == 8 && m.L <= 128) return launch_fused_t<2, 8>(fa, a, kw, tune, s);
     return set_error(FCT_E_UNSUPPORTED, "fused path supports node degree <= 8 and nl <= 128");
+}
+
+size_t fused_args_size() { return sizeof(FusedArgs); }
+
+int fill_fused_args(void* out, const DevMesh& m, const NodeTab& tab, const int2* ekr,
+                    const fct_step_args* a, const FusedTuning& tune, double* vflux, double* LO,
+                    double* gnode, double* adfh, double* pf, double* mf) {
+    if (a->ntracers > FCT_KCHUNK)
+        return set_error(FCT_E_UNSUPPORTED, "batched instances take at most %d tracers", FCT_KCHUNK);
+    FusedArgs fa;
+    base_args(fa, m, tab, ekr, a, tune, vflux, LO, gnode, adfh, pf, mf);
+    fa.K = a->ntracers;
+    for (int j = 0; j < FCT_KCHUNK; ++j) {
+        const int jj = j < fa.K ? j : 0;
+        fa.T[j] = a->T[jj]; fa.Told[j] = a->T_old[jj]; fa.Tnew[j] = a->T_new[jj];
+    }
+    memcpy(out, &fa, sizeof(fa));
+    return FCT_OK;
+}
+
+template <int NS, int DM>
+static void launch_batch_t(const BatchGroup& g, const FusedTuning& t, hipStream_t s) {
+    const dim3 blk(64 * t.wpb);
+    if (g.nb_v) {
+        if (t.vflux_epw == 2)
+            hipLaunchKernelGGL((kb_vflux<NS, 2>), dim3(g.nb_v), blk, 0, s, g.desc, g.blk_v, t.xcd);
+        else
+            hipLaunchKernelGGL((kb_vflux<NS, 4>), dim3(g.nb_v), blk, 0, s, g.desc, g.blk_v, t.xcd);
+    }
+    if (g.nb_n) hipLaunchKernelGGL((kb_pass1<NS, DM>), dim3(g.nb_n), blk, 0, s, g.desc, g.blk_n, t.xcd);
+    if (g.nb_a) {
+        if (t.adfh_epw == 1)
+            hipLaunchKernelGGL((kb_adfh<NS, 1>), dim3(g.nb_a), blk, 0, s, g.desc, g.blk_a, t.xcd);
+        else
+            hipLaunchKernelGGL((kb_adfh<NS, 2>), dim3(g.nb_a), blk, 0, s, g.desc, g.blk_a, t.xcd);
+    }
+    if (g.nb_n) {
+        hipLaunchKernelGGL((kb_pass2<NS, DM>), dim3(g.nb_n), blk, 0, s, g.desc, g.blk_n, t.xcd);
+        hipLaunchKernelGGL((kb_pass3<NS, DM>), dim3(g.nb_n), blk, 0, s, g.desc, g.blk_n, t.xcd);
+    }
+}
+
+int launch_batch_group(const BatchGroup& g, const FusedTuning& t, hipStream_t s) {
+    if (!tuning_ok(t)) return set_error(FCT_E_INVALID, "bad launch-shape knobs");
+    if (g.dm == 6 && g.ns == 1) launch_batch_t<1, 6>(g, t, s);
+    else if (g.dm == 6 && g.ns == 2) launch_batch_t<2, 6>(g, t, s);
+    else if (g.dm == 8 && g.ns == 1) launch_batch_t<1, 8>(g, t, s);
+    else if (g.dm == 8 && g.ns == 2) launch_batch_t<2, 8>(g, t, s);
+    else return set_error(FCT_E_UNSUPPORTED, "batch group (%d slots, degree %d)", g.ns, g.dm);
+    return check_launch("batched fused passes");
 }
 
 }  // namespace fct

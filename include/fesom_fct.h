@@ -110,9 +110,14 @@ enum {
 };
 
 enum {
-    FCT_TUNE_XCD = 1      /* blocks remapped so each of N XCDs sweeps one contiguous
-                             range of nodes (1 = hardware order); default 8           */
+    FCT_TUNE_XCD = 1,               /* blocks remapped so each of N XCDs sweeps one contiguous
+                                       range of nodes (1 = hardware order); default 8     */
+    FCT_TUNE_WAVES_PER_GROUP = 2,   /* wavefronts per workgroup: 1, 2 or 4 (default 4)     */
+    FCT_TUNE_VFLUX_EDGES = 3,       /* edges per wavefront in the vflux pass: 2 or 4 (4)   */
+    FCT_TUNE_ADFH_EDGES = 4         /* edges per wavefront in the adf_h pass: 1 or 2 (2)   */
 };
+
+#define FCT_BATCH_SLOTS 4
 
 typedef struct fct_plan_info {
     int32_t nod2D, elem2D, edge2D, nl;
@@ -133,7 +138,9 @@ const char* fct_last_error(void);
 /* Build the device-resident plan for a mesh on the current HIP device: topology,
  * CSR node->(edge,side) lists in reference summation order, geometry and workspace.
  * Replaces the per-call mesh handling of fesom_jax/tracer_adv.py:532-578 (the Mesh
- * pytree, fesom_jax/mesh.py:62-129). Allocates device memory. */
+ * pytree, fesom_jax/mesh.py:62-129). Allocates device memory: the topology and the
+ * fused path's workspace here; the phased path's workspace on the first call that takes
+ * that path (exch hook, w_i, or meshes the fused path does not support). */
 int fct_plan_create(const fct_mesh_desc* mesh, fct_plan** out);
 
 int fct_plan_get_info(const fct_plan* plan, fct_plan_info* info);
@@ -161,6 +168,35 @@ int fct_forward_ex(fct_plan* plan, const fct_step_args* args, int32_t path, void
  * 1..4 for a tracer; phase 4 writes args->T_new[tracer]. */
 int fct_forward_phase(fct_plan* plan, const fct_step_args* args, int32_t tracer,
                       int32_t phase, const fct_phase_bufs* bufs, void* stream);
+
+/* ---- batches: several independent instances stepped together -------------------
+ * No reference counterpart: the reference steps one mesh per call. A batch groups
+ * plans (one per instance, each with its own workspace) and runs one fused FCT step of
+ * every instance with one launch per pass and (levels <= 64 or <= 128, node degree)
+ * class, over the concatenated task ranges. Results are bit-identical to fct_forward of
+ * each instance alone. */
+typedef struct fct_batch fct_batch;
+
+/* plans: `count` distinct plans on one device that can take the fused path. */
+int fct_batch_create(fct_plan* const* plans, int32_t count, fct_batch** out);
+
+/* Same knobs as fct_plan_set_tuning; rebuilds the bound slots' tables. */
+int fct_batch_set_tuning(fct_batch* batch, int32_t knob, int32_t value);
+
+/* Upload the step arguments of every instance (args[count], ntracers <= the plan's
+ * max_tracers, w_i == NULL) into slot `slot` (0 <= slot < FCT_BATCH_SLOTS).
+ * Synchronous; allocates the slot's device tables. */
+int fct_batch_bind(fct_batch* batch, int32_t slot, const fct_step_args* args);
+
+/* One step of every instance with the arguments bound in `slot`, on `stream`. Kernel
+ * launches only: no copies, no allocation. */
+int fct_batch_launch(fct_batch* batch, int32_t slot, void* stream);
+
+/* As fct_batch_launch, through a HIP graph of the slot's launches that the batch
+ * captures on first use (and again after a re-bind or a tuning change). */
+int fct_batch_graph_launch(fct_batch* batch, int32_t slot, void* stream);
+
+int fct_batch_destroy(fct_batch* batch);
 
 #ifdef __cplusplus
 }

@@ -19,6 +19,14 @@ def pytest_configure(config):
 
 
 @pytest.fixture(scope="session")
+def torch_dev():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no ROCm GPU visible")
+    return torch.device("cuda", 0)
+
+
+@pytest.fixture(scope="session")
 def pi_mesh():
     from fesom_jax_amd.mesh import load_mesh
     return load_mesh(PI_MESH_DIR)

@@ -14,13 +14,6 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-@pytest.fixture(scope="module")
-def torch_dev():
-    import torch
-    if not torch.cuda.is_available():
-        pytest.skip("no ROCm GPU visible")
-    return torch.device("cuda", 0)
-
 
 def _gpu_step(mesh, case, dev, *, j=None, path="auto", **kw):
     import torch
