@@ -41,7 +41,8 @@ EXPORTED_SYMBOLS = ("fct_abi_version", "fct_last_error", "fct_plan_create",
                     "fct_plan_get_info", "fct_plan_destroy", "fct_forward",
                     "fct_forward_ex", "fct_forward_phase", "fct_plan_set_tuning",
                     "fct_batch_create", "fct_batch_set_tuning", "fct_batch_bind",
-                    "fct_batch_launch", "fct_batch_graph_launch", "fct_batch_destroy")
+                    "fct_batch_launch", "fct_batch_graph_launch", "fct_batch_destroy",
+                    "fct_backward")
 
 
 class FctMeshDesc(ctypes.Structure):
@@ -64,6 +65,11 @@ class FctStepArgs(ctypes.Structure):
                 ("T", POINTER(c_void_p)), ("T_old", POINTER(c_void_p)),
                 ("T_new", POINTER(c_void_p)),
                 ("w_i", c_void_p), ("Z3d", c_void_p), ("zbar3", c_void_p)]
+
+
+class FctAdjointArgs(ctypes.Structure):
+    _fields_ = [("g_T_new", POINTER(c_void_p)), ("g_T", POINTER(c_void_p)),
+                ("g_T_old", POINTER(c_void_p)), ("g_uv", c_void_p), ("g_w_e", c_void_p)]
 
 
 class FctPhaseBufs(ctypes.Structure):
@@ -124,6 +130,9 @@ def load(path: Path | None = None, *, require_torch: bool = True):
     lib.fct_batch_graph_launch.argtypes = [c_void_p, c_int32, c_void_p]
     lib.fct_batch_destroy.restype = c_int32
     lib.fct_batch_destroy.argtypes = [c_void_p]
+    lib.fct_backward.restype = c_int32
+    lib.fct_backward.argtypes = [c_void_p, POINTER(FctStepArgs), POINTER(FctAdjointArgs),
+                                 c_void_p]
     ver = lib.fct_abi_version()
     if ver != FCT_ABI_VERSION:
         raise RuntimeError(f"libfesom_fct ABI {ver} != expected {FCT_ABI_VERSION}")

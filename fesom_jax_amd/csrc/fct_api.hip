@@ -60,6 +60,10 @@ struct fct_plan {
     FusedTuning tune;
     NodeTab tab;
     double *fLO = nullptr, *fgnode = nullptr, *fadfh = nullptr, *fpf = nullptr, *fmf = nullptr;
+    // adjoint workspace (allocated on the first fct_backward)
+    bool adj_ready = false;
+    AdjointWork aw{};
+    std::vector<int> host_edge_tri, host_updn;   // [2D] each, -1 sentinels (adjoint lists)
 };
 
 namespace {
@@ -203,6 +207,62 @@ int ensure_phased(fct_plan* p) {
         (rc = dev_alloc(p, &p->pf, NL)) || (rc = dev_alloc(p, &p->mf, NL)) ||
         (rc = dev_alloc(p, &p->LO, NL)))
         return rc;
+    return FCT_OK;
+}
+
+// Adjoint workspace: the phased tape plus cotangent scratch and the element-side lists
+// (element -> (edge, up/down role) for the interior edges' gradient triangles, element
+// -> (edge, side) for the edges' two triangles), in increasing edge order.
+int ensure_adjoint(fct_plan* p) {
+    if (p->adj_ready) return FCT_OK;
+    int rc = ensure_phased(p);
+    if (rc) return rc;
+    const DevMesh& m = p->m;
+    const size_t NL = (size_t)m.N * m.L, DL = (size_t)m.D * m.L, EL = (size_t)m.E * m.L;
+    AdjointWork& w = p->aw;
+    w.vflux = p->vflux; w.LO = p->LO; w.trxy = p->trxy; w.gnode = p->gnode; w.adfh = p->adfh;
+    w.adfv = p->adfv; w.tvmax = p->tvmax; w.tvmin = p->tvmin; w.fplus = p->fplus;
+    w.fminus = p->fminus; w.pf = p->pf; w.mf = p->mf;
+    double** node[] = {&w.gLO, &w.gx, &w.ghsc, &w.g_pf, &w.g_mf, &w.g_fp, &w.g_fm, &w.g_fmax,
+                       &w.g_fmin, &w.g_adfv, &w.gA, &w.g_fv, &w.g_tvx, &w.g_tvn, &w.sh_x,
+                       &w.sh_n, &w.g_div, &w.gw};
+    for (double** q : node)
+        if ((rc = dev_alloc(p, q, NL))) return rc;
+    if ((rc = dev_alloc(p, &w.g_num, 2 * NL)) || (rc = dev_alloc(p, &w.g_adfh, DL)) ||
+        (rc = dev_alloc(p, &w.eA, 2 * DL)) || (rc = dev_alloc(p, &w.eT, 2 * DL)) ||
+        (rc = dev_alloc(p, &w.eud, 4 * DL)) || (rc = dev_alloc(p, &w.g_vf, DL)) ||
+        (rc = dev_alloc(p, &w.g_tr, 2 * EL)))
+        return rc;
+    const std::vector<int>& ud = p->host_updn;
+    const std::vector<int>& et = p->host_edge_tri;
+    std::vector<int> ud_cnt(m.E + 1, 0), et_cnt(m.E + 1, 0);
+    for (int e = 0; e < m.D; ++e) {
+        if (ud[2 * e] >= 0 && ud[2 * e + 1] >= 0) {
+            ud_cnt[ud[2 * e] + 1]++;
+            ud_cnt[ud[2 * e + 1] + 1]++;
+        }
+        for (int side = 0; side < 2; ++side)
+            if (et[2 * e + side] >= 0) et_cnt[et[2 * e + side] + 1]++;
+    }
+    for (int e = 0; e < m.E; ++e) { ud_cnt[e + 1] += ud_cnt[e]; et_cnt[e + 1] += et_cnt[e]; }
+    std::vector<int> ud_ent(ud_cnt[m.E]), et_ent(et_cnt[m.E]);
+    std::vector<int> up(ud_cnt.begin(), ud_cnt.end() - 1), ep(et_cnt.begin(), et_cnt.end() - 1);
+    for (int e = 0; e < m.D; ++e) {
+        if (ud[2 * e] >= 0 && ud[2 * e + 1] >= 0) {
+            ud_ent[up[ud[2 * e]]++] = 2 * e;
+            ud_ent[up[ud[2 * e + 1]]++] = 2 * e + 1;
+        }
+        for (int side = 0; side < 2; ++side) {
+            const int el = et[2 * e + side];
+            if (el >= 0) et_ent[ep[el]++] = 2 * e + side;
+        }
+    }
+    if ((rc = dev_upload(p, &w.ud_off, ud_cnt.data(), ud_cnt.size())) ||
+        (rc = dev_upload(p, &w.ud_ent, ud_ent.data(), ud_ent.size())) ||
+        (rc = dev_upload(p, &w.et_off, et_cnt.data(), et_cnt.size())) ||
+        (rc = dev_upload(p, &w.et_ent, et_ent.data(), et_ent.size())))
+        return rc;
+    p->adj_ready = true;
     return FCT_OK;
 }
 
@@ -406,6 +466,8 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
         return rc;
     }
     p->kw = std::max(1, std::min((int)FCT_KCHUNK, d->max_tracers > 0 ? d->max_tracers : 2));
+    p->host_edge_tri.assign(d->edge_tri, d->edge_tri + 2 * (size_t)D);
+    p->host_updn.assign(d->edge_up_dn_tri, d->edge_up_dn_tri + 2 * (size_t)D);
     p->tab.dm = dm;
     if (dm && ((rc = dev_upload(p, &p->tab.cnt, h_tcnt.data(), h_tcnt.size())) ||
                (rc = dev_upload(p, &p->tab.edge, h_tedge.data(), h_tedge.size())) ||
@@ -574,6 +636,8 @@ struct fct_batch {
 namespace {
 
 void release_slot(BatchSlot& s) {
+    // launches already queued may still read the slot's tables
+    if (s.exec || !s.allocs.empty()) (void)hipDeviceSynchronize();
     if (s.exec) (void)hipGraphExecDestroy(s.exec);
     s.exec = nullptr;
     for (void* p : s.allocs) (void)hipFree(p);
@@ -790,6 +854,34 @@ int fct_batch_destroy(fct_batch* b) {
     if (cur >= 0 && cur != b->device) (void)hipSetDevice(cur);
     delete b;
     return FCT_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int fct_backward(fct_plan* p, const fct_step_args* a, const fct_adjoint_args* g, void* stream) {
+    if (!p || !a || !g) return set_error(FCT_E_INVALID, "NULL plan, forward or adjoint args");
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != p->device)
+        return set_error(FCT_E_INVALID, "current device %d is not the plan's device %d", cur,
+                         p->device);
+    if (a->ntracers <= 0 || !a->T || !a->T_old)
+        return set_error(FCT_E_INVALID, "need ntracers > 0 and T, T_old arrays");
+    if (!a->uv || !a->w_e || !a->helem || !a->hnode || !a->hnode_new)
+        return set_error(FCT_E_INVALID, "NULL flow or thickness field");
+    if (a->w_i) return set_error(FCT_E_UNSUPPORTED, "the adjoint does not support w_i");
+    if (a->Z3d || a->zbar3)
+        return set_error(FCT_E_UNSUPPORTED, "the adjoint supports the static Z stencil only");
+    if (!g->g_T_new || !g->g_T || !g->g_T_old)
+        return set_error(FCT_E_INVALID, "g_T_new, g_T and g_T_old arrays must be given");
+    for (int j = 0; j < a->ntracers; ++j)
+        if (!a->T[j] || !a->T_old[j] || !g->g_T_new[j] || !g->g_T[j] || !g->g_T_old[j])
+            return set_error(FCT_E_INVALID, "tracer %d has a NULL field", j);
+    int rc = ensure_adjoint(p);
+    if (rc) return rc;
+    return launch_backward(p->m, p->aw, a, g->g_T_new, g->g_T, g->g_T_old, g->g_uv, g->g_w_e,
+                           (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -72,6 +72,30 @@ int launch_final(const DevMesh& m, const fct_step_args* a, int j, const double* 
                  const double* tvmin, const double* fplus, const double* fminus, double* pf,
                  double* mf, hipStream_t s);
 
+int launch_fac(const DevMesh& m, const fct_step_args* a, const double* LO, const double* tvmax,
+               const double* tvmin, const double* fplus, const double* fminus, double* pf,
+               double* mf, hipStream_t s);
+
+// Workspace of the adjoint (fct_adjoint.hip): the forward tape of one tracer (dense
+// phased intermediates), cotangent scratch, and the element-side CSR lists.
+struct AdjointWork {
+    // tape
+    double *vflux, *LO, *trxy, *gnode, *adfh, *adfv, *tvmax, *tvmin, *fplus, *fminus, *pf, *mf;
+    // node scratch [N*L] (g_num is [N*L*2])
+    double *gLO, *gx, *ghsc, *g_pf, *g_mf, *g_fp, *g_fm, *g_fmax, *g_fmin, *g_adfv, *gA;
+    double *g_fv, *g_tvx, *g_tvn, *sh_x, *sh_n, *g_div, *g_num, *gw;
+    // edge scratch: g_adfh [D*L], eA and eT [D*L*2], eud [D*L*4], g_vf [D*L]
+    double *g_adfh, *eA, *eT, *eud, *g_vf;
+    // element scratch: g_tr [E*L*2]
+    double* g_tr;
+    // element -> (edge, up/down role) and element -> (edge, side) lists
+    const int *ud_off, *ud_ent, *et_off, *et_ent;
+};
+
+int launch_backward(const DevMesh& m, const AdjointWork& w, const fct_step_args* a,
+                    const double* const* g_out, double* const* gT, double* const* gTold,
+                    double* g_uv, double* g_w, hipStream_t s);
+
 // Launch-shape knobs of the fused path (fct_plan_set_tuning / fct_batch_set_tuning);
 // results do not depend on them.
 struct FusedTuning {

@@ -139,6 +139,35 @@ __device__ __forceinline__ bool single_wave(int64_t total, int xcd, int& w) {
     return true;
 }
 
+// Memory is accessed through global-address-space pointers (G). Pointers that a
+// batched kernel reads from its descriptor array are generic to the compiler, which
+// would otherwise emit flat loads for them.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> G(T* p) { return (gptr<T>)p; }
+
+// HIP's vector types are classes; load and store them as clang vectors of the same size.
+typedef int fct_iv2 __attribute__((ext_vector_type(2)));
+typedef int fct_iv4 __attribute__((ext_vector_type(4)));
+typedef double fct_dv2 __attribute__((ext_vector_type(2)));
+typedef double fct_dv4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ int2 ldv(const int2* p, size_t i) {
+    return __builtin_bit_cast(int2, ((gptr<const fct_iv2>)p)[i]);
+}
+__device__ __forceinline__ int4 ldv(const int4* p, size_t i) {
+    return __builtin_bit_cast(int4, ((gptr<const fct_iv4>)p)[i]);
+}
+__device__ __forceinline__ double2 ldv(const double2* p, size_t i) {
+    return __builtin_bit_cast(double2, ((gptr<const fct_dv2>)p)[i]);
+}
+__device__ __forceinline__ double4 ldv(const double4* p, size_t i) {
+    return __builtin_bit_cast(double4, ((gptr<const fct_dv4>)p)[i]);
+}
+__device__ __forceinline__ void stv(double2* p, size_t i, double2 v) {
+    ((gptr<fct_dv2>)p)[i] = __builtin_bit_cast(fct_dv2, v);
+}
+
 __device__ __forceinline__ int batch_wave(const int2* __restrict__ blk, int xcd, int& inst) {
     const int2 bi = blk[block_tile(xcd)];
     inst = __builtin_amdgcn_readfirstlane(bi.x);
@@ -160,11 +189,11 @@ __device__ __forceinline__ void vflux_group(const FusedArgs& a, int g) {
     for (int q = 0; q < EPW; ++q) {
         const int eq = g * EPW + q;
         e[q] = eq < m.D ? eq : m.D - 1;
-        et[q] = m.edge_tri[e[q]];
-        kr[q] = eq < m.D ? a.ekr[e[q]] : make_int2(0, 0);
-        c[q] = m.cross[e[q]];
-        r1[q] = et[q].x >= 0 ? m.elev[et[q].x] : make_int2(0, 0);
-        r2[q] = et[q].y >= 0 ? m.elev[et[q].y] : make_int2(0, 0);
+        et[q] = ldv(m.edge_tri, e[q]);
+        kr[q] = eq < m.D ? ldv(a.ekr, e[q]) : make_int2(0, 0);
+        c[q] = ldv(m.cross, e[q]);
+        r1[q] = et[q].x >= 0 ? ldv(m.elev, et[q].x) : make_int2(0, 0);
+        r2[q] = et[q].y >= 0 ? ldv(m.elev, et[q].y) : make_int2(0, 0);
         kmax = kr[q].y > kmax ? kr[q].y : kmax;
     }
 #pragma unroll
@@ -178,10 +207,10 @@ __device__ __forceinline__ void vflux_group(const FusedArgs& a, int g) {
             for (int q = 0; q < EPW; ++q) {
                 const size_t i1 = (size_t)(et[q].x >= 0 ? et[q].x : 0) * L + k;
                 const size_t i2 = (size_t)(et[q].y >= 0 ? et[q].y : 0) * L + k;
-                u1[q] = reinterpret_cast<const double2*>(a.uv)[i1];
-                u2[q] = reinterpret_cast<const double2*>(a.uv)[i2];
-                h1[q] = a.helem[i1];
-                h2[q] = a.helem[i2];
+                u1[q] = ldv(reinterpret_cast<const double2*>(a.uv), i1);
+                u2[q] = ldv(reinterpret_cast<const double2*>(a.uv), i2);
+                h1[q] = G(a.helem)[i1];
+                h2[q] = G(a.helem)[i2];
             }
         }
 #pragma unroll
@@ -192,7 +221,7 @@ __device__ __forceinline__ void vflux_group(const FusedArgs& a, int g) {
                 t1 = (u1[q].x * c[q].y - u1[q].y * c[q].x) * h1[q];
             if (et[q].y >= 0 && in_layer(r2[q], k))
                 t2 = (u2[q].y * c[q].z - u2[q].x * c[q].w) * h2[q];
-            a.vflux[(size_t)e[q] * L + k] = t1 + t2;
+            G(a.vflux)[(size_t)e[q] * L + k] = t1 + t2;
         }
     }
 }
@@ -204,9 +233,9 @@ template <int NS, int DM>
 __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
     const DevMesh& m = a.m;
     const int L = m.L;
-    const int2 nr = m.nlev[n];
-    const int2 cnt = a.tab.cnt[n];          // (edges, elements)
-    const double* __restrict__ T = a.T[j];
+    const int2 nr = ldv(m.nlev, n);
+    const int2 cnt = ldv(a.tab.cnt, n);          // (edges, elements)
+    const auto T = G(a.T[j]);
     const size_t b0 = (size_t)n * L;
     const size_t jo = (size_t)j * m.N * L;
     const int kneed = nr.y + 1 < L ? nr.y + 1 : L;
@@ -214,8 +243,8 @@ __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
     int4 el[DM];
 #pragma unroll
     for (int q = 0; q < DM; ++q) {
-        es[q] = a.tab.edge[(size_t)n * DM + (q < cnt.x ? q : 0)];
-        el[q] = a.tab.elem[(size_t)n * DM + (q < cnt.y ? q : 0)];
+        es[q] = ldv(a.tab.edge, (size_t)n * DM + (q < cnt.x ? q : 0));
+        el[q] = ldv(a.tab.elem, (size_t)n * DM + (q < cnt.y ? q : 0));
     }
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -233,7 +262,7 @@ __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
         if (wet) {
 #pragma unroll
             for (int q = 0; q < DM; ++q) {
-                vf[q] = a.vflux[(size_t)(es[q].x >> 1) * L + k];
+                vf[q] = G(a.vflux)[(size_t)(es[q].x >> 1) * L + k];
                 To[q] = T[(size_t)es[q].y * L + k];
                 tv[q][0] = T[(size_t)el[q].y * L + k];
                 tv[q][1] = T[(size_t)el[q].z * L + k];
@@ -258,41 +287,42 @@ __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
         const size_t i = b0 + k;
         double fv0, fv1 = 0.0;
         {
-            const double wk = a.w_e[i], aw = fabs(wk);
-            fv0 = (-0.5 * (Tk * (wk + aw) + Tm * (wk - aw))) * m.area[i];
+            const double wk = G(a.w_e)[i], aw = fabs(wk);
+            fv0 = (-0.5 * (Tk * (wk + aw) + Tm * (wk - aw))) * G(m.area)[i];
         }
         if (in_layer(nr, k + 1)) {
-            const double wk = a.w_e[i + 1], aw = fabs(wk);
-            fv1 = (-0.5 * (Tp * (wk + aw) + Tk * (wk - aw))) * m.area[i + 1];
+            const double wk = G(a.w_e)[i + 1], aw = fabs(wk);
+            fv1 = (-0.5 * (Tp * (wk + aw) + Tk * (wk - aw))) * G(m.area)[i + 1];
         }
-        const double av = m.areasvol[i], hn = a.hnn[i];
+        const double av = G(m.areasvol)[i], hn = G(a.hnn)[i];
         const double safe_a = av > 0.0 ? av : 1.0, safe_hn = hn > 0.0 ? hn : 1.0;
-        const double numer = Tk * a.hnode[i] + (acc + (fv0 - fv1)) * a.dt / safe_a;
+        const double numer = Tk * G(a.hnode)[i] + (acc + (fv0 - fv1)) * a.dt / safe_a;
         const double lo = numer / safe_hn;
-        a.LO[jo + i] = (hn > 0.0 && av > 0.0) ? lo : 0.0;
+        G(a.LO)[jo + i] = (hn > 0.0 && av > 0.0) ? lo : 0.0;
         // gnode: elements in increasing id
         double nx = 0.0, ny = 0.0, den = 0.0;
 #pragma unroll
         for (int q = 0; q < DM; ++q) {
             const int e = el[q].x;
             const bool use = q < cnt.y && in_bytes(es[q].w, 0, k);
-            const double* gs = m.gs + 6 * (size_t)e;
+            const auto gs = G(m.gs) + 6 * (size_t)e;
             const double gx = gs[0] * tv[q][0] + gs[1] * tv[q][1] + gs[2] * tv[q][2];
             const double gy = gs[3] * tv[q][0] + gs[4] * tv[q][1] + gs[5] * tv[q][2];
-            const double ea = m.earea[e];
+            const double ea = G(m.earea)[e];
             nx = nx + (use ? gx * ea : 0.0);
             ny = ny + (use ? gy * ea : 0.0);
             den = den + (use ? ea : 0.0);
         }
         const double d = den > 0.0 ? den : 1.0;
-        a.gnode[jo + i] = make_double2(nx / d, ny / d);
+        stv(a.gnode, jo + i, make_double2(nx / d, ny / d));
     }
 }
 
 // ---------------------------------------------------------------------------
 // adf_h(e, k) for EPW consecutive edges, tracer j
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double2 grad_of(const double* gs, double t0, double t1, double t2) {
+__device__ __forceinline__ double2 grad_of(gptr<const double> gs, double t0, double t1,
+                                           double t2) {
     double2 g;
     g.x = gs[0] * t0 + gs[1] * t1 + gs[2] * t2;
     g.y = gs[3] * t0 + gs[4] * t1 + gs[5] * t2;
@@ -303,10 +333,10 @@ template <int NS, int EPW>
 __device__ __forceinline__ void adfh_group(const FusedArgs& a, int g, int j) {
     const DevMesh& m = a.m;
     const int L = m.L;
-    const double* __restrict__ T = a.T[j];
-    const double* __restrict__ To = a.Told[j];
-    const double2* __restrict__ gn = a.gnode + (size_t)j * m.N * L;
-    double* __restrict__ out = a.adfh + (size_t)j * m.D * L;
+    const auto T = G(a.T[j]);
+    const auto To = G(a.Told[j]);
+    const double2* gn = a.gnode + (size_t)j * m.N * L;
+    const auto out = G(a.adfh + (size_t)j * m.D * L);
     int e[EPW], u[EPW][3], d[EPW][3];
     int2 kr[EPW], nd[EPW], ur[EPW], ud[EPW];
     double2 cc[EPW];
@@ -315,15 +345,15 @@ __device__ __forceinline__ void adfh_group(const FusedArgs& a, int g, int j) {
     for (int q = 0; q < EPW; ++q) {
         const int eq = g * EPW + q;
         e[q] = eq < m.D ? eq : m.D - 1;
-        kr[q] = eq < m.D ? a.ekr[e[q]] : make_int2(0, 0);
-        nd[q] = m.edges[e[q]];
-        ur[q] = m.utri_rng[e[q]];
-        ud[q] = m.updn[e[q]];
-        cc[q] = m.edge_c[e[q]];
+        kr[q] = eq < m.D ? ldv(a.ekr, e[q]) : make_int2(0, 0);
+        nd[q] = ldv(m.edges, e[q]);
+        ur[q] = ldv(m.utri_rng, e[q]);
+        ud[q] = ldv(m.updn, e[q]);
+        cc[q] = ldv(m.edge_c, e[q]);
 #pragma unroll
         for (int v = 0; v < 3; ++v) {
-            u[q][v] = m.en[3 * ud[q].x + v];
-            d[q][v] = m.en[3 * ud[q].y + v];
+            u[q][v] = G(m.en)[3 * ud[q].x + v];
+            d[q][v] = G(m.en)[3 * ud[q].y + v];
         }
         kmax = kr[q].y > kmax ? kr[q].y : kmax;
     }
@@ -340,9 +370,9 @@ __device__ __forceinline__ void adfh_group(const FusedArgs& a, int g, int j) {
 #pragma unroll
             for (int q = 0; q < EPW; ++q) {
                 const size_t i1 = (size_t)nd[q].x * L + k, i2 = (size_t)nd[q].y * L + k;
-                vf[q] = a.vflux[(size_t)e[q] * L + k];
+                vf[q] = G(a.vflux)[(size_t)e[q] * L + k];
                 T1[q] = T[i1]; T2[q] = T[i2]; O1[q] = To[i1]; O2[q] = To[i2];
-                ga[q] = gn[i1]; gb[q] = gn[i2];
+                ga[q] = ldv(gn, i1); gb[q] = ldv(gn, i2);
 #pragma unroll
                 for (int v = 0; v < 3; ++v) {
                     tu[q][v] = T[(size_t)u[q][v] * L + k];
@@ -355,8 +385,8 @@ __device__ __forceinline__ void adfh_group(const FusedArgs& a, int g, int j) {
             if (k < kr[q].x || k >= kr[q].y) continue;
             const bool tri = k >= ur[q].x && k < ur[q].y;
             const double A1 = ab2(O1[q], T1[q]), A2 = ab2(O2[q], T2[q]);
-            const double* gsu = m.gs + 6 * (size_t)ud[q].x;
-            const double* gsd = m.gs + 6 * (size_t)ud[q].y;
+            const auto gsu = G(m.gs) + 6 * (size_t)ud[q].x;
+            const auto gsd = G(m.gs) + 6 * (size_t)ud[q].y;
             const double2 gu = grad_of(gsu, tu[q][0], tu[q][1], tu[q][2]);
             const double2 gd = grad_of(gsd, td[q][0], td[q][1], td[q][2]);
             const double g0 = tri ? gu.x : ga[q].x, g1 = tri ? gd.x : gb[q].x;
@@ -380,8 +410,8 @@ __device__ __forceinline__ void adfh_group(const FusedArgs& a, int g, int j) {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double adfv_at(const DevMesh& m, int k, int2 nr, size_t i, size_t b0,
                                           double A2, double A1, double A, double Ap, double Tm,
-                                          double Tk, double W, double ar, const double* Z3d,
-                                          const double* zbar3) {
+                                          double Tk, double W, double ar,
+                                          gptr<const double> Z3d, gptr<const double> zbar3) {
     const int L = m.L;
     double hi = 0.0;
     if (in_iface(nr, k)) {
@@ -399,8 +429,8 @@ __device__ __forceinline__ double adfv_at(const DevMesh& m, int k, int2 nr, size
                 Znz = Z3d[i]; Zm1 = Z3d[b0 + km1]; Zm2 = Z3d[b0 + km2]; Zp1 = Z3d[b0 + kp1];
                 zb = zbar3[i];
             } else {
-                Znz = m.Zs[k]; Zm1 = m.Zs[L + k]; Zm2 = m.Zs[2 * L + k]; Zp1 = m.Zs[3 * L + k];
-                zb = m.zbar[k];
+                Znz = G(m.Zs)[k]; Zm1 = G(m.Zs)[L + k]; Zm2 = G(m.Zs)[2 * L + k]; Zp1 = G(m.Zs)[3 * L + k];
+                zb = G(m.zbar)[k];
             }
             const double qc = (A1 - A) / nz_safe(Zm1 - Znz);
             const double qu = (A - Ap) / nz_safe(Znz - Zp1);
@@ -438,7 +468,7 @@ __device__ __forceinline__ void adfv_column(const FusedArgs& a, int n, int2 nr,
         if (k <= nr.y && k < L) {
             const size_t i = b0 + k;
             out[s] = adfv_at(m, k, nr, i, b0, A2[s], A1[s], Ac[s], Ap[s], Tm[s], Tc[s],
-                             a.w_e[i], m.area[i], a.Z3d, a.zbar3);
+                             G(a.w_e)[i], G(m.area)[i], G(a.Z3d), G(a.zbar3));
         }
     }
 }
@@ -450,19 +480,19 @@ template <int NS, int DM>
 __device__ __forceinline__ void pass2_body(const FusedArgs& a, int n, int j, int2 nr) {
     const DevMesh& m = a.m;
     const int L = m.L;
-    const int2 cnt = a.tab.cnt[n];
+    const int2 cnt = ldv(a.tab.cnt, n);
     const size_t jo = (size_t)j * m.N * L;
     const size_t b0 = (size_t)n * L;
-    const double* __restrict__ Tb = a.T[j];
-    const double* __restrict__ LOb = a.LO + jo;
-    const double* __restrict__ Tr = Tb + b0;
-    const double* __restrict__ Or = a.Told[j] + b0;
-    const double* __restrict__ LOr = LOb + b0;
-    const double* __restrict__ adfh = a.adfh + (size_t)j * m.D * L;
+    const auto Tb = G(a.T[j]);
+    const auto LOb = G(a.LO + jo);
+    const auto Tr = G(Tb + b0);
+    const auto Or = G(a.Told[j] + b0);
+    const auto LOr = G(LOb + b0);
+    const auto adfh = G(a.adfh + (size_t)j * m.D * L);
     const int kneed = nr.y + 1 < L ? nr.y + 1 : L;
     int4 es[DM];
 #pragma unroll
-    for (int q = 0; q < DM; ++q) es[q] = a.tab.edge[(size_t)n * DM + (q < cnt.x ? q : 0)];
+    for (int q = 0; q < DM; ++q) es[q] = ldv(a.tab.edge, (size_t)n * DM + (q < cnt.x ? q : 0));
     double Tc[NS], Ac[NS], LOc[NS], tvx[NS], tvn[NS], ap[NS], am[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -541,19 +571,19 @@ __device__ __forceinline__ void pass2_body(const FusedArgs& a, int n, int j, int
         const double neg_v = fmin(fvc[s], 0.0) + fmin(-fvb[s], 0.0);
         const double fplus = pos_v + ap[s];
         const double fminus = neg_v + am[s];
-        const double av = m.areasvol[i], hn = a.hnn[i];
+        const double av = G(m.areasvol)[i], hn = G(a.hnn)[i];
         const double safe_a = av > 0.0 ? av : 1.0, safe_hn = hn > 0.0 ? hn : 1.0;
         const double flux_pos = fplus * a.dt / safe_a / safe_hn + FCT_FLUX_EPS;
         const double flux_neg = fminus * a.dt / safe_a / safe_hn - FCT_FLUX_EPS;
         const bool valid = av > 0.0 && hn > 0.0;
-        a.pf[jo + i] = valid ? fmin(1.0, ttf_max / flux_pos) : 1.0;
-        a.mf[jo + i] = valid ? fmin(1.0, ttf_min / flux_neg) : 1.0;
+        G(a.pf)[jo + i] = valid ? fmin(1.0, ttf_max / flux_pos) : 1.0;
+        G(a.mf)[jo + i] = valid ? fmin(1.0, ttf_min / flux_neg) : 1.0;
     }
 }
 
 template <int NS, int DM>
 __device__ __forceinline__ void pass2_task(const FusedArgs& a, int n, int j) {
-    const int2 nr = a.m.nlev[n];
+    const int2 nr = ldv(a.m.nlev, n);
     if (NS == 1 || nr.y < 64) pass2_body<1, DM>(a, n, j, nr);
     else pass2_body<NS, DM>(a, n, j, nr);
 }
@@ -565,18 +595,18 @@ template <int NS, int DM>
 __device__ __forceinline__ void pass3_body(const FusedArgs& a, int n, int j, int2 nr) {
     const DevMesh& m = a.m;
     const int L = m.L;
-    const int2 cnt = a.tab.cnt[n];
-    const double* __restrict__ T = a.T[j];
-    const double* __restrict__ To = a.Told[j];
+    const int2 cnt = ldv(a.tab.cnt, n);
+    const auto T = G(a.T[j]);
+    const auto To = G(a.Told[j]);
     const size_t jo = (size_t)j * m.N * L;
-    const double* __restrict__ pf = a.pf + jo;
-    const double* __restrict__ mf = a.mf + jo;
-    const double* __restrict__ adfh = a.adfh + (size_t)j * m.D * L;
+    const auto pf = G(a.pf + jo);
+    const auto mf = G(a.mf + jo);
+    const auto adfh = G(a.adfh + (size_t)j * m.D * L);
     const size_t b0 = (size_t)n * L;
     const int kneed = nr.y + 1 < L ? nr.y + 1 : L;
     int4 es[DM];
 #pragma unroll
-    for (int q = 0; q < DM; ++q) es[q] = a.tab.edge[(size_t)n * DM + (q < cnt.x ? q : 0)];
+    for (int q = 0; q < DM; ++q) es[q] = ldv(a.tab.edge, (size_t)n * DM + (q < cnt.x ? q : 0));
     double Tc[NS], Ac[NS], Pc[NS], Mc[NS], acc[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -633,8 +663,8 @@ __device__ __forceinline__ void pass3_body(const FusedArgs& a, int n, int j, int
     }
     double vl1[NS];
     col_next<NS>(vl, vl1, 0.0);
-    const double* __restrict__ LO = a.LO + jo;
-    double* __restrict__ Tn = a.Tnew[j];
+    const auto LO = G(a.LO + jo);
+    const auto Tn = G(a.Tnew[j]);
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         const int k = 64 * s + lane_id();
@@ -645,7 +675,7 @@ __device__ __forceinline__ void pass3_body(const FusedArgs& a, int n, int j, int
             Tn[i] = Ti + 0.0;
             continue;
         }
-        const double av = m.areasvol[i], hn = a.hnn[i], hnd = a.hnode[i];
+        const double av = G(m.areasvol)[i], hn = G(a.hnn)[i], hnd = G(a.hnode)[i];
         const double safe_a = av > 0.0 ? av : 1.0, safe_hn = hn > 0.0 ? hn : 1.0;
         const double Ti = Tc[s];
         const double lo_trans = -Ti * hnd + LO[i] * hn;
@@ -663,7 +693,7 @@ __device__ __forceinline__ void pass3_body(const FusedArgs& a, int n, int j, int
 
 template <int NS, int DM>
 __device__ __forceinline__ void pass3_task(const FusedArgs& a, int n, int j) {
-    const int2 nr = a.m.nlev[n];
+    const int2 nr = ldv(a.m.nlev, n);
     if (NS == 1 || nr.y < 64) pass3_body<1, DM>(a, n, j, nr);
     else pass3_body<NS, DM>(a, n, j, nr);
 }

@@ -465,12 +465,21 @@ int launch_pm(const DevMesh& m, const fct_step_args* a, int j, const double* vfl
     return check_launch("phase 3 kernels");
 }
 
+int launch_fac(const DevMesh& m, const fct_step_args* a, const double* LO, const double* tvmax,
+               const double* tvmin, const double* fplus, const double* fminus, double* pf,
+               double* mf, hipStream_t s) {
+    FCT_LAUNCH(k_fac, (int64_t)m.N * m.L, s, m, LO, tvmax, tvmin, fplus, fminus, a->hnode_new,
+               a->dt, pf, mf);
+    return check_launch("k_fac");
+}
+
 int launch_final(const DevMesh& m, const fct_step_args* a, int j, const double* LO,
                  const double* adfh, const double* adfv, const double* tvmax,
                  const double* tvmin, const double* fplus, const double* fminus, double* pf,
                  double* mf, hipStream_t s) {
     const int64_t nl = (int64_t)m.N * m.L;
-    FCT_LAUNCH(k_fac, nl, s, m, LO, tvmax, tvmin, fplus, fminus, a->hnode_new, a->dt, pf, mf);
+    int rc = launch_fac(m, a, LO, tvmax, tvmin, fplus, fminus, pf, mf, s);
+    if (rc) return rc;
     FCT_LAUNCH(k_final, nl, s, m, adfh, adfv, pf, mf, a->T[j], LO, a->hnode, a->hnode_new,
                a->dt, a->T_new[j]);
     return check_launch("phase 4 kernels");

@@ -169,6 +169,27 @@ int fct_forward_ex(fct_plan* plan, const fct_step_args* args, int32_t path, void
 int fct_forward_phase(fct_plan* plan, const fct_step_args* args, int32_t tracer,
                       int32_t phase, const fct_phase_bufs* bufs, void* stream);
 
+/* ---- reverse mode ------------------------------------------------------------------
+ * Cotangents of one FCT step: for the cotangents g_T_new[j] of T_new[j], returns
+ * g_T[j], g_T_old[j] and (summed over the tracers) g_uv and g_w_e, with the conventions
+ * of jax.grad applied to fesom_jax/tracer_adv.py:532-578 (the reference's backward is
+ * jax.grad of the forward, fesom_jax/docs/LIMITER_GRADIENTS.md:45-75: max/min send the
+ * cotangent to the winner, half each on a tie; max/min reductions share it among tied
+ * arguments; abs'(0) = +1). The forward intermediates are recomputed, not stored.
+ * Supports w_i == NULL and the static QR4C stencil (Z3d == NULL). The first call
+ * allocates the adjoint workspace; later calls allocate nothing. */
+typedef struct fct_adjoint_args {
+    const double* const* g_T_new;  /* ntracers x [nod2D][nl], device */
+    double* const* g_T;            /* out: ntracers x [nod2D][nl] */
+    double* const* g_T_old;        /* out: ntracers x [nod2D][nl] */
+    double* g_uv;                  /* out: [elem2D][nl][2] or NULL (not computed) */
+    double* g_w_e;                 /* out: [nod2D][nl] or NULL (not computed) */
+} fct_adjoint_args;
+
+/* fwd: the step's forward arguments (T_new is not read and may be NULL). */
+int fct_backward(fct_plan* plan, const fct_step_args* fwd, const fct_adjoint_args* adj,
+                 void* stream);
+
 /* ---- batches: several independent instances stepped together -------------------
  * No reference counterpart: the reference steps one mesh per call. A batch groups
  * plans (one per instance, each with its own workspace) and runs one fused FCT step of
