@@ -76,10 +76,13 @@ def main():
     ap.add_argument("--dt", type=float, default=2700.0)
     ap.add_argument("--cpu-level", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("plain", "batched"), default="plain",
+    ap.add_argument("--workload", choices=("plain", "batched", "serve"), default="plain",
                     help="plain: one icosa-9 instance (the metric); batched: 64 uneven "
                          "instances in one batched launch vs 64 sequential plain launches")
     ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--rate", type=float, default=20.0, help="serve: requests per second")
+    ap.add_argument("--duration", type=float, default=10.0, help="serve: seconds of arrivals")
+    ap.add_argument("--levels", type=str, default="4,7", help="serve: icosa level range")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -100,6 +103,8 @@ def main():
     dev = torch.device("cuda", 0)
     if args.workload == "batched":
         return run_batched(args, dev)
+    if args.workload == "serve":
+        return run_serve(args, dev)
     K = args.tracers
 
     t0 = time.time()
@@ -269,6 +274,47 @@ def run_batched(args, dev):
         "cpu_baseline": None,
     }
     batch.close()
+    print(json.dumps(out), flush=True)
+
+
+def run_serve(args, dev):
+    """A Poisson stream of forward (70%) and forward+adjoint (30%) requests of 1-24 steps,
+    2-4 tracers, nl 47/70, icosa levels in --levels, served with continuous batching
+    (200 GB HBM budget) and, separately, by the plain per-request FIFO loop."""
+    import torch
+    from fesom_jax_amd.serve import MeshCache, Server, fifo_loop, make_stream
+    lo, hi = (int(x) for x in args.levels.split(","))
+    specs = make_stream(args.rate, args.duration, levels=(lo, hi), seed=args.seed)
+    meshes = MeshCache(seed=args.seed)
+    t0 = time.time()
+    for lv in range(lo, hi + 1):
+        for nl in (47, 70):
+            meshes.get(lv, nl)
+    log(f"{len(specs)} requests over {args.duration}s; meshes ready ({time.time() - t0:.1f}s)")
+    # warm-up: one small request through each path (compiles nothing; loads the library)
+    warm = make_stream(5.0, 0.5, levels=(lo, lo), steps=(1, 2), seed=args.seed + 1)[:2]
+    Server(dev, meshes=meshes).run(warm)
+    fifo_loop(warm, dev, meshes=meshes)
+    torch.cuda.synchronize()
+    srv = Server(dev, budget_bytes=200 << 30, max_active=64, meshes=meshes)
+    served = srv.run(specs)
+    log(f"served: {json.dumps(served)}")
+    torch.cuda.empty_cache()
+    fifo = fifo_loop(specs, dev, meshes=meshes)
+    fifo.pop("results")
+    log(f"fifo: {json.dumps(fifo)}")
+    out = {
+        "metric": "FP64 FCT served p50/p99 latency and node-level updates/s vs plain FIFO loop",
+        "value": served["updates_per_s"], "unit": "node-level updates/s", "n_gpus": 1,
+        "steps": None, "warmup": None, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"Poisson {args.rate}/s for {args.duration}s, icosa {lo}-{hi}, "
+                               "nl 47/70, 2-4 tracers, 1-24 steps, 30% with adjoint",
+                   "requests": len(specs), "hbm_budget_GB": 200, "parallelism": "single GPU"},
+        "served": served, "fifo_plain_loop": fifo,
+        "speedup_updates_per_s": served["updates_per_s"] / fifo["updates_per_s"],
+        "roofline": None, "cpu_baseline": None,
+    }
     print(json.dumps(out), flush=True)
 
 

@@ -41,14 +41,16 @@ class Instance:
     _keep: list = field(default_factory=list, repr=False)
 
     @classmethod
-    def from_case(cls, mesh, case, device, *, max_tracers=None):
+    def from_case(cls, mesh, case, device, *, max_tracers=None, plan=None):
         """Upload a case dict (``uv, w_e, helem, hnode, hnode_new, T, T_old, dt``; numpy
-        or torch) and build a private plan for it."""
+        or torch) and build a private plan for it (or take ``plan``, which must be
+        built for this mesh and not be used by another live instance)."""
         torch = _torch()
         t = lambda a: (a if isinstance(a, torch.Tensor) else torch.from_numpy(a)).to(
             device=device, dtype=torch.float64).contiguous()
         K = len(case["T"])
-        plan = FctPlan(mesh, device, max_tracers=max_tracers or K)
+        if plan is None:
+            plan = FctPlan(mesh, device, max_tracers=max_tracers or K)
         bufs = []
         for T, To in zip(case["T"], case["T_old"]):
             Tt, Tot = t(T), t(To)
