@@ -76,7 +76,8 @@ def main():
     ap.add_argument("--dt", type=float, default=2700.0)
     ap.add_argument("--cpu-level", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("plain", "batched", "serve"), default="plain",
+    ap.add_argument("--workload", choices=("plain", "batched", "serve", "adjoint"),
+                    default="plain",
                     help="plain: one icosa-9 instance (the metric); batched: 64 uneven "
                          "instances in one batched launch vs 64 sequential plain launches")
     ap.add_argument("--instances", type=int, default=64)
@@ -105,6 +106,8 @@ def main():
         return run_batched(args, dev)
     if args.workload == "serve":
         return run_serve(args, dev)
+    if args.workload == "adjoint":
+        return run_adjoint(args, dev)
     K = args.tracers
 
     t0 = time.time()
@@ -291,8 +294,12 @@ def run_serve(args, dev):
         for nl in (47, 70):
             meshes.get(lv, nl)
     log(f"{len(specs)} requests over {args.duration}s; meshes ready ({time.time() - t0:.1f}s)")
-    # warm-up: one small request through each path (compiles nothing; loads the library)
-    warm = make_stream(5.0, 0.5, levels=(lo, lo), steps=(1, 2), seed=args.seed + 1)[:2]
+    # warm-up, untimed: one forward+adjoint request per (level, nl) through both paths,
+    # so that base inputs, plans and adjoint workspaces exist before either is timed
+    from fesom_jax_amd.serve import RequestSpec
+    warm = [RequestSpec(rid=i, arrival=0.0, level=lv, nl=nl, K=2, nsteps=2, adjoint=True,
+                        seed=i) for i, (lv, nl) in enumerate(
+                            (lv, nl) for lv in range(lo, hi + 1) for nl in (47, 70))]
     Server(dev, meshes=meshes).run(warm)
     fifo_loop(warm, dev, meshes=meshes)
     torch.cuda.synchronize()
@@ -313,6 +320,101 @@ def run_serve(args, dev):
                    "requests": len(specs), "hbm_budget_GB": 200, "parallelism": "single GPU"},
         "served": served, "fifo_plain_loop": fifo,
         "speedup_updates_per_s": served["updates_per_s"] / fifo["updates_per_s"],
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_adjoint(args, dev):
+    """12 chained steps on icosa-7 x nl=47 with 2 tracers, loss = sum over wet surface
+    nodes of (T_final - T_target)^2 (T_target = T0 + 0.2 N(0,1)), gradients with respect
+    to the initial T, S and uv through the hand-written adjoint (per-step recompute).
+    Times forward and forward+backward, and checks the gradient against central
+    differences of the GPU forward along 20 seeded directions (h = 1e-6 relative)."""
+    import torch
+    from fesom_jax_amd.adjoint import rollout
+    from fesom_jax_amd.inputs import make_case_device
+    from fesom_jax_amd.meshgen import icosahedral_mesh
+    from fesom_jax_amd.perfmodel import updates_per_step
+    level, nl, K, nsteps = 7, 47, 2, 12
+    mesh = icosahedral_mesh(level, nl, seed=args.seed)
+    case = make_case_device(mesh, dev, seed=args.seed + 1, dt=args.dt, tracers=("T", "S"))
+    g = torch.Generator(device=dev).manual_seed(args.seed)
+    f64 = dict(dtype=torch.float64, device=dev)
+    nlm = torch.from_numpy(mesh.node_layer_mask).to(dev)
+    zero = torch.zeros((), **f64)
+    target = [t + torch.where(nlm, 0.2 * torch.randn(t.shape, generator=g, **f64), zero)
+              for t in case["T"]]
+    surf = nlm[:, 0].to(torch.float64)
+    fixed = (case["w_e"], case["helem"], case["hnode"], case["hnode_new"])
+
+    def loss_of(T0, To0, uv):
+        fin = rollout(mesh, uv, *fixed, T0, To0, nsteps, dt=args.dt)
+        return sum((((f[:, 0] - tg[:, 0]) ** 2) * surf).sum() for f, tg in zip(fin, target))
+
+    T0 = [t.clone().requires_grad_(True) for t in case["T"]]
+    To0 = [t.clone() for t in case["T"]]
+    uv = case["uv"].clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        loss_of([t.detach() for t in T0], To0, uv.detach())
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    L0 = loss_of(T0, To0, uv)
+    grads = torch.autograd.grad(L0, [*T0, uv])
+    L0 = float(L0.detach())
+    torch.cuda.synchronize()
+    t_fb = time.perf_counter() - t0
+    log(f"forward {t_fwd:.3f}s, forward+adjoint {t_fb:.3f}s, loss {float(L0):.6e}")
+    # central differences along 20 seeded directions, each on one random column (a wet
+    # node's T or S column, or a triangle's uv column) near the surface signal, so that
+    # few limiter switches lie on the segment; step h = 1e-6 * max|x| (and 1e-5, 1e-7:
+    # the best agreement of the three is kept, which skips a switch inside one stencil)
+    rel = []
+    rng = np.random.default_rng(args.seed + 7)
+    base = [x.detach() for x in T0] + [uv.detach()]
+    wet_nodes = np.nonzero(mesh.node_layer_mask[:, 0])[0]
+    with torch.no_grad():
+        for r in range(20):
+            which = r % 3
+            d = torch.zeros_like(base[which])
+            if which < 2:
+                nd = int(rng.choice(wet_nodes))
+                col = rng.standard_normal(mesh.nl) * mesh.node_layer_mask[nd]
+                d[nd] = torch.from_numpy(col).to(dev)
+            else:
+                el = int(rng.integers(mesh.elem2D))
+                col = rng.standard_normal((mesh.nl, 2)) * mesh.elem_layer_mask[el][:, None]
+                d[el] = torch.from_numpy(col).to(dev)
+            ad = float((grads[which] * d).sum())
+            best = np.inf
+            for hr in (1e-6, 1e-5, 1e-7):
+                h = hr * float(base[which].abs().max())
+                def at(sign):
+                    xs = [b.clone() for b in base]
+                    xs[which] = xs[which] + sign * h * d
+                    return float(loss_of(xs[:2], To0, xs[2]))
+                fd = (at(1.0) - at(-1.0)) / (2 * h)
+                best = min(best, abs(fd - ad) / max(abs(fd), abs(ad), 1e-300))
+            rel.append(best)
+    rel = np.array(rel)
+    log(f"FD check: median rel {np.median(rel):.2e}, max {rel.max():.2e}, "
+        f"{int((rel < 1e-5).sum())}/20 below 1e-5")
+    U = updates_per_step(mesh, K) * nsteps
+    out = {
+        "metric": "FP64 FCT forward+adjoint rollout (12 steps) node-level updates/s",
+        "value": U / t_fb, "unit": "node-level updates/s (forward updates / forward+backward time)",
+        "n_gpus": 1, "steps": nsteps, "warmup": 0, "ms_per_step": 1e3 * t_fb / nsteps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"icosa-{level} x nl={nl}, {K} tracers, {nsteps} steps + adjoint",
+                   "nodes": mesh.nod2D, "updates_forward": U},
+        "forward_s": t_fwd, "forward_backward_s": t_fb, "backward_over_forward": (t_fb - t_fwd) / t_fwd,
+        "fd_check": {"directions": 20, "kind": "one random column each, h in 1e-6/1e-5/1e-7",
+                     "median_rel": float(np.median(rel)),
+                     "max_rel": float(rel.max()), "below_1e-5": int((rel < 1e-5).sum())},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

@@ -12,8 +12,8 @@
 //   x / y: g / y to x and (-g * x) * (1 / (y * y)) to y.
 // Every transpose of a scatter is a gather over a CSR list (node -> (edge, side),
 // node -> element, element -> (edge, side), element -> (edge, up/down role)), so the
-// result is deterministic. Flat kernels run one thread per (entity, level); kernels
-// that couple levels run one thread per column.
+// result is deterministic. Every kernel runs one thread per (entity, level); terms that
+// couple levels are gathered by the receiving level (no thread writes another's lane).
 
 #include "fct_common.h"
 
@@ -60,39 +60,56 @@ ka_final(DevMesh m, const double* __restrict__ g_out, const double* __restrict__
     ghsc[i] = g_in / safe_a * dt;
 }
 
-// ---- limited vertical flux (tracer_adv.py:453-459), one thread per column --------
+// ---- limited vertical flux (tracer_adv.py:453-459) -----------------------------------
+// cotangent of the limited flux at interface kk (g_vl = gx[kk] - gx[kk-1]), its factor
+// ae and g_vl * adf_v
+__device__ __forceinline__ void vlim_at(const double* gx, const double* adfv, const double* pf,
+                                        const double* mf, size_t b0, int kk, int2 nr,
+                                        double& gvl, double& ae, double& g_aev, bool& pv) {
+    const size_t i = b0 + kk, im = b0 + (kk > 0 ? kk - 1 : 0);
+    const double gv = kk > 0 ? gx[i] + (-gx[im]) : gx[i];
+    gvl = in_iface(nr, kk) ? gv : 0.0;
+    const double f = adfv[i];
+    pv = f >= 0.0;
+    if (kk == nr.x) ae = pv ? pf[i] : mf[i];
+    else ae = pv ? fmin(mf[im], pf[i]) : fmin(pf[im], mf[i]);
+    g_aev = gvl * f;
+}
+
+// g_adfv (limited-flux part) and the factors' cotangents at level k, which gather the
+// terms of interface k (its own factors) and of interface k+1 (min(mf[k], pf[k+1]) or
+// min(pf[k], mf[k+1])); interface 0's "k-1" factor is level 0 itself.
 __global__ void __launch_bounds__(256)
 ka_vlim(DevMesh m, const double* __restrict__ gx, const double* __restrict__ adfv,
         const double* __restrict__ pf, const double* __restrict__ mf,
         double* __restrict__ g_adfv, double* __restrict__ g_pf, double* __restrict__ g_mf) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= m.N) return;
+    FLAT_NL(m);
     const int L = m.L;
     const int2 nr = m.nlev[n];
-    const size_t b0 = (size_t)n * L;
-    for (int k = 0; k < L; ++k) { g_pf[b0 + k] = 0.0; g_mf[b0 + k] = 0.0; }
-    for (int k = 0; k < L; ++k) {
-        const size_t i = b0 + k, im = b0 + (k > 0 ? k - 1 : 0);
-        const double gv = k > 0 ? gx[i] + (-gx[im]) : gx[i];
-        const double g_vl = in_iface(nr, k) ? gv : 0.0;
-        const double f = adfv[i];
-        const bool pv = f >= 0.0;
-        const double sdm = mf[im], sdp = pf[im];
-        double ae;
-        if (k == nr.x) ae = pv ? pf[i] : mf[i];
-        else ae = pv ? fmin(sdm, pf[i]) : fmin(sdp, mf[i]);
-        g_adfv[i] = g_vl * ae;
-        const double g_aev = g_vl * f;
-        if (k == nr.x) {
-            if (pv) g_pf[i] += g_aev; else g_mf[i] += g_aev;
-        } else if (pv) {
-            g_mf[im] += gmin_x(g_aev, sdm, pf[i]);
-            g_pf[i] += gmin_x(g_aev, pf[i], sdm);
-        } else {
-            g_pf[im] += gmin_x(g_aev, sdp, mf[i]);
-            g_mf[i] += gmin_x(g_aev, mf[i], sdp);
-        }
+    const size_t b0 = (size_t)n * L, im = b0 + (k > 0 ? k - 1 : 0);
+    double gvl, ae, g_aev;
+    bool pv;
+    vlim_at(gx, adfv, pf, mf, b0, k, nr, gvl, ae, g_aev, pv);
+    g_adfv[i] = gvl * ae;
+    double gp = 0.0, gm = 0.0;
+    if (k == nr.x) {
+        if (pv) gp += g_aev; else gm += g_aev;
+    } else if (pv) {
+        gp += gmin_x(g_aev, pf[i], mf[im]);
+        if (k == 0) gm += gmin_x(g_aev, mf[im], pf[i]);
+    } else {
+        gm += gmin_x(g_aev, mf[i], pf[im]);
+        if (k == 0) gp += gmin_x(g_aev, pf[im], mf[i]);
     }
+    if (k + 1 < L && k + 1 != nr.x) {
+        double gvl1, ae1, g1;
+        bool pv1;
+        vlim_at(gx, adfv, pf, mf, b0, k + 1, nr, gvl1, ae1, g1, pv1);
+        if (pv1) gm += gmin_x(g1, mf[i], pf[i + 1]);
+        else gp += gmin_x(g1, pf[i], mf[i + 1]);
+    }
+    g_pf[i] = gp;
+    g_mf[i] = gm;
 }
 
 // ---- limited horizontal flux (tracer_adv.py:460-466): factors' cotangents --------
@@ -196,107 +213,147 @@ ka_adfh(DevMesh m, const double* __restrict__ ghsc, const double* __restrict__ a
     g_adfh[t] = g;
 }
 
-// ---- one thread per column: P+/P- vertical part, QR4C backward, cluster backward --
+// ---- P+/P- vertical terms (tracer_adv.py:426-435) and the cluster bounds (a4) ------
+// cotangents of a 3-level cluster at level j: (a, b, c) = (tv[j-1], tv[j], tv[j+1]),
+// tv[-1] := tv[0], tv[L] := 0, combined as max(max(a, b), c) (min for the lower bound)
+template <bool MAX>
+__device__ __forceinline__ void cluster_vjp(const double* tv, size_t b0, int j, int L, double g,
+                                            double& ga, double& gb, double& gc) {
+    const double a = tv[b0 + (j > 0 ? j - 1 : 0)], b = tv[b0 + j];
+    const double c = (j + 1 < L) ? tv[b0 + j + 1] : 0.0;
+    if (MAX) {
+        const double m1 = fmax(a, b);
+        const double gm1 = gmax_x(g, m1, c);
+        gc = gmax_x(g, c, m1);
+        ga = gmax_x(gm1, a, b);
+        gb = gmax_x(gm1, b, a);
+    } else {
+        const double m1 = fmin(a, b);
+        const double gm1 = gmin_x(g, m1, c);
+        gc = gmin_x(g, c, m1);
+        ga = gmin_x(gm1, a, b);
+        gb = gmin_x(gm1, b, a);
+    }
+}
+
+template <bool MAX>
+__device__ __forceinline__ double cluster_gather(const double* tv, const double* g, size_t b0,
+                                                 int k, int L, int2 nr) {
+    auto own = [&](int j) { return (j == nr.x) || (j == nr.y - 1); };
+    double ga, gb, gc, out;
+    if (own(k)) {
+        out = g[b0 + k];
+    } else {
+        cluster_vjp<MAX>(tv, b0, k, L, g[b0 + k], ga, gb, gc);
+        out = gb + (k == 0 ? ga : 0.0);
+    }
+    if (k + 1 < L && !own(k + 1)) {
+        cluster_vjp<MAX>(tv, b0, k + 1, L, g[b0 + k + 1], ga, gb, gc);
+        out += ga;
+    }
+    if (k > 0 && !own(k - 1)) {
+        cluster_vjp<MAX>(tv, b0, k - 1, L, g[b0 + k - 1], ga, gb, gc);
+        out += gc;
+    }
+    return out;
+}
+
 __global__ void __launch_bounds__(256)
-ka_column(DevMesh m, const double* __restrict__ T, const double* __restrict__ Told,
-          const double* __restrict__ w, const double* __restrict__ adfv,
-          const double* __restrict__ g_fp, const double* __restrict__ g_fm,
-          const double* __restrict__ g_fmax, const double* __restrict__ g_fmin,
-          const double* __restrict__ tvmax, const double* __restrict__ tvmin,
-          double* __restrict__ g_adfv, double* __restrict__ gA, double* __restrict__ g_fv,
-          double* __restrict__ g_tvx, double* __restrict__ g_tvn, double* __restrict__ gw) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= m.N) return;
+ka_adfv(DevMesh m, const double* __restrict__ adfv, const double* __restrict__ g_fp,
+        const double* __restrict__ g_fm, const double* __restrict__ g_fmax,
+        const double* __restrict__ g_fmin, const double* __restrict__ tvmax,
+        const double* __restrict__ tvmin, double* __restrict__ g_adfv,
+        double* __restrict__ g_fv, double* __restrict__ g_tvx, double* __restrict__ g_tvn) {
+    FLAT_NL(m);
     const int L = m.L;
     const int2 nr = m.nlev[n];
     const size_t b0 = (size_t)n * L;
-    for (int k = 0; k < L; ++k) { gA[b0 + k] = 0.0; g_tvx[b0 + k] = 0.0; g_tvn[b0 + k] = 0.0; }
-    // b1 vertical terms: max(adf_v[k], 0) and max(-adf_v[k+1], 0) (and the mins)
-    for (int k = 0; k < L; ++k) {
-        const size_t i = b0 + k;
-        const bool nlm = in_layer(nr, k);
-        const double gpv = nlm ? g_fp[i] : 0.0, gnv = nlm ? g_fm[i] : 0.0;
-        const double f = adfv[i];
-        g_adfv[i] += gmax_x(gpv, f, 0.0) + gmin_x(gnv, f, 0.0);
+    const bool nlm = in_layer(nr, k);
+    const double f = adfv[i];
+    double g = g_adfv[i];
+    g += gmax_x(nlm ? g_fp[i] : 0.0, f, 0.0) + gmin_x(nlm ? g_fm[i] : 0.0, f, 0.0);
+    if (k > 0) {   // max(-adf_v[k], 0) in P+ of level k-1
+        const bool nlm1 = in_layer(nr, k - 1);
+        const double gpv = nlm1 ? g_fp[i - 1] : 0.0, gnv = nlm1 ? g_fm[i - 1] : 0.0;
+        g += -gmax_x(gpv, -f, 0.0) - gmin_x(gnv, -f, 0.0);
     }
-    for (int k = L - 2; k >= 0; --k) {
-        const size_t i = b0 + k;
-        const bool nlm = in_layer(nr, k);
-        const double gpv = nlm ? g_fp[i] : 0.0, gnv = nlm ? g_fm[i] : 0.0;
-        const double fb = adfv[i + 1];
-        const double gfb = -gmax_x(gpv, -fb, 0.0) - gmin_x(gnv, -fb, 0.0);
-        g_adfv[i + 1] += gfb;
+    g_adfv[i] = g;
+    g_fv[i] = -g;
+    g_tvx[i] = cluster_gather<true>(tvmax, g_fmax, b0, k, L, nr);
+    g_tvn[i] = cluster_gather<false>(tvmin, g_fmin, b0, k, L, nr);
+}
+
+// ---- QR4C backward (adv_flux_ver_ho, tracer_adv.py:241-295) --------------------------
+// contributions of interface j to A[j], A[j-1], A[j-2], A[j+1] and to w[j]
+struct QrTerms {
+    double a0 = 0.0, am1 = 0.0, am2 = 0.0, ap1 = 0.0, w = 0.0;
+};
+
+__device__ __forceinline__ QrTerms qr_vjp(const DevMesh& m, const double* T, const double* Told,
+                                          const double* w, const double* g_adfv, size_t b0,
+                                          int j, int2 nr) {
+    QrTerms r;
+    const int L = m.L;
+    if (!in_iface(nr, j) || j == nr.y) return r;
+    const size_t i = b0 + j, im = b0 + (j > 0 ? j - 1 : 0);
+    const double gav = g_adfv[i];
+    const double W = w[i], ar = m.area[i];
+    const double A = ab2a(Told[i], T[i]);
+    const double Am1 = ab2a(Told[im], T[im]);
+    if (j == nr.x) {
+        r.a0 = -(gav * ar * W);
+        r.w = gav * ar * (-A);
+    } else if (j == nr.x + 1 || j == nr.y - 1) {
+        const double gcm = gav * ar;
+        const double gs = -0.5 * (gcm * W);
+        r.w = gcm * (-0.5 * (Am1 + A));
+        r.a0 = gs;
+        r.am1 = gs;
+    } else if (j >= nr.x + 2 && j <= nr.y - 2) {
+        const size_t im2 = b0 + (j > 1 ? j - 2 : 0);
+        const size_t ip = b0 + (j + 1 < L ? j + 1 : L - 1);
+        const double Am2 = ab2a(Told[im2], T[im2]);
+        const double Ap1 = (j + 1 < L) ? ab2a(Told[ip], T[ip]) : 0.0;
+        const double Znz = m.Zs[j], Zm1 = m.Zs[L + j], Zm2 = m.Zs[2 * L + j],
+                     Zp1 = m.Zs[3 * L + j], zb = m.zbar[j];
+        const double dc = nz_safe(Zm1 - Znz), du = nz_safe(Znz - Zp1), dd = nz_safe(Zm2 - Zm1);
+        const double qc = (Am1 - A) / dc, qu = (A - Ap1) / du, qd = (Am2 - Am1) / dd;
+        const double Tmean1 = A + (2.0 * qc + qu) * (zb - Znz) / 3.0;
+        const double Tmean2 = Am1 + (2.0 * qc + qd) * (zb - Zm1) / 3.0;
+        const double gqm = gav * ar;
+        r.w = gqm * (-0.5 * (Tmean1 + Tmean2));
+        const double gtm = -0.5 * (gqm * W);
+        const double gr1 = gtm / 3.0 * (zb - Znz), gr2 = gtm / 3.0 * (zb - Zm1);
+        const double gqc = 2.0 * gr1 + 2.0 * gr2;
+        double gAk = gtm, gAm1 = gtm;
+        gAm1 += gqc / dc;
+        gAk -= gqc / dc;
+        gAk += gr1 / du;
+        r.ap1 = (j + 1 < L) ? -(gr1 / du) : 0.0;
+        r.am2 = gr2 / dd;
+        gAm1 -= gr2 / dd;
+        r.a0 = gAk;
+        r.am1 = gAm1;
     }
-    // adf_v = hov(A) - fv(T): QR4C backward (tracer_adv.py:241-295)
-    for (int k = 0; k < L; ++k) {
-        const size_t i = b0 + k;
-        const double gav = g_adfv[i];
-        g_fv[i] = -gav;
-        if (!in_iface(nr, k) || k == nr.y) continue;
-        const double W = w[i], ar = m.area[i];
-        const double A = ab2a(Told[i], T[i]);
-        const size_t im = b0 + (k > 0 ? k - 1 : 0);
-        const double Am1 = ab2a(Told[im], T[im]);
-        if (k == nr.x) {
-            gA[i] += -(gav * ar * W);
-            gw[i] += gav * ar * (-A);
-        } else if (k == nr.x + 1 || k == nr.y - 1) {
-            const double gcm = gav * ar;
-            const double gs = -0.5 * (gcm * W);
-            gw[i] += gcm * (-0.5 * (Am1 + A));
-            gA[i] += gs;
-            gA[im] += gs;
-        } else if (k >= nr.x + 2 && k <= nr.y - 2) {
-            const size_t im2 = b0 + (k > 1 ? k - 2 : 0);
-            const size_t ip = b0 + (k + 1 < L ? k + 1 : L - 1);
-            const double Am2 = ab2a(Told[im2], T[im2]);
-            const double Ap1 = (k + 1 < L) ? ab2a(Told[ip], T[ip]) : 0.0;
-            const double Znz = m.Zs[k], Zm1 = m.Zs[L + k], Zm2 = m.Zs[2 * L + k],
-                         Zp1 = m.Zs[3 * L + k], zb = m.zbar[k];
-            const double dc = nz_safe(Zm1 - Znz), du = nz_safe(Znz - Zp1), dd = nz_safe(Zm2 - Zm1);
-            const double qc = (Am1 - A) / dc, qu = (A - Ap1) / du, qd = (Am2 - Am1) / dd;
-            const double Tmean1 = A + (2.0 * qc + qu) * (zb - Znz) / 3.0;
-            const double Tmean2 = Am1 + (2.0 * qc + qd) * (zb - Zm1) / 3.0;
-            const double gqm = gav * ar;
-            gw[i] += gqm * (-0.5 * (Tmean1 + Tmean2));
-            const double gtm = -0.5 * (gqm * W);
-            const double gr1 = gtm / 3.0 * (zb - Znz), gr2 = gtm / 3.0 * (zb - Zm1);
-            const double gqc = 2.0 * gr1 + 2.0 * gr2;
-            double gAk = gtm, gAm1 = gtm;
-            gAm1 += gqc / dc;
-            gAk -= gqc / dc;
-            gAk += gr1 / du;
-            const double gAp1 = -(gr1 / du);
-            const double gAm2 = gr2 / dd;
-            gAm1 -= gr2 / dd;
-            gA[i] += gAk;
-            gA[im] += gAm1;
-            gA[im2] += gAm2;
-            if (k + 1 < L) gA[ip] += gAp1;
-        }
-    }
-    // a4 cluster backward: cmax = max(max(tv[k-1], tv[k]), tv[k+1]) off the own levels
-    for (int k = 0; k < L; ++k) {
-        const size_t i = b0 + k, im = b0 + (k > 0 ? k - 1 : 0);
-        const double gx_ = g_fmax[i], gn_ = g_fmin[i];
-        if ((k == nr.x) || (k == nr.y - 1)) {
-            g_tvx[i] += gx_;
-            g_tvn[i] += gn_;
-            continue;
-        }
-        const double a = tvmax[im], b = tvmax[i], c = (k + 1 < L) ? tvmax[i + 1] : 0.0;
-        const double m1 = fmax(a, b);
-        const double gm1 = gmax_x(gx_, m1, c), gc = gmax_x(gx_, c, m1);
-        g_tvx[im] += gmax_x(gm1, a, b);
-        g_tvx[i] += gmax_x(gm1, b, a);
-        if (k + 1 < L) g_tvx[i + 1] += gc;
-        const double an = tvmin[im], bn = tvmin[i], cn = (k + 1 < L) ? tvmin[i + 1] : 0.0;
-        const double m1n = fmin(an, bn);
-        const double gm1n = gmin_x(gn_, m1n, cn), gcn = gmin_x(gn_, cn, m1n);
-        g_tvn[im] += gmin_x(gm1n, an, bn);
-        g_tvn[i] += gmin_x(gm1n, bn, an);
-        if (k + 1 < L) g_tvn[i + 1] += gcn;
-    }
+    return r;
+}
+
+// A[k] gathers the terms of interfaces k, k+1, k+2 and k-1; w[k] those of interface k
+__global__ void __launch_bounds__(256)
+ka_qr(DevMesh m, const double* __restrict__ T, const double* __restrict__ Told,
+      const double* __restrict__ w, const double* __restrict__ g_adfv, double* __restrict__ gA,
+      double* __restrict__ gw) {
+    FLAT_NL(m);
+    const int L = m.L;
+    const int2 nr = m.nlev[n];
+    const size_t b0 = (size_t)n * L;
+    const QrTerms t0 = qr_vjp(m, T, Told, w, g_adfv, b0, k, nr);
+    double ga = t0.a0;
+    if (k + 1 < L) ga += qr_vjp(m, T, Told, w, g_adfv, b0, k + 1, nr).am1;
+    if (k + 2 < L) ga += qr_vjp(m, T, Told, w, g_adfv, b0, k + 2, nr).am2;
+    if (k > 0) ga += qr_vjp(m, T, Told, w, g_adfv, b0, k - 1, nr).ap1;
+    gA[i] = ga;
+    gw[i] += t0.w;
 }
 
 // element extremum of max/min(LO, T) over its vertices at level k (a2)
@@ -364,42 +421,58 @@ ka_vertex(DevMesh m, const double* __restrict__ LO, const double* __restrict__ T
     gT[i] = (gT[i] + gmax_x(gmx, t, lo)) + gmin_x(gmn, t, lo);
 }
 
-// ---- LO backward (tracer_adv.py:298-313) and the LO vertical flux, per column -----
+// ---- LO backward (tracer_adv.py:298-313) ---------------------------------------------
+__device__ __forceinline__ double lo_div(const DevMesh& m, const double* gLO, const double* hnn,
+                                         double dt, size_t i, int k, int2 nr) {
+    const double a = m.areasvol[i], hn = hnn[i];
+    const double safe_a = a > 0.0 ? a : 1.0, safe_hn = hn > 0.0 ? hn : 1.0;
+    const bool valid = in_layer(nr, k) && hn > 0.0 && a > 0.0;
+    return (valid ? gLO[i] : 0.0) / safe_hn / safe_a * dt;
+}
+
 __global__ void __launch_bounds__(256)
-ka_lo(DevMesh m, const double* __restrict__ T, const double* __restrict__ w,
-      const double* __restrict__ hnode, const double* __restrict__ hnn, double dt,
-      const double* __restrict__ gLO, double* __restrict__ g_fv, double* __restrict__ g_div,
+ka_lo(DevMesh m, const double* __restrict__ hnode, const double* __restrict__ hnn, double dt,
+      const double* __restrict__ gLO, double* __restrict__ g_div, double* __restrict__ gT) {
+    FLAT_NL(m);
+    const int2 nr = m.nlev[n];
+    const double a = m.areasvol[i], hn = hnn[i];
+    const double safe_hn = hn > 0.0 ? hn : 1.0;
+    const bool valid = in_layer(nr, k) && hn > 0.0 && a > 0.0;
+    const double gnum = (valid ? gLO[i] : 0.0) / safe_hn;
+    gT[i] += gnum * hnode[i];
+    g_div[i] = lo_div(m, gLO, hnn, dt, i, k, nr);
+}
+
+// ---- LO vertical flux (adv_flux_ver, tracer_adv.py:108-120) --------------------------
+// fv[k] = mask(-0.5 (T[k] (w+|w|) + T[k-1] (w-|w|)) * area), T[-1] := T[0]; its
+// cotangent is -g_adf_v + (g_div[k] - g_div[k-1]) (the LO divergence fv[k] - fv[k+1])
+__device__ __forceinline__ double fv_cot(const DevMesh& m, const double* g_fv, const double* g_div,
+                                         size_t b0, int k, int2 nr) {
+    const size_t i = b0 + k;
+    const double gd = g_div[i];
+    const double g = g_fv[i] + (k > 0 ? gd + (-g_div[i - 1]) : gd);
+    return -0.5 * ((in_layer(nr, k) ? g : 0.0) * m.area[i]);
+}
+
+__global__ void __launch_bounds__(256)
+ka_fv(DevMesh m, const double* __restrict__ T, const double* __restrict__ w,
+      const double* __restrict__ g_fv, const double* __restrict__ g_div,
       double* __restrict__ gT, double* __restrict__ gw) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= m.N) return;
+    FLAT_NL(m);
     const int L = m.L;
     const int2 nr = m.nlev[n];
-    const size_t b0 = (size_t)n * L;
-    double prev = 0.0;
-    for (int k = 0; k < L; ++k) {
-        const size_t i = b0 + k;
-        const bool nlm = in_layer(nr, k);
-        const double a = m.areasvol[i], hn = hnn[i];
-        const double safe_a = a > 0.0 ? a : 1.0, safe_hn = hn > 0.0 ? hn : 1.0;
-        const bool valid = nlm && hn > 0.0 && a > 0.0;
-        const double gnum = (valid ? gLO[i] : 0.0) / safe_hn;
-        gT[i] += gnum * hnode[i];
-        const double gd = gnum / safe_a * dt;
-        g_div[i] = gd;
-        g_fv[i] = g_fv[i] + (k > 0 ? gd + (-prev) : gd);
-        prev = gd;
+    const size_t b0 = (size_t)n * L, im = b0 + (k > 0 ? k - 1 : 0);
+    const double gs = fv_cot(m, g_fv, g_div, b0, k, nr);
+    const double W = w[i], aw = fabs(W);
+    double t = gs * (W + aw);
+    if (k == 0) t += gs * (W - aw);
+    if (k + 1 < L) {
+        const double W1 = w[i + 1];
+        t += fv_cot(m, g_fv, g_div, b0, k + 1, nr) * (W1 - fabs(W1));
     }
-    // fv = mask(-0.5 (T[k] (w+|w|) + T[k-1] (w-|w|)) * area), T[-1] := T[0]
-    for (int k = 0; k < L; ++k) {
-        const size_t i = b0 + k, im = b0 + (k > 0 ? k - 1 : 0);
-        const double gf = (in_layer(nr, k) ? g_fv[i] : 0.0) * m.area[i];
-        const double gs = -0.5 * gf;
-        const double W = w[i], aw = fabs(W);
-        gT[i] += gs * (W + aw);
-        gT[im] += gs * (W - aw);
-        const double gwp = gs * T[i], gwm = gs * T[im];
-        gw[i] += gwp + gwm + gabs(gwp - gwm, W);
-    }
+    gT[i] += t;
+    const double gwp = gs * T[i], gwm = gs * T[im];
+    gw[i] += gwp + gwm + gabs(gwp - gwm, W);
 }
 
 // ---- edges: adf_h = HO(A, eud) - fh(T) and fh's LO-divergence cotangent ------------
@@ -596,7 +669,7 @@ int launch_backward(const DevMesh& m, const AdjointWork& w, const fct_step_args*
         // backward
         hipLaunchKernelGGL(ka_final, dim3(grid(NL)), b, 0, s, m, g_out[j], a->hnode,
                            a->hnode_new, a->dt, gT[j], w.gLO, w.gx, w.ghsc);
-        hipLaunchKernelGGL(ka_vlim, dim3(grid(m.N)), b, 0, s, m, w.gx, w.adfv, w.pf, w.mf,
+        hipLaunchKernelGGL(ka_vlim, dim3(grid(NL)), b, 0, s, m, w.gx, w.adfv, w.pf, w.mf,
                            w.g_adfv, w.g_pf, w.g_mf);
         hipLaunchKernelGGL(ka_hlim, dim3(grid(NL)), b, 0, s, m, w.ghsc, w.adfh, w.pf, w.mf,
                            w.g_pf, w.g_mf);
@@ -605,15 +678,19 @@ int launch_backward(const DevMesh& m, const AdjointWork& w, const fct_step_args*
                            w.g_fmax, w.g_fmin, w.gLO);
         hipLaunchKernelGGL(ka_adfh, dim3(grid(DL)), b, 0, s, m, w.ghsc, w.adfh, w.pf, w.mf,
                            w.g_fp, w.g_fm, w.g_adfh);
-        hipLaunchKernelGGL(ka_column, dim3(grid(m.N)), b, 0, s, m, a->T[j], a->T_old[j], a->w_e,
-                           w.adfv, w.g_fp, w.g_fm, w.g_fmax, w.g_fmin, w.tvmax, w.tvmin,
-                           w.g_adfv, w.gA, w.g_fv, w.g_tvx, w.g_tvn, gw);
+        hipLaunchKernelGGL(ka_adfv, dim3(grid(NL)), b, 0, s, m, w.adfv, w.g_fp, w.g_fm,
+                           w.g_fmax, w.g_fmin, w.tvmax, w.tvmin, w.g_adfv, w.g_fv, w.g_tvx,
+                           w.g_tvn);
+        hipLaunchKernelGGL(ka_qr, dim3(grid(NL)), b, 0, s, m, a->T[j], a->T_old[j], a->w_e,
+                           w.g_adfv, w.gA, gw);
         hipLaunchKernelGGL(ka_share, dim3(grid(NL)), b, 0, s, m, w.LO, a->T[j], w.tvmax,
                            w.tvmin, w.g_tvx, w.g_tvn, w.sh_x, w.sh_n);
         hipLaunchKernelGGL(ka_vertex, dim3(grid(NL)), b, 0, s, m, w.LO, a->T[j], w.tvmax,
                            w.tvmin, w.sh_x, w.sh_n, w.gLO, gT[j]);
-        hipLaunchKernelGGL(ka_lo, dim3(grid(m.N)), b, 0, s, m, a->T[j], a->w_e, a->hnode,
-                           a->hnode_new, a->dt, w.gLO, w.g_fv, w.g_div, gT[j], gw);
+        hipLaunchKernelGGL(ka_lo, dim3(grid(NL)), b, 0, s, m, a->hnode, a->hnode_new, a->dt,
+                           w.gLO, w.g_div, gT[j]);
+        hipLaunchKernelGGL(ka_fv, dim3(grid(NL)), b, 0, s, m, a->T[j], a->w_e, w.g_fv, w.g_div,
+                           gT[j], gw);
         hipLaunchKernelGGL(ka_edge, dim3(grid(DL)), b, 0, s, m, a->T[j], a->T_old[j], w.vflux,
                            (const double2*)w.trxy, (const double2*)w.gnode, w.g_adfh, w.g_div,
                            w.g_vf, w.eA, w.eT, (double4*)w.eud);

@@ -646,15 +646,19 @@ void release_slot(BatchSlot& s) {
     s.bound = false;
 }
 
+// Copies go through the batch's own stream, so a bind does not wait for compute queued
+// on other streams.
 template <class T>
-int slot_upload(BatchSlot& s, const T** out, const std::vector<T>& host) {
+int slot_upload(BatchSlot& s, hipStream_t st, const T** out, const std::vector<T>& host) {
     void* d = nullptr;
     const size_t bytes = std::max<size_t>(1, host.size()) * sizeof(T);
     if (hipMalloc(&d, bytes) != hipSuccess)
         return set_error(FCT_E_NOMEM, "hipMalloc(%zu bytes) for a batch table failed", bytes);
     s.allocs.push_back(d);
-    if (!host.empty())
-        HIPCHK(hipMemcpy(d, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (!host.empty()) {
+        HIPCHK(hipMemcpyAsync(d, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
     *out = static_cast<const T*>(d);
     return FCT_OK;
 }
@@ -699,8 +703,10 @@ int build_slot(fct_batch* b, BatchSlot& s, const fct_step_args* args) {
         std::vector<unsigned char> dcopy(desc);
         const unsigned char* dd = nullptr;
         int rc;
-        if ((rc = slot_upload(s, &dd, dcopy)) || (rc = slot_upload(s, &g.blk_v, bv)) ||
-            (rc = slot_upload(s, &g.blk_n, bn)) || (rc = slot_upload(s, &g.blk_a, ba)))
+        if ((rc = slot_upload(s, b->capture, &dd, dcopy)) ||
+            (rc = slot_upload(s, b->capture, &g.blk_v, bv)) ||
+            (rc = slot_upload(s, b->capture, &g.blk_n, bn)) ||
+            (rc = slot_upload(s, b->capture, &g.blk_a, ba)))
             return rc;
         g.desc = reinterpret_cast<const FusedArgs*>(dd);
         g.nb_v = (unsigned)bv.size(); g.nb_n = (unsigned)bn.size(); g.nb_a = (unsigned)ba.size();

@@ -102,8 +102,9 @@ class _Active:
 
 class Server:
     def __init__(self, device, *, budget_bytes=200 << 30, max_active=64, meshes=None,
-                 dt=2700.0):
+                 dt=2700.0, chunk=4):
         self.device = device
+        self.chunk = int(chunk)        # steps per iteration while requests are waiting
         self.budget = int(budget_bytes)
         self.max_active = int(max_active)
         self.meshes = meshes or MeshCache()
@@ -138,13 +139,30 @@ class Server:
             a.traj.append(([x.clone() for x in inst.T()], [x.clone() for x in inst.T_old()]))
         return a
 
-    def _finish(self, a):
-        res = {"T": [x.cpu().numpy() for x in a.inst.T()]}
-        if a.spec.adjoint:
-            res.update(adjoint_of_rollout(a, self.dt))
-        key = (a.spec.level, a.spec.nl, a.spec.K)
-        self.pool[key].append(a.inst.plan)
-        return res
+    def _finish_async(self, a, side):
+        """Queue the request's finishing work (adjoint, download into pinned host
+        memory) on the side stream after the steps queued so far; returns the event
+        that marks its completion and the host buffers."""
+        import torch
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(self.device))
+        side.wait_event(ev)
+        with torch.cuda.stream(side):
+            out = {"T": [x for x in a.inst.T()]}
+            if a.spec.adjoint:
+                out.update(adjoint_of_rollout(a, self.dt, to_host=False))
+            host = {}
+            for k, v in out.items():
+                vs = v if isinstance(v, list) else [v]
+                hs = [torch.empty(x.shape, dtype=x.dtype, pin_memory=True) for x in vs]
+                for h, x in zip(hs, vs):
+                    h.copy_(x, non_blocking=True)
+                host[k] = hs if isinstance(v, list) else hs[0]
+            done = torch.cuda.Event()
+            done.record(side)
+        # the trajectory and the request's buffers stay referenced (through `a`) until
+        # `done` has completed, so the allocator cannot hand them out earlier
+        return done, host, out
 
     # -- the loop --------------------------------------------------------------------
     def run(self, specs, *, graph=False, log=None):
@@ -156,11 +174,13 @@ class Server:
         pending = collections.deque(sorted(specs, key=lambda s: s.arrival))
         active, used = [], 0
         self.active = active
-        batch, changed = None, False
+        batch, changed, retired = None, False, []
+        finishing = []
+        side = torch.cuda.Stream(self.device)      # default priority
         t0 = time.perf_counter()
         busy = 0.0
         updates = 0
-        while pending or active:
+        while pending or active or finishing:
             now = time.perf_counter() - t0
             while (pending and pending[0].arrival <= now and len(active) < self.max_active):
                 s = pending[0]
@@ -175,39 +195,63 @@ class Server:
                 active.append(a)
                 changed = True
             if not active:
-                wait = pending[0].arrival - (time.perf_counter() - t0)
-                if wait > 0:
-                    time.sleep(min(wait, 0.01))
-                continue
-            tb = time.perf_counter()
-            if changed:
-                if batch is not None:
-                    batch.close()
-                batch = FctBatch([a.inst for a in active])
-                batch.prepare()
-                changed = False
-            batch.step(graph=graph)
-            for a in active:
-                a.remaining -= 1
-                updates += a.inst.updates
-                if a.spec.adjoint:
-                    a.traj.append(([x.clone() for x in a.inst.T()],
-                                   [x.clone() for x in a.inst.T_old()]))
-            done = [a for a in active if a.remaining == 0]
-            if done:
-                torch.cuda.synchronize(self.device)
+                if finishing:
+                    # nothing to step: wait for the oldest finishing request
+                    finishing[0][1].synchronize()
+                elif pending:
+                    wait = pending[0].arrival - (time.perf_counter() - t0)
+                    if wait > 0:
+                        time.sleep(min(wait, 0.01))
+            else:
+                tb = time.perf_counter()
+                if changed:
+                    if batch is not None:
+                        retired.append(batch)     # closed once the GPU is idle
+                    batch = FctBatch([a.inst for a in active])
+                    batch.prepare()
+                    changed = False
+                # steps until the next completion; at most `chunk` while requests wait
+                nsteps = min(a.remaining for a in active)
+                if pending:
+                    nsteps = min(nsteps, self.chunk)
+                for _ in range(nsteps):
+                    batch.step(graph=graph)
+                    for a in active:
+                        a.remaining -= 1
+                        updates += a.inst.updates
+                        if a.spec.adjoint:
+                            a.traj.append(([x.clone() for x in a.inst.T()],
+                                           [x.clone() for x in a.inst.T_old()]))
+                done = [a for a in active if a.remaining == 0]
                 for a in done:
-                    self.results[a.spec.rid] = self._finish(a)
-                    t_end = time.perf_counter() - t0
-                    self.latency[a.spec.rid] = t_end - a.spec.arrival
-                    used -= a.bytes
+                    # the instance leaves the batch; its finishing work runs on the side
+                    # stream while the remaining requests keep stepping
+                    ev, host, keep = self._finish_async(a, side)
+                    finishing.append((a, ev, host, keep))
                     active.remove(a)
+                    changed = True
+                busy += time.perf_counter() - tb
+            still = []
+            for a, ev, host, keep in finishing:
+                if ev.query():
+                    self.results[a.spec.rid] = {
+                        k: ([h.numpy() for h in v] if isinstance(v, list) else v.numpy())
+                        for k, v in host.items()}
+                    self.latency[a.spec.rid] = time.perf_counter() - t0 - a.spec.arrival
+                    used -= a.bytes
+                    self.pool[(a.spec.level, a.spec.nl, a.spec.K)].append(a.inst.plan)
                     if log:
                         log(f"req {a.spec.rid} done in {self.latency[a.spec.rid]:.3f}s")
-                changed = True
-            busy += time.perf_counter() - tb
-        if batch is not None:
-            batch.close()
+                else:
+                    still.append((a, ev, host, keep))
+            finishing[:] = still
+            if len(retired) > 16 and not finishing and not active:
+                for old in retired:
+                    old.close()
+                retired.clear()
+        torch.cuda.synchronize(self.device)
+        for old in retired + ([batch] if batch is not None else []):
+            old.close()
         wall = time.perf_counter() - t0
         return summarize(specs, self.latency, self.setup, wall, busy, self.meshes)
 
@@ -247,7 +291,7 @@ def request_case(mesh, spec, device, dt):
     return case
 
 
-def adjoint_of_rollout(a, dt):
+def adjoint_of_rollout(a, dt, to_host=True):
     """Gradients of sum_surface (T_final - T_target)^2 with respect to the initial
     tracers (T and T_old of step 0) and uv, through the stored trajectory."""
     import torch
@@ -269,6 +313,8 @@ def adjoint_of_rollout(a, dt):
         g_uv += guv
         g = [x + y for x, y in zip(gT, gTo)]
         gTo = gTo_s
+    if not to_host:
+        return {"g_T0": g, "g_T_old0": gTo, "g_uv": g_uv}
     return {"g_T0": [x.cpu().numpy() for x in g], "g_T_old0": [x.cpu().numpy() for x in gTo],
             "g_uv": g_uv.cpu().numpy()}
 
