@@ -81,6 +81,9 @@ def main():
                     help="plain: one icosa-9 instance (the metric); batched: 64 uneven "
                          "instances in one batched launch vs 64 sequential plain launches")
     ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the timed steps, run a 1 GiB read + 1 GiB write elementwise "
+                         "kernel (the reference for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: requests per second")
     ap.add_argument("--duration", type=float, default=10.0, help="serve: seconds of arrivals")
     ap.add_argument("--levels", type=str, default="4,7", help="serve: icosa level range")
@@ -169,6 +172,14 @@ def main():
     ms = start.elapsed_time(end) / args.steps
     log(f"timed {args.steps} steps: {ms:.3f} ms/step (wall {1e3 * t_wall / args.steps:.3f})")
 
+    if args.copy_calibration:
+        x = torch.ones(1 << 27, dtype=torch.float64, device=dev)     # 1 GiB
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 x (1 GiB read + 1 GiB write) done")
+        del x, y
     # sanity: the chained field stays finite and bounded
     last = bufs[0][(args.warmup + args.steps + 1) % 3]
     m = torch.from_numpy(mesh.node_layer_mask).to(dev)

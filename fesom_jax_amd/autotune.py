@@ -22,7 +22,7 @@ SPACE = {
     _lib.FCT_TUNE_VFLUX_EDGES: (2, 4),
     _lib.FCT_TUNE_ADFH_EDGES: (1, 2),
 }
-DEFAULT = {_lib.FCT_TUNE_XCD: 8, _lib.FCT_TUNE_WAVES_PER_GROUP: 4,
+DEFAULT = {_lib.FCT_TUNE_XCD: 1, _lib.FCT_TUNE_WAVES_PER_GROUP: 4,
            _lib.FCT_TUNE_VFLUX_EDGES: 4, _lib.FCT_TUNE_ADFH_EDGES: 2}
 NAMES = {_lib.FCT_TUNE_XCD: "xcd", _lib.FCT_TUNE_WAVES_PER_GROUP: "waves_per_group",
          _lib.FCT_TUNE_VFLUX_EDGES: "vflux_edges", _lib.FCT_TUNE_ADFH_EDGES: "adfh_edges"}

@@ -391,15 +391,15 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
         maxdeg = std::max(maxdeg, std::max(h_eoff[n + 1] - h_eoff[n], h_noff[n + 1] - h_noff[n]));
     int dm = maxdeg <= 6 ? 6 : (maxdeg <= 8 ? 8 : 0);
     if (dm && (L > 128 || !fused_mesh_ok(d, h_eoff, h_ecsr, h_noff, h_ncsr))) dm = 0;
-    std::vector<int2> h_tcnt;
+    std::vector<int4> h_thdr;
     std::vector<int4> h_tedge, h_telem;
     if (dm) {
-        h_tcnt.resize(N);
+        h_thdr.resize(N);
         h_tedge.assign((size_t)N * dm, make_int4(0, 0, 0, 0));
         h_telem.assign((size_t)N * dm, make_int4(0, 0, 0, 0));
         for (int n = 0; n < N; ++n) {
             const int de = h_eoff[n + 1] - h_eoff[n], ne = h_noff[n + 1] - h_noff[n];
-            h_tcnt[n] = make_int2(de, ne);
+            h_thdr[n] = make_int4(d->ulevels_nod2D[n] - 1, d->nlevels_nod2D[n] - 1, de, ne);
             for (int q = 0; q < de; ++q) {
                 const int ent = h_ecsr[h_eoff[n] + q];
                 const int e = ent >> 1;
@@ -428,6 +428,37 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
                                                         d->elem_nodes[3 * el + 1],
                                                         d->elem_nodes[3 * el + 2]);
             }
+            // padding slots repeat slot 0 (the element-slot range .w stays as set)
+            for (int q = de; q < dm && de > 0; ++q) {
+                int4& t = h_tedge[(size_t)n * dm + q];
+                const int4 f = h_tedge[(size_t)n * dm];
+                t.x = f.x; t.y = f.y; t.z = f.z;
+            }
+            for (int q = ne; q < dm && ne > 0; ++q)
+                h_telem[(size_t)n * dm + q] = h_telem[(size_t)n * dm];
+        }
+    }
+    // fused-path edge tables: packed triangle layer ranges and gradient-triangle vertices
+    std::vector<int> h_etr;
+    std::vector<int4> h_udv;
+    if (dm) {
+        h_etr.assign(D, 0);
+        h_udv.assign(2 * (size_t)D, make_int4(0, 0, 0, 0));
+        for (int e = 0; e < D; ++e) {
+            unsigned packed = 0;
+            for (int side = 0; side < 2; ++side) {
+                const int el = d->edge_tri[2 * e + side];
+                if (el < 0) continue;
+                const unsigned lo = (unsigned)(d->ulevels[el] - 1);
+                const unsigned hi = (unsigned)std::max(d->nlevels[el] - 1, d->ulevels[el] - 1);
+                packed |= (lo | (hi << 8)) << (16 * side);
+            }
+            h_etr[e] = (int)packed;
+            const int u = h_updn[e].x, w = h_updn[e].y;
+            h_udv[2 * (size_t)e] = make_int4(d->elem_nodes[3 * u], d->elem_nodes[3 * u + 1],
+                                            d->elem_nodes[3 * u + 2], d->elem_nodes[3 * w]);
+            h_udv[2 * (size_t)e + 1] = make_int4(d->elem_nodes[3 * w + 1],
+                                                d->elem_nodes[3 * w + 2], 0, 0);
         }
     }
     // static QR4C stencil depths (_z_stencil, tracer_adv.py:228-238)
@@ -469,7 +500,9 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
     p->host_edge_tri.assign(d->edge_tri, d->edge_tri + 2 * (size_t)D);
     p->host_updn.assign(d->edge_up_dn_tri, d->edge_up_dn_tri + 2 * (size_t)D);
     p->tab.dm = dm;
-    if (dm && ((rc = dev_upload(p, &p->tab.cnt, h_tcnt.data(), h_tcnt.size())) ||
+    if (dm && ((rc = dev_upload(p, &p->tab.hdr, h_thdr.data(), h_thdr.size())) ||
+               (rc = dev_upload(p, &m.etr, h_etr.data(), h_etr.size())) ||
+               (rc = dev_upload(p, &m.udv, h_udv.data(), h_udv.size())) ||
                (rc = dev_upload(p, &p->tab.edge, h_tedge.data(), h_tedge.size())) ||
                (rc = dev_upload(p, &p->tab.elem, h_telem.data(), h_telem.size())))) {
         free_plan(p);

@@ -39,16 +39,23 @@ struct DevMesh {
     const double* areasvol;   // [N*L]
     const double* Zs;         // [4*L] static QR4C stencil depths Z[k], Z[k-1], Z[k-2], Z[k+1]
     const double* zbar;       // [L]
+    // fused-path edge tables (nl <= 128): the layer ranges of an edge's two triangles as
+    // bytes lo1|hi1|lo2|hi2 (a missing triangle: empty), and the vertices of its up and
+    // down gradient triangles (u0, u1, u2, d0), (d1, d2, -, -)
+    const int* etr;           // [D]
+    const int4* udv;          // [2*D]
 };
 
 // Per-node stencil table of the fused path (dm slots per node).
 struct NodeTab {
     int dm = 0;               // slots per node (6 or 8); 0 = mesh not supported
-    const int2* cnt;          // [N] (incident edges, incident elements)
+    const int4* hdr;          // [N] (nzmin, nzmax, incident edges, incident elements)
     const int4* edge;         // [N*dm] (2*edge + side, other node, layer ranges of the
                               //  edge's triangles as bytes lo1|hi1|lo2|hi2, layer range of
                               //  element slot q as bytes lo|hi), edges in reference order
     const int4* elem;         // [N*dm] (element, v0, v1, v2), increasing element id
+    // slots past a node's counts repeat its first slot, so a kernel can load every slot
+    // without waiting for the counts
 };
 
 __device__ __forceinline__ bool in_layer(int2 r, int k) { return k >= r.x && k < r.y; }
@@ -99,7 +106,8 @@ int launch_backward(const DevMesh& m, const AdjointWork& w, const fct_step_args*
 // Launch-shape knobs of the fused path (fct_plan_set_tuning / fct_batch_set_tuning);
 // results do not depend on them.
 struct FusedTuning {
-    int xcd = 8;        // remap blocks so each of `xcd` XCDs sweeps one contiguous tile range
+    int xcd = 1;        // remap blocks so each of `xcd` XCDs sweeps one contiguous tile range
+                        // (1 = hardware order, the fastest measured on a single large mesh)
     int wpb = 4;        // wavefronts per workgroup (1, 2 or 4)
     int vflux_epw = 4;  // edges per wavefront in kf_vflux (2 or 4)
     int adfh_epw = 2;   // edges per wavefront in kf_adfh (1 or 2)

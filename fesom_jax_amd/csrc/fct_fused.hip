@@ -181,8 +181,8 @@ template <int NS, int EPW>
 __device__ __forceinline__ void vflux_group(const FusedArgs& a, int g) {
     const DevMesh& m = a.m;
     const int L = m.L;
-    int e[EPW];
-    int2 et[EPW], kr[EPW], r1[EPW], r2[EPW];
+    int e[EPW], tr[EPW];
+    int2 et[EPW], kr[EPW];
     double4 c[EPW];
     int kmax = 0;
 #pragma unroll
@@ -192,8 +192,7 @@ __device__ __forceinline__ void vflux_group(const FusedArgs& a, int g) {
         et[q] = ldv(m.edge_tri, e[q]);
         kr[q] = eq < m.D ? ldv(a.ekr, e[q]) : make_int2(0, 0);
         c[q] = ldv(m.cross, e[q]);
-        r1[q] = et[q].x >= 0 ? ldv(m.elev, et[q].x) : make_int2(0, 0);
-        r2[q] = et[q].y >= 0 ? ldv(m.elev, et[q].y) : make_int2(0, 0);
+        tr[q] = G(m.etr)[e[q]];
         kmax = kr[q].y > kmax ? kr[q].y : kmax;
     }
 #pragma unroll
@@ -217,9 +216,9 @@ __device__ __forceinline__ void vflux_group(const FusedArgs& a, int g) {
         for (int q = 0; q < EPW; ++q) {
             if (k < kr[q].x || k >= kr[q].y) continue;
             double t1 = 0.0, t2 = 0.0;
-            if (et[q].x >= 0 && in_layer(r1[q], k))
+            if (in_bytes(tr[q], 0, k))
                 t1 = (u1[q].x * c[q].y - u1[q].y * c[q].x) * h1[q];
-            if (et[q].y >= 0 && in_layer(r2[q], k))
+            if (in_bytes(tr[q], 16, k))
                 t2 = (u2[q].y * c[q].z - u2[q].x * c[q].w) * h2[q];
             G(a.vflux)[(size_t)e[q] * L + k] = t1 + t2;
         }
@@ -233,8 +232,8 @@ template <int NS, int DM>
 __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
     const DevMesh& m = a.m;
     const int L = m.L;
-    const int2 nr = ldv(m.nlev, n);
-    const int2 cnt = ldv(a.tab.cnt, n);          // (edges, elements)
+    const int4 hd = ldv(a.tab.hdr, n);
+    const int2 nr = make_int2(hd.x, hd.y), cnt = make_int2(hd.z, hd.w);   // levels, counts
     const auto T = G(a.T[j]);
     const size_t b0 = (size_t)n * L;
     const size_t jo = (size_t)j * m.N * L;
@@ -243,8 +242,8 @@ __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
     int4 el[DM];
 #pragma unroll
     for (int q = 0; q < DM; ++q) {
-        es[q] = ldv(a.tab.edge, (size_t)n * DM + (q < cnt.x ? q : 0));
-        el[q] = ldv(a.tab.elem, (size_t)n * DM + (q < cnt.y ? q : 0));
+        es[q] = ldv(a.tab.edge, (size_t)n * DM + q);
+        el[q] = ldv(a.tab.elem, (size_t)n * DM + q);
     }
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -350,11 +349,9 @@ __device__ __forceinline__ void adfh_group(const FusedArgs& a, int g, int j) {
         ur[q] = ldv(m.utri_rng, e[q]);
         ud[q] = ldv(m.updn, e[q]);
         cc[q] = ldv(m.edge_c, e[q]);
-#pragma unroll
-        for (int v = 0; v < 3; ++v) {
-            u[q][v] = G(m.en)[3 * ud[q].x + v];
-            d[q][v] = G(m.en)[3 * ud[q].y + v];
-        }
+        const int4 va = ldv(m.udv, 2 * (size_t)e[q]), vb = ldv(m.udv, 2 * (size_t)e[q] + 1);
+        u[q][0] = va.x; u[q][1] = va.y; u[q][2] = va.z;
+        d[q][0] = va.w; d[q][1] = vb.x; d[q][2] = vb.y;
         kmax = kr[q].y > kmax ? kr[q].y : kmax;
     }
 #pragma unroll
@@ -477,10 +474,9 @@ __device__ __forceinline__ void adfv_column(const FusedArgs& a, int n, int2 nr,
 // pass 2 of node n, tracer j: cluster bounds, P+/P-, limiter factors
 // ---------------------------------------------------------------------------
 template <int NS, int DM>
-__device__ __forceinline__ void pass2_body(const FusedArgs& a, int n, int j, int2 nr) {
+__device__ __forceinline__ void pass2_body(const FusedArgs& a, int n, int j, int2 nr, int2 cnt) {
     const DevMesh& m = a.m;
     const int L = m.L;
-    const int2 cnt = ldv(a.tab.cnt, n);
     const size_t jo = (size_t)j * m.N * L;
     const size_t b0 = (size_t)n * L;
     const auto Tb = G(a.T[j]);
@@ -492,7 +488,7 @@ __device__ __forceinline__ void pass2_body(const FusedArgs& a, int n, int j, int
     const int kneed = nr.y + 1 < L ? nr.y + 1 : L;
     int4 es[DM];
 #pragma unroll
-    for (int q = 0; q < DM; ++q) es[q] = ldv(a.tab.edge, (size_t)n * DM + (q < cnt.x ? q : 0));
+    for (int q = 0; q < DM; ++q) es[q] = ldv(a.tab.edge, (size_t)n * DM + q);
     double Tc[NS], Ac[NS], LOc[NS], tvx[NS], tvn[NS], ap[NS], am[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -583,19 +579,19 @@ __device__ __forceinline__ void pass2_body(const FusedArgs& a, int n, int j, int
 
 template <int NS, int DM>
 __device__ __forceinline__ void pass2_task(const FusedArgs& a, int n, int j) {
-    const int2 nr = ldv(a.m.nlev, n);
-    if (NS == 1 || nr.y < 64) pass2_body<1, DM>(a, n, j, nr);
-    else pass2_body<NS, DM>(a, n, j, nr);
+    const int4 hd = ldv(a.tab.hdr, n);
+    const int2 nr = make_int2(hd.x, hd.y), cnt = make_int2(hd.z, hd.w);
+    if (NS == 1 || nr.y < 64) pass2_body<1, DM>(a, n, j, nr, cnt);
+    else pass2_body<NS, DM>(a, n, j, nr, cnt);
 }
 
 // ---------------------------------------------------------------------------
 // pass 3 of node n, tracer j: limited fluxes, divergence, reconstruction
 // ---------------------------------------------------------------------------
 template <int NS, int DM>
-__device__ __forceinline__ void pass3_body(const FusedArgs& a, int n, int j, int2 nr) {
+__device__ __forceinline__ void pass3_body(const FusedArgs& a, int n, int j, int2 nr, int2 cnt) {
     const DevMesh& m = a.m;
     const int L = m.L;
-    const int2 cnt = ldv(a.tab.cnt, n);
     const auto T = G(a.T[j]);
     const auto To = G(a.Told[j]);
     const size_t jo = (size_t)j * m.N * L;
@@ -606,7 +602,7 @@ __device__ __forceinline__ void pass3_body(const FusedArgs& a, int n, int j, int
     const int kneed = nr.y + 1 < L ? nr.y + 1 : L;
     int4 es[DM];
 #pragma unroll
-    for (int q = 0; q < DM; ++q) es[q] = ldv(a.tab.edge, (size_t)n * DM + (q < cnt.x ? q : 0));
+    for (int q = 0; q < DM; ++q) es[q] = ldv(a.tab.edge, (size_t)n * DM + q);
     double Tc[NS], Ac[NS], Pc[NS], Mc[NS], acc[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -693,21 +689,27 @@ __device__ __forceinline__ void pass3_body(const FusedArgs& a, int n, int j, int
 
 template <int NS, int DM>
 __device__ __forceinline__ void pass3_task(const FusedArgs& a, int n, int j) {
-    const int2 nr = ldv(a.m.nlev, n);
-    if (NS == 1 || nr.y < 64) pass3_body<1, DM>(a, n, j, nr);
-    else pass3_body<NS, DM>(a, n, j, nr);
+    const int4 hd = ldv(a.tab.hdr, n);
+    const int2 nr = make_int2(hd.x, hd.y), cnt = make_int2(hd.z, hd.w);
+    if (NS == 1 || nr.y < 64) pass3_body<1, DM>(a, n, j, nr, cnt);
+    else pass3_body<NS, DM>(a, n, j, nr, cnt);
 }
 
 // ---------------------------------------------------------------------------
 // kernels: single instance (descriptor by value) and batched (descriptor array)
 // ---------------------------------------------------------------------------
+// Occupancy targets (wavefronts per SIMD) per kernel: the most the compiler reaches
+// without spilling to scratch (node degree 8 tables need more registers); the passes
+// are latency-bound, so more resident wavefronts hide more of each one's memory waits.
+#define FCT_OCC(n) __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(n)))
+
 template <int NS, int EPW>
-__global__ void __launch_bounds__(256) kf_vflux(FusedArgs a) {
+__global__ void FCT_OCC(8) kf_vflux(FusedArgs a) {
     int w;
     if (single_wave((a.m.D + EPW - 1) / EPW, a.xcd, w)) vflux_group<NS, EPW>(a, w);
 }
 template <int NS, int EPW>
-__global__ void __launch_bounds__(256) kb_vflux(const FusedArgs* __restrict__ d,
+__global__ void FCT_OCC(8) kb_vflux(const FusedArgs* __restrict__ d,
                                                 const int2* __restrict__ blk, int xcd) {
     int inst;
     const int w = batch_wave(blk, xcd, inst);
@@ -716,14 +718,14 @@ __global__ void __launch_bounds__(256) kb_vflux(const FusedArgs* __restrict__ d,
 }
 
 template <int NS, int EPW>
-__global__ void __launch_bounds__(256) kf_adfh(FusedArgs a) {
+__global__ void FCT_OCC(6) kf_adfh(FusedArgs a) {
     int w;
     if (!single_wave((int64_t)((a.m.D + EPW - 1) / EPW) * a.K, a.xcd, w)) return;
     const int g = w / a.K;
     adfh_group<NS, EPW>(a, g, w - g * a.K);
 }
 template <int NS, int EPW>
-__global__ void __launch_bounds__(256) kb_adfh(const FusedArgs* __restrict__ d,
+__global__ void FCT_OCC(6) kb_adfh(const FusedArgs* __restrict__ d,
                                                const int2* __restrict__ blk, int xcd) {
     int inst;
     const int w = batch_wave(blk, xcd, inst);
@@ -733,16 +735,16 @@ __global__ void __launch_bounds__(256) kb_adfh(const FusedArgs* __restrict__ d,
     adfh_group<NS, EPW>(a, g, w - g * a.K);
 }
 
-#define FCT_NODE_KERNELS(PASS)                                                          \
+#define FCT_NODE_KERNELS(PASS, OCC)                                                     \
     template <int NS, int DM>                                                           \
-    __global__ void __launch_bounds__(256) kf_##PASS(FusedArgs a) {                     \
+    __global__ void FCT_OCC(DM == 6 ? OCC : 4) kf_##PASS(FusedArgs a) {                 \
         int w;                                                                          \
         if (!single_wave((int64_t)a.m.N * a.K, a.xcd, w)) return;                       \
         const int n = w / a.K;                                                          \
         PASS##_task<NS, DM>(a, n, w - n * a.K);                                         \
     }                                                                                   \
     template <int NS, int DM>                                                           \
-    __global__ void __launch_bounds__(256) kb_##PASS(const FusedArgs* __restrict__ d,   \
+    __global__ void FCT_OCC(DM == 6 ? OCC : 4) kb_##PASS(const FusedArgs* __restrict__ d, \
                                                      const int2* __restrict__ blk,      \
                                                      int xcd) {                         \
         int inst;                                                                       \
@@ -753,9 +755,9 @@ __global__ void __launch_bounds__(256) kb_adfh(const FusedArgs* __restrict__ d,
         PASS##_task<NS, DM>(a, n, w - n * a.K);                                         \
     }
 
-FCT_NODE_KERNELS(pass1)
-FCT_NODE_KERNELS(pass2)
-FCT_NODE_KERNELS(pass3)
+FCT_NODE_KERNELS(pass1, 5)
+FCT_NODE_KERNELS(pass2, 5)
+FCT_NODE_KERNELS(pass3, 6)
 #undef FCT_NODE_KERNELS
 
 // ---------------------------------------------------------------------------

@@ -224,3 +224,28 @@ def test_icosa_nl70_two_slots(torch_dev):
     sh = sharp_case(m, case, 2.0)
     g = _gpu_step(m, sh, torch_dev, path="fused")[0]
     _check("icosa4 nl70 fused sharp", g, _oracle(m, sh, 0), m)
+
+
+@pytest.mark.parametrize("knobs", [{"XCD": 8}, {"WAVES_PER_GROUP": 2, "XCD": 3},
+                                   {"WAVES_PER_GROUP": 1, "VFLUX_EDGES": 2, "ADFH_EDGES": 1}])
+def test_launch_shapes_do_not_change_results(knobs, torch_dev):
+    """Other launch shapes give the default launch's bits."""
+    from fesom_jax_amd import _lib
+    from fesom_jax_amd.inputs import make_case
+    from fesom_jax_amd.meshgen import icosahedral_mesh
+    from fesom_jax_amd.tracer_adv import get_plan
+    m = icosahedral_mesh(4, 70, seed=23)
+    case = sharp_case(m, make_case(m, seed=24, dt=2700.0), 2.0)
+    ref = _gpu_step(m, case, torch_dev, path="fused")
+    plan = get_plan(m, torch_dev)
+    try:
+        for k, v in knobs.items():
+            _lib.check(plan.lib.fct_plan_set_tuning(plan.ptr, getattr(_lib, "FCT_TUNE_" + k), v),
+                       "tuning")
+        got = _gpu_step(m, case, torch_dev, path="fused")
+    finally:
+        for k, v in (("XCD", 1), ("WAVES_PER_GROUP", 4), ("VFLUX_EDGES", 4),
+                     ("ADFH_EDGES", 2)):
+            plan.lib.fct_plan_set_tuning(plan.ptr, getattr(_lib, "FCT_TUNE_" + k), v)
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
