@@ -61,7 +61,24 @@ def cpu_baseline(level, nl, ntracers, seed, dt):
     return {"value": u / el, "unit": "node-level updates/s", "cores": 1, "kind": "port",
             "sample": (f"numpy FP64 restatement of the reference FCT (oracle/fct_numpy.py), "
                        f"1 step x {ntracers} tracers on icosa-{level} ({mesh.nod2D} nodes, "
-                       f"nl={nl}, {u} updates), {el:.2f} s on {cpu_model()}")}
+                       f"nl={nl}, {u} updates), {el:.2f} s single-threaded on {cpu_model()} "
+                       f"({os.cpu_count()} logical CPUs on the host)")}
+
+
+def measured_traffic(args, mesh, K):
+    """HBM-side bytes per step of the default workload from the committed rocprofv3
+    summary (FETCH_SIZE + WRITE_SIZE of the step's kernels, each scaled by the counter's
+    reading for a copy kernel of known size); None for other workloads."""
+    if (args.level, mesh.nl, K, args.seed) != (9, 70, 2, 2024):
+        return None
+    p = ROOT / "profiles" / "fct_step_icosa9_pmc.md"
+    try:
+        for line in p.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_step:"):
+                return int(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
 
 
 def main():
@@ -74,7 +91,7 @@ def main():
     ap.add_argument("--tracers", type=int, default=2)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--dt", type=float, default=2700.0)
-    ap.add_argument("--cpu-level", type=int, default=6)
+    ap.add_argument("--cpu-level", type=int, default=7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=("plain", "batched", "serve", "adjoint"),
                     default="plain",
@@ -191,8 +208,9 @@ def main():
     value = U / (ms * 1e-3)
     achieved = B / (ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "fct_step (all kernels of one step)",
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": measured_traffic(args, mesh, K),
+                "kernel": "fct_step (the five launches of one step)",
                 "bytes_per_step": B}
     cpu = None
     if not args.no_cpu_baseline:
