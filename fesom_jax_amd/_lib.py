@@ -42,7 +42,7 @@ EXPORTED_SYMBOLS = ("fct_abi_version", "fct_last_error", "fct_plan_create",
                     "fct_forward_ex", "fct_forward_phase", "fct_plan_set_tuning",
                     "fct_batch_create", "fct_batch_set_tuning", "fct_batch_bind",
                     "fct_batch_launch", "fct_batch_graph_launch", "fct_batch_destroy",
-                    "fct_backward")
+                    "fct_backward", "fct_forward_upwind")
 
 
 class FctMeshDesc(ctypes.Structure):
@@ -130,6 +130,8 @@ def load(path: Path | None = None, *, require_torch: bool = True):
     lib.fct_batch_graph_launch.argtypes = [c_void_p, c_int32, c_void_p]
     lib.fct_batch_destroy.restype = c_int32
     lib.fct_batch_destroy.argtypes = [c_void_p]
+    lib.fct_forward_upwind.restype = c_int32
+    lib.fct_forward_upwind.argtypes = [c_void_p, POINTER(FctStepArgs), c_void_p]
     lib.fct_backward.restype = c_int32
     lib.fct_backward.argtypes = [c_void_p, POINTER(FctStepArgs), POINTER(FctAdjointArgs),
                                  c_void_p]

@@ -611,6 +611,16 @@ int fct_forward(fct_plan* p, const fct_step_args* a, void* stream) {
     return fct_forward_ex(p, a, FCT_PATH_AUTO, stream);
 }
 
+int fct_forward_upwind(fct_plan* p, const fct_step_args* a, void* stream) {
+    int rc = check_args(p, a);
+    if (rc) return rc;
+    if (a->w_i || a->Z3d)
+        return set_error(FCT_E_INVALID, "the upwind step takes no w_i, Z3d or zbar3");
+    if (p->tab.dm == 0)
+        return set_error(FCT_E_UNSUPPORTED, "this mesh cannot take the node-table kernels");
+    return launch_upwind(p->m, p->tab, p->ekr, a, p->tune, p->vflux, (hipStream_t)stream);
+}
+
 int fct_forward_phase(fct_plan* p, const fct_step_args* a, int32_t j, int32_t phase,
                       const fct_phase_bufs* b, void* stream) {
     int rc = check_args(p, a);

@@ -132,6 +132,10 @@ int launch_fused(const DevMesh& m, const NodeTab& tab, const int2* ekr, const fc
                  int kw, const FusedTuning& tune, double* vflux, double* LO, double* gnode,
                  double* adfh, double* pf, double* mf, hipStream_t s);
 
+// Upwind-only step (fct_fused.hip).
+int launch_upwind(const DevMesh& m, const NodeTab& tab, const int2* ekr, const fct_step_args* a,
+                  const FusedTuning& tune, double* vflux, hipStream_t s);
+
 // Host-side descriptor of one instance's fused step (K <= kw tracers, one chunk).
 size_t fused_args_size();
 int fill_fused_args(void* out, const DevMesh& m, const NodeTab& tab, const int2* ekr,

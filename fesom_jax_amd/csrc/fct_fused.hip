@@ -696,6 +696,80 @@ __device__ __forceinline__ void pass3_task(const FusedArgs& a, int n, int j) {
 }
 
 // ---------------------------------------------------------------------------
+// upwind-only step of node n, tracer j (advect_one, tracer_adv.py:154-165): upwind
+// fluxes of ttfAB (adv_flux_hor :96-105, adv_flux_ver :108-120), flux2dtracer
+// (:123-139) and ale_reconstruct (:142-151); every lane of T_new is written
+// ---------------------------------------------------------------------------
+template <int NS, int DM>
+__device__ __forceinline__ void upwind_task(const FusedArgs& a, int n, int j) {
+    const DevMesh& m = a.m;
+    const int L = m.L;
+    const int4 hd = ldv(a.tab.hdr, n);
+    const int2 nr = make_int2(hd.x, hd.y), cnt = make_int2(hd.z, hd.w);
+    const auto T = G(a.T[j]);
+    const auto To = G(a.Told[j]);
+    const auto Tn = G(a.Tnew[j]);
+    const size_t b0 = (size_t)n * L;
+    int4 es[DM];
+#pragma unroll
+    for (int q = 0; q < DM; ++q) es[q] = ldv(a.tab.edge, (size_t)n * DM + q);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int k = 64 * s + lane_id();
+        if (64 * s >= L) break;
+        const bool wet = in_layer(nr, k);
+        const size_t i = b0 + k;
+        double Ti = 0.0;
+        if (k < L) Ti = T[i];
+        if (!wet) {
+            if (k < L) Tn[i] = Ti + 0.0;
+            continue;
+        }
+        const double A = ab2(To[i], Ti);
+        const size_t im = k > 0 ? i - 1 : i;
+        const double Am = ab2(To[im], T[im]);
+        const bool below = in_layer(nr, k + 1);
+        const double Ap = below ? ab2(To[i + 1], T[i + 1]) : 0.0;
+        double vf[DM], A2[DM];
+#pragma unroll
+        for (int q = 0; q < DM; ++q) {
+            vf[q] = G(a.vflux)[(size_t)(es[q].x >> 1) * L + k];
+            const size_t r = (size_t)es[q].y * L + k;
+            A2[q] = ab2(To[r], T[r]);
+        }
+        double raw = 0.0;
+#pragma unroll
+        for (int q = 0; q < DM; ++q) {
+            const double v = vf[q];
+            const bool side = es[q].x & 1;
+            const double T1 = side ? A2[q] : A;
+            const double T2 = side ? A : A2[q];
+            const double av = fabs(v);
+            const double f = -0.5 * (T1 * (v + av) + T2 * (v - av));
+            const bool use = q < cnt.x && v != 0.0;
+            raw = raw + (use ? (side ? -f : f) : 0.0);
+        }
+        double fv0, fv1 = 0.0;
+        {
+            const double wk = G(a.w_e)[i], aw = fabs(wk);
+            fv0 = (-0.5 * (A * (wk + aw) + Am * (wk - aw))) * G(m.area)[i];
+        }
+        if (below) {
+            const double wk = G(a.w_e)[i + 1], aw = fabs(wk);
+            fv1 = (-0.5 * (Ap * (wk + aw) + A * (wk - aw))) * G(m.area)[i + 1];
+        }
+        const double av = G(m.areasvol)[i], hn = G(a.hnn)[i], hnd = G(a.hnode)[i];
+        const double safe_a = av > 0.0 ? av : 1.0, safe_hn = hn > 0.0 ? hn : 1.0;
+        const double dttf_v = (fv0 - fv1) * a.dt / safe_a;
+        const double dttf_h = raw * a.dt / safe_a;
+        double del = dttf_h + dttf_v;
+        del = del + Ti * (hnd - hn);
+        const double dT = hn > 0.0 ? del / safe_hn : 0.0;
+        Tn[i] = Ti + dT;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // kernels: single instance (descriptor by value) and batched (descriptor array)
 // ---------------------------------------------------------------------------
 // Occupancy targets (wavefronts per SIMD) per kernel: the most the compiler reaches
@@ -755,6 +829,7 @@ __global__ void FCT_OCC(6) kb_adfh(const FusedArgs* __restrict__ d,
         PASS##_task<NS, DM>(a, n, w - n * a.K);                                         \
     }
 
+FCT_NODE_KERNELS(upwind, 5)
 FCT_NODE_KERNELS(pass1, 5)
 FCT_NODE_KERNELS(pass2, 5)
 FCT_NODE_KERNELS(pass3, 6)
@@ -837,6 +912,40 @@ int launch_fused(const DevMesh& m, const NodeTab& tab, const int2* ekr, const fc
     if (tab.dm == 8 && m.L <= 64) return launch_fused_t<1, 8>(fa, a, kw, tune, s);
     if (tab.dm == 8 && m.L <= 128) return launch_fused_t<2, 8>(fa, a, kw, tune, s);
     return set_error(FCT_E_UNSUPPORTED, "fused path supports node degree <= 8 and nl <= 128");
+}
+
+template <int NS, int DM>
+static int launch_upwind_t(FusedArgs& fa, const fct_step_args* a, const FusedTuning& t,
+                           hipStream_t s) {
+    const DevMesh& m = fa.m;
+    fa.K = 1;
+    if (m.D) launch_single<NS, DM>(fa, t, s, 0);
+    int rc = check_launch("kf_vflux");
+    if (rc) return rc;
+    for (int j0 = 0; j0 < a->ntracers; j0 += FCT_KCHUNK) {
+        const int K = (a->ntracers - j0) < FCT_KCHUNK ? (a->ntracers - j0) : FCT_KCHUNK;
+        fa.K = K;
+        for (int j = 0; j < FCT_KCHUNK; ++j) {
+            const int jj = j0 + (j < K ? j : 0);
+            fa.T[j] = a->T[jj]; fa.Told[j] = a->T_old[jj]; fa.Tnew[j] = a->T_new[jj];
+        }
+        const unsigned gn = blocks_for(node_waves(m.N, K), t.wpb);
+        hipLaunchKernelGGL((kf_upwind<NS, DM>), dim3(gn), dim3(64 * t.wpb), 0, s, fa);
+        if ((rc = check_launch("kf_upwind"))) return rc;
+    }
+    return FCT_OK;
+}
+
+int launch_upwind(const DevMesh& m, const NodeTab& tab, const int2* ekr, const fct_step_args* a,
+                  const FusedTuning& tune, double* vflux, hipStream_t s) {
+    if (!tuning_ok(tune)) return set_error(FCT_E_INVALID, "bad launch-shape knobs");
+    FusedArgs fa;
+    base_args(fa, m, tab, ekr, a, tune, vflux, nullptr, nullptr, nullptr, nullptr, nullptr);
+    if (tab.dm == 6 && m.L <= 64) return launch_upwind_t<1, 6>(fa, a, tune, s);
+    if (tab.dm == 6 && m.L <= 128) return launch_upwind_t<2, 6>(fa, a, tune, s);
+    if (tab.dm == 8 && m.L <= 64) return launch_upwind_t<1, 8>(fa, a, tune, s);
+    if (tab.dm == 8 && m.L <= 128) return launch_upwind_t<2, 8>(fa, a, tune, s);
+    return set_error(FCT_E_UNSUPPORTED, "upwind path supports node degree <= 8 and nl <= 128");
 }
 
 size_t fused_args_size() { return sizeof(FusedArgs); }

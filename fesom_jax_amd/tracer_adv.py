@@ -189,7 +189,10 @@ def _run(mesh, uv, w_e, helem, hnode, hnode_new, Ts, T_olds, dt, exch, Z3d, zbar
         zbar3=None if zb3_t is None else zb3_t.data_ptr())
     with torch.cuda.device(device):
         stream = torch.cuda.current_stream(device).cuda_stream
-        if exch is None:
+        if path == _UPWIND:
+            _lib.check(plan.lib.fct_forward_upwind(plan.ptr, ctypes.byref(args), c_void_p(stream)),
+                       "fct_forward_upwind")
+        elif exch is None:
             _lib.check(plan.lib.fct_forward_ex(plan.ptr, ctypes.byref(args), int(path),
                                                c_void_p(stream)), "fct_forward")
         else:
@@ -239,9 +242,21 @@ def _run_phased(plan, args, j, exch, device, N, E, L, stream):
                "phase FINAL")
 
 
+_UPWIND = -1   # _run path code of the upwind-only step
+
+
 # --------------------------------------------------------------------------
 # public entry points
 # --------------------------------------------------------------------------
+def advect_one(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, *, dt: float = DT_DEFAULT):
+    """One tracer's first-order upwind advection of the AB2 field + ALE reconstruction;
+    returns ``(T_new, T)`` (fesom_jax/tracer_adv.py:154-165)."""
+    kind = _kind_of(T)
+    out = _run(mesh, uv, w_e, helem, hnode, hnode_new, [T], [T_old], dt, None, None, None,
+               None, _UPWIND)
+    return _from_device(out[0], kind, T), T
+
+
 def advect_one_fct(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, *,
                    dt: float = DT_DEFAULT, exch=None, Z3d=None, zbar3=None, w_i=None):
     """One tracer's FCT advection + ALE reconstruction; returns ``(T_new, T)``

@@ -169,6 +169,12 @@ int fct_forward_ex(fct_plan* plan, const fct_step_args* args, int32_t path, void
 int fct_forward_phase(fct_plan* plan, const fct_step_args* args, int32_t tracer,
                       int32_t phase, const fct_phase_bufs* bufs, void* stream);
 
+/* Upwind-only step (the reference's other driver, fesom_jax/tracer_adv.py:154-165):
+ * T_new[k] = advect_one(mesh, uv, w_e, helem, hnode, hnode_new, T[k], T_old[k], dt=dt)[0],
+ * first-order upwind fluxes of the AB2 field, no limiter. w_i, Z3d and zbar3 must be
+ * NULL. Needs a mesh that takes the fused path (FCT_PATH_BIT_FUSED). */
+int fct_forward_upwind(fct_plan* plan, const fct_step_args* args, void* stream);
+
 /* ---- reverse mode ------------------------------------------------------------------
  * Cotangents of one FCT step: for the cotangents g_T_new[j] of T_new[j], returns
  * g_T[j], g_T_old[j] and (summed over the tracers) g_uv and g_w_e, with the conventions
