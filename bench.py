@@ -93,10 +93,11 @@ def main():
     ap.add_argument("--dt", type=float, default=2700.0)
     ap.add_argument("--cpu-level", type=int, default=7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("plain", "batched", "serve", "adjoint"),
+    ap.add_argument("--workload", choices=("plain", "batched", "serve", "adjoint", "vdiff"),
                     default="plain",
                     help="plain: one icosa-9 instance (the metric); batched: 64 uneven "
-                         "instances in one batched launch vs 64 sequential plain launches")
+                         "instances in one batched launch vs 64 sequential plain launches; "
+                         "vdiff: implicit vertical diffusion of T and S on the plain mesh")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, run a 1 GiB read + 1 GiB write elementwise "
@@ -128,6 +129,8 @@ def main():
         return run_serve(args, dev)
     if args.workload == "adjoint":
         return run_adjoint(args, dev)
+    if args.workload == "vdiff":
+        return run_vdiff(args, dev)
     K = args.tracers
 
     t0 = time.time()
@@ -445,6 +448,88 @@ def run_adjoint(args, dev):
                      "median_rel": float(np.median(rel)),
                      "max_rel": float(rel.max()), "below_1e-5": int((rel < 1e-5).sum())},
         "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_vdiff(args, dev):
+    """Implicit vertical diffusion (the step after the FCT) of T and S with one Kv on the
+    plain workload's mesh, in place, repeated. Algorithmic bytes per call: T and S read
+    and written in full (2 x 2 x 8 B per node-level: dry levels are copied), Kv, area,
+    areasvol and hnode_new read on the wet layers plus the interface below
+    (4 x 8 B per such entry)."""
+    import torch
+    from fesom_jax_amd.inputs import make_case_device
+    from fesom_jax_amd.meshgen import icosahedral_mesh
+    from fesom_jax_amd.perfmodel import HBM_PEAK_GBS
+    from fesom_jax_amd.tracer_diff import diffuse_tracers
+    K = 2
+    mesh = icosahedral_mesh(args.level, args.nl, seed=args.seed)
+    case = make_case_device(mesh, dev, seed=args.seed + 1, dt=args.dt, tracers=("T", "S"))
+    g = torch.Generator(device=dev).manual_seed(args.seed + 3)
+    iface = torch.from_numpy(mesh.node_iface_mask).to(dev)
+    Kv = torch.where(iface, 1e-5 + 3e-2 * torch.rand((mesh.nod2D, mesh.nl), generator=g,
+                                                    dtype=torch.float64, device=dev),
+                     torch.zeros((), dtype=torch.float64, device=dev))
+    Ts = [case["T"][0].clone(), case["T"][1].clone()]
+    hnn = case["hnode_new"]
+    del case
+    wet = mesh.node_layer_mask
+    n_wet = int(wet.sum())
+    cols_wet = int(wet.any(axis=1).sum())
+    U = K * n_wet
+    B = 8 * (2 * K * mesh.nod2D * mesh.nl + 4 * (n_wet + cols_wet))
+    stream = torch.cuda.current_stream(dev)
+
+    def call():
+        diffuse_tracers(mesh, Ts, Kv, hnn, dt=args.dt, floor_tracer=1, out=Ts)
+
+    for _ in range(max(1, args.warmup)):
+        call()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(args.steps):
+        call()
+    end.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / args.steps
+    m = torch.from_numpy(wet).to(dev)
+    if not bool(torch.isfinite(Ts[0][m]).all()):
+        raise RuntimeError("non-finite tracer after the timed calls")
+    achieved = B / (ms * 1e-3) / 1e9
+    log(f"vdiff icosa-{args.level}: {ms:.3f} ms/call, {achieved:.0f} GB/s algorithmic")
+    cpu = None
+    if not args.no_cpu_baseline:
+        from fesom_jax_amd.inputs import make_case
+        from oracle import vert_diff as VD
+        cm = icosahedral_mesh(args.cpu_level, args.nl, seed=args.seed)
+        cc = make_case(cm, seed=args.seed + 1, dt=args.dt)
+        rng = np.random.default_rng(args.seed + 3)
+        ckv = np.where(cm.node_iface_mask, 1e-5 + 3e-2 * rng.random((cm.nod2D, cm.nl)), 0.0)
+        reps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 10.0 or reps < 1:
+            VD.impl_vert_diff(cm, cc["T"][0], cc["T"][1], ckv, cc["hnode_new"], dt=args.dt)
+            reps += 1
+        sec = (time.perf_counter() - t0) / reps
+        cu = 2 * int(cm.node_layer_mask.sum())
+        cpu = {"value": cu / sec, "unit": "node-level updates/s", "cores": 1, "kind": "port",
+               "sample": f"numpy oracle impl_vert_diff, icosa-{args.cpu_level} x nl={cm.nl}, "
+                         f"{reps} calls in {sec * reps:.1f} s on {cpu_model()}"}
+    out = {
+        "metric": "FP64 implicit vertical diffusion node-level updates/s",
+        "value": U / (ms * 1e-3), "unit": "node-level updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"icosa-{args.level} x nl={mesh.nl}, T and S, impl_vert_diff "
+                               f"+ salinity floor, in place", "nodes": mesh.nod2D,
+                   "levels": mesh.nl, "updates_per_call": U, "dt": args.dt},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": None, "kernel": "k_vdiff", "bytes_per_call": B},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

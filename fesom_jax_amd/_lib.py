@@ -31,6 +31,7 @@ FCT_TUNE_WAVES_PER_GROUP = 2
 FCT_TUNE_VFLUX_EDGES = 3
 FCT_TUNE_ADFH_EDGES = 4
 FCT_BATCH_SLOTS = 4
+FCT_VDIFF_MAX_TRACERS = 8
 
 FCT_PHASE_LO = 1
 FCT_PHASE_GRAD = 2
@@ -42,7 +43,7 @@ EXPORTED_SYMBOLS = ("fct_abi_version", "fct_last_error", "fct_plan_create",
                     "fct_forward_ex", "fct_forward_phase", "fct_plan_set_tuning",
                     "fct_batch_create", "fct_batch_set_tuning", "fct_batch_bind",
                     "fct_batch_launch", "fct_batch_graph_launch", "fct_batch_destroy",
-                    "fct_backward", "fct_forward_upwind")
+                    "fct_backward", "fct_forward_upwind", "fct_impl_vert_diff")
 
 
 class FctMeshDesc(ctypes.Structure):
@@ -70,6 +71,13 @@ class FctStepArgs(ctypes.Structure):
 class FctAdjointArgs(ctypes.Structure):
     _fields_ = [("g_T_new", POINTER(c_void_p)), ("g_T", POINTER(c_void_p)),
                 ("g_T_old", POINTER(c_void_p)), ("g_uv", c_void_p), ("g_w_e", c_void_p)]
+
+
+class FctVdiffArgs(ctypes.Structure):
+    _fields_ = [("ntracers", c_int32), ("floor_tracer", c_int32), ("dt", c_double),
+                ("floor_value", c_double), ("Kv", c_void_p), ("hnode_new", c_void_p),
+                ("Z3d", c_void_p), ("T", POINTER(c_void_p)), ("bc_surf", POINTER(c_void_p)),
+                ("sw_3d", POINTER(c_void_p)), ("T_new", POINTER(c_void_p))]
 
 
 class FctPhaseBufs(ctypes.Structure):
@@ -132,6 +140,8 @@ def load(path: Path | None = None, *, require_torch: bool = True):
     lib.fct_batch_destroy.argtypes = [c_void_p]
     lib.fct_forward_upwind.restype = c_int32
     lib.fct_forward_upwind.argtypes = [c_void_p, POINTER(FctStepArgs), c_void_p]
+    lib.fct_impl_vert_diff.restype = c_int32
+    lib.fct_impl_vert_diff.argtypes = [c_void_p, POINTER(FctVdiffArgs), c_void_p]
     lib.fct_backward.restype = c_int32
     lib.fct_backward.argtypes = [c_void_p, POINTER(FctStepArgs), POINTER(FctAdjointArgs),
                                  c_void_p]

@@ -64,6 +64,9 @@ struct fct_plan {
     bool adj_ready = false;
     AdjointWork aw{};
     std::vector<int> host_edge_tri, host_updn;   // [2D] each, -1 sentinels (adjoint lists)
+    // implicit vertical diffusion scratch (allocated on the first call)
+    double* vd_scratch = nullptr;
+    int vd_grid = 0;
 };
 
 namespace {
@@ -619,6 +622,53 @@ int fct_forward_upwind(fct_plan* p, const fct_step_args* a, void* stream) {
     if (p->tab.dm == 0)
         return set_error(FCT_E_UNSUPPORTED, "this mesh cannot take the node-table kernels");
     return launch_upwind(p->m, p->tab, p->ekr, a, p->tune, p->vflux, (hipStream_t)stream);
+}
+
+int fct_impl_vert_diff(fct_plan* p, const fct_vdiff_args* a, void* stream) {
+    if (!p || !a) return set_error(FCT_E_INVALID, "NULL plan or args");
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != p->device)
+        return set_error(FCT_E_INVALID, "current device %d is not the plan's device %d", cur,
+                         p->device);
+    const int K = a->ntracers;
+    if (K < 1 || K > FCT_VDIFF_MAX_TRACERS)
+        return set_error(FCT_E_INVALID, "ntracers %d outside 1..%d", K, FCT_VDIFF_MAX_TRACERS);
+    if (!a->Kv || !a->hnode_new || !a->T || !a->T_new)
+        return set_error(FCT_E_INVALID, "NULL Kv, hnode_new, T or T_new");
+    if (a->floor_tracer < -1 || a->floor_tracer >= K)
+        return set_error(FCT_E_INVALID, "floor_tracer %d outside -1..%d", a->floor_tracer, K - 1);
+    for (int j = 0; j < K; ++j)
+        if (!a->T[j] || !a->T_new[j])
+            return set_error(FCT_E_INVALID, "tracer %d has a NULL field", j);
+    if (!p->vd_scratch) {
+        int cus = 0;
+        HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
+        int grid = 0, rc;
+        if ((rc = vdiff_max_grid(cus, &grid))) return rc;
+        if ((rc = dev_alloc(p, &p->vd_scratch, vdiff_scratch_doubles(p->m.L, grid)))) return rc;
+        p->vd_grid = grid;
+    }
+    for (int j0 = 0; j0 < K; j0 += 2) {
+        VdiffCall c;
+        c.K = std::min(2, K - j0);
+        c.dt = a->dt;
+        c.Kv = a->Kv;
+        c.hnn = a->hnode_new;
+        c.Z3d = a->Z3d;
+        for (int j = 0; j < c.K; ++j) {
+            c.T[j] = a->T[j0 + j];
+            c.Tn[j] = a->T_new[j0 + j];
+            c.bc[j] = a->bc_surf ? a->bc_surf[j0 + j] : nullptr;
+            c.sw[j] = a->sw_3d ? a->sw_3d[j0 + j] : nullptr;
+        }
+        if (a->floor_tracer >= j0 && a->floor_tracer < j0 + c.K) {
+            c.floor_j = a->floor_tracer - j0;
+            c.floor_v = a->floor_value;
+        }
+        const int rc = launch_vdiff(p->m, c, p->vd_grid, p->vd_scratch, (hipStream_t)stream);
+        if (rc) return rc;
+    }
+    return FCT_OK;
 }
 
 int fct_forward_phase(fct_plan* p, const fct_step_args* a, int32_t j, int32_t phase,

@@ -149,4 +149,24 @@ inline int64_t adfh_waves(int D, int K, const FusedTuning& t) {
     return (int64_t)((D + t.adfh_epw - 1) / t.adfh_epw) * K;
 }
 
+// Implicit vertical diffusion of K <= 2 tracers sharing one matrix (fct_vdiff.hip).
+struct VdiffCall {
+    int K = 0;
+    double dt = 0.0;
+    const double* Kv = nullptr;
+    const double* hnn = nullptr;
+    const double* Z3d = nullptr;
+    const double* T[2] = {nullptr, nullptr};
+    const double* bc[2] = {nullptr, nullptr};
+    const double* sw[2] = {nullptr, nullptr};
+    double* Tn[2] = {nullptr, nullptr};
+    int floor_j = -1;               // index within the call, -1 = none
+    double floor_v = 0.0;
+};
+// Workgroups the GPU holds at once over every kernel variant (cus compute units).
+int vdiff_max_grid(int cus, int* out);
+size_t vdiff_scratch_doubles(int L, int grid);
+int launch_vdiff(const DevMesh& m, const VdiffCall& c, int max_grid, double* scratch,
+                 hipStream_t s);
+
 }  // namespace fct

@@ -175,6 +175,33 @@ int fct_forward_phase(fct_plan* plan, const fct_step_args* args, int32_t tracer,
  * NULL. Needs a mesh that takes the fused path (FCT_PATH_BIT_FUSED). */
 int fct_forward_upwind(fct_plan* plan, const fct_step_args* args, void* stream);
 
+/* ---- implicit vertical diffusion ---------------------------------------------------
+ * The step that follows the FCT in the reference (fesom_jax/step.py:452-456):
+ *   T_new[j] = impl_vert_diff_one(mesh, T[j], Kv, hnode_new, dt=dt,
+ *                                 bc_surf=bc_surf[j], sw_3d=sw_3d[j], Z3d=Z3d)
+ * (fesom_jax/tracer_diff.py:68-128; impl_vert_diff, :131-141, is the two-tracer call
+ * with sw_3d on the first tracer only). One tridiagonal solve per node column, the
+ * same matrix for every tracer of the call. Optionally clamps one tracer from below on
+ * wet layers, the salinity floor of fesom_jax/step.py:456. The first call allocates a
+ * scratch bounded by the GPU's resident workgroups (3 * nl * 64 doubles each), later
+ * calls allocate nothing. T_new[j] may be T[j] (in place). */
+#define FCT_VDIFF_MAX_TRACERS 8
+typedef struct fct_vdiff_args {
+    int32_t ntracers;                 /* 1 .. FCT_VDIFF_MAX_TRACERS */
+    int32_t floor_tracer;             /* tracer clamped to >= floor_value on wet layers; -1: none */
+    double dt;
+    double floor_value;
+    const double* Kv;                 /* [nod2D][nl] interface diffusivity */
+    const double* hnode_new;          /* [nod2D][nl] */
+    const double* Z3d;                /* [nod2D][nl] layer depths, or NULL: the mesh's Z */
+    const double* const* T;           /* ntracers x [nod2D][nl] */
+    const double* const* bc_surf;     /* NULL, or ntracers x ([nod2D] or NULL) */
+    const double* const* sw_3d;       /* NULL, or ntracers x ([nod2D][nl] or NULL) */
+    double* const* T_new;             /* out: ntracers x [nod2D][nl] */
+} fct_vdiff_args;
+
+int fct_impl_vert_diff(fct_plan* plan, const fct_vdiff_args* args, void* stream);
+
 /* ---- reverse mode ------------------------------------------------------------------
  * Cotangents of one FCT step: for the cotangents g_T_new[j] of T_new[j], returns
  * g_T[j], g_T_old[j] and (summed over the tracers) g_uv and g_w_e, with the conventions
