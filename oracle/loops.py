@@ -1,0 +1,62 @@
+"""ctypes front end of the C loop oracle (oracle/fct_loops.c), test infrastructure only:
+one FCT tracer step with the original C kernel's loop semantics, as restated by the
+reference's loop oracle (fesom_jax/tests/test_tracers.py:325-516)."""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_double, c_int
+
+import numpy as np
+
+from . import build as _build
+
+_INT_FIELDS = ("edges", "edge_tri", "edge_up_dn_tri", "elem_nodes", "ulevels", "nlevels",
+               "ulevels_nod2D", "nlevels_nod2D", "ulevels_nod2D_max", "nlevels_nod2D_min",
+               "nod_in_elem2D_offsets", "nod_in_elem2D")
+_F64_FIELDS = ("edge_cross_dxdy", "edge_dxdy", "gradient_sca", "elem_area", "elem_cos",
+               "area", "areasvol", "Z", "zbar")
+
+
+class _LoopMesh(ctypes.Structure):
+    _fields_ = ([("nod2D", c_int), ("elem2D", c_int), ("edge2D", c_int), ("nl", c_int)]
+                + [(f, POINTER(c_int)) for f in _INT_FIELDS]
+                + [(f, POINTER(c_double)) for f in _F64_FIELDS])
+
+
+_LIB = None
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = ctypes.CDLL(str(_build.ensure()))
+        _LIB.fct_loops_step.restype = c_int
+        _LIB.fct_loops_step.argtypes = [POINTER(_LoopMesh)] + [POINTER(c_double)] * 7 + [
+            c_double, POINTER(c_double)]
+    return _LIB
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def advect_one_fct(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, *, dt):
+    """T_new of one FCT step (same arguments as the reference's advect_one_fct)."""
+    keep = {}
+    kw = {}
+    for f in _INT_FIELDS:
+        keep[f] = np.ascontiguousarray(getattr(mesh, f), dtype=np.int32)
+        kw[f] = keep[f].ctypes.data_as(POINTER(c_int))
+    for f in _F64_FIELDS:
+        keep[f] = _f64(getattr(mesh, f))
+        kw[f] = keep[f].ctypes.data_as(POINTER(c_double))
+    m = _LoopMesh(nod2D=int(mesh.nod2D), elem2D=int(mesh.elem2D), edge2D=int(mesh.edge2D),
+                  nl=int(mesh.nl), **kw)
+    args = [_f64(x) for x in (uv, w_e, helem, hnode, hnode_new, T, T_old)]
+    out = np.empty((int(mesh.nod2D), int(mesh.nl)))
+    P = lambda a: a.ctypes.data_as(POINTER(c_double))
+    rc = _lib().fct_loops_step(ctypes.byref(m), *[P(a) for a in args], float(dt), P(out))
+    if rc != 0:
+        raise MemoryError("fct_loops_step: allocation failed")
+    return out
