@@ -101,11 +101,13 @@ def main():
     ap.add_argument("--dt", type=float, default=2700.0)
     ap.add_argument("--cpu-level", type=int, default=7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("plain", "batched", "serve", "adjoint", "vdiff"),
+    ap.add_argument("--workload",
+                    choices=("plain", "batched", "serve", "adjoint", "vdiff", "compute_w"),
                     default="plain",
                     help="plain: one icosa-9 instance (the metric); batched: 64 uneven "
                          "instances in one batched launch vs 64 sequential plain launches; "
-                         "vdiff: implicit vertical diffusion of T and S on the plain mesh")
+                         "vdiff: implicit vertical diffusion of T and S on the plain mesh; "
+                         "compute_w: vertical velocity from continuity on the plain mesh")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, run a 1 GiB read + 1 GiB write elementwise "
@@ -139,6 +141,8 @@ def main():
         return run_adjoint(args, dev)
     if args.workload == "vdiff":
         return run_vdiff(args, dev)
+    if args.workload == "compute_w":
+        return run_compute_w(args, dev)
     K = args.tracers
 
     t0 = time.time()
@@ -538,6 +542,55 @@ def run_vdiff(args, dev):
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": None, "kernel": "k_vdiff", "bytes_per_call": B},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_compute_w(args, dev):
+    """Vertical velocity by continuity (the w the step takes) on the plain workload's
+    mesh. Algorithmic bytes per call: uv and helem on wet element layers (24 B each),
+    area on wet node interfaces (8 B), w written in full (8 B per node-level), and the
+    edge tables (edge triangles, cross vectors, CSR entries: 56 B per edge)."""
+    import torch
+    from fesom_jax_amd.ale import compute_w
+    from fesom_jax_amd.inputs import make_case_device
+    from fesom_jax_amd.meshgen import icosahedral_mesh
+    from fesom_jax_amd.perfmodel import HBM_PEAK_GBS
+    mesh = icosahedral_mesh(args.level, args.nl, seed=args.seed)
+    case = make_case_device(mesh, dev, seed=args.seed + 1, dt=args.dt, tracers=("T",))
+    uv, helem = case["uv"], case["helem"]
+    del case
+    n_le = int(mesh.elem_layer_mask.sum())
+    n_in = int(mesh.node_iface_mask.sum())
+    B = 24 * n_le + 8 * n_in + 8 * mesh.nod2D * mesh.nl + 56 * mesh.edge2D
+    stream = torch.cuda.current_stream(dev)
+    for _ in range(max(1, args.warmup)):
+        w = compute_w(mesh, uv, helem)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(args.steps):
+        w = compute_w(mesh, uv, helem)
+    end.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / args.steps
+    if not bool(torch.isfinite(w).all()):
+        raise RuntimeError("non-finite w")
+    achieved = B / (ms * 1e-3) / 1e9
+    log(f"compute_w icosa-{args.level}: {ms:.3f} ms/call, {achieved:.0f} GB/s algorithmic")
+    out = {
+        "metric": "FP64 compute_w node-interface values/s",
+        "value": n_in / (ms * 1e-3), "unit": "node interfaces/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"icosa-{args.level} x nl={mesh.nl}, compute_w",
+                   "nodes": mesh.nod2D, "levels": mesh.nl, "interfaces": n_in},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": None, "kernel": "k_compute_w", "bytes_per_call": B},
+        "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 

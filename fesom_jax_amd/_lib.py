@@ -43,7 +43,8 @@ EXPORTED_SYMBOLS = ("fct_abi_version", "fct_last_error", "fct_plan_create",
                     "fct_forward_ex", "fct_forward_phase", "fct_plan_set_tuning",
                     "fct_batch_create", "fct_batch_set_tuning", "fct_batch_bind",
                     "fct_batch_launch", "fct_batch_graph_launch", "fct_batch_destroy",
-                    "fct_backward", "fct_forward_upwind", "fct_impl_vert_diff")
+                    "fct_backward", "fct_forward_upwind", "fct_impl_vert_diff",
+                    "fct_compute_w")
 
 
 class FctMeshDesc(ctypes.Structure):
@@ -140,6 +141,8 @@ def load(path: Path | None = None, *, require_torch: bool = True):
     lib.fct_batch_destroy.argtypes = [c_void_p]
     lib.fct_forward_upwind.restype = c_int32
     lib.fct_forward_upwind.argtypes = [c_void_p, POINTER(FctStepArgs), c_void_p]
+    lib.fct_compute_w.restype = c_int32
+    lib.fct_compute_w.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
     lib.fct_impl_vert_diff.restype = c_int32
     lib.fct_impl_vert_diff.argtypes = [c_void_p, POINTER(FctVdiffArgs), c_void_p]
     lib.fct_backward.restype = c_int32

@@ -624,6 +624,16 @@ int fct_forward_upwind(fct_plan* p, const fct_step_args* a, void* stream) {
     return launch_upwind(p->m, p->tab, p->ekr, a, p->tune, p->vflux, (hipStream_t)stream);
 }
 
+int fct_compute_w(fct_plan* p, const double* uv, const double* helem, double* w,
+                  void* stream) {
+    if (!p || !uv || !helem || !w) return set_error(FCT_E_INVALID, "NULL plan or field");
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != p->device)
+        return set_error(FCT_E_INVALID, "current device %d is not the plan's device %d", cur,
+                         p->device);
+    return launch_compute_w(p->m, uv, helem, w, (hipStream_t)stream);
+}
+
 int fct_impl_vert_diff(fct_plan* p, const fct_vdiff_args* a, void* stream) {
     if (!p || !a) return set_error(FCT_E_INVALID, "NULL plan or args");
     int cur = -1;

@@ -149,6 +149,10 @@ inline int64_t adfh_waves(int D, int K, const FusedTuning& t) {
     return (int64_t)((D + t.adfh_epw - 1) / t.adfh_epw) * K;
 }
 
+// Vertical velocity from continuity (fct_ale.hip).
+int launch_compute_w(const DevMesh& m, const double* uv, const double* helem, double* w,
+                     hipStream_t s);
+
 // Implicit vertical diffusion of K <= 2 tracers sharing one matrix (fct_vdiff.hip).
 struct VdiffCall {
     int K = 0;

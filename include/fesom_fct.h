@@ -175,6 +175,14 @@ int fct_forward_phase(fct_plan* plan, const fct_step_args* args, int32_t tracer,
  * NULL. Needs a mesh that takes the fused path (FCT_PATH_BIT_FUSED). */
 int fct_forward_upwind(fct_plan* plan, const fct_step_args* args, void* stream);
 
+/* ---- vertical velocity ---------------------------------------------------------------
+ * w = compute_w(mesh, uv, helem) (fesom_jax/ale.py:309-365), the w_e the step takes:
+ * per-level edge divergence into the nodes, cumulative sum from the bottom, divided by
+ * the interface area; zero outside each node's interfaces [nzmin, nzmax]. uv is
+ * [elem2D][nl][2], helem [elem2D][nl], w [nod2D][nl] (device). nl <= 128. */
+int fct_compute_w(fct_plan* plan, const double* uv, const double* helem, double* w,
+                  void* stream);
+
 /* ---- implicit vertical diffusion ---------------------------------------------------
  * The step that follows the FCT in the reference (fesom_jax/step.py:452-456):
  *   T_new[j] = impl_vert_diff_one(mesh, T[j], Kv, hnode_new, dt=dt,

@@ -357,3 +357,44 @@ int fct_loops_step(const loop_mesh* m, const double* uv, const double* w, const 
     free(qmax); free(qmin); free(sp); free(sm); free(rp); free(rm); free(inc);
     return 0;
 }
+
+/* Vertical velocity by continuity with the original C kernel's loops
+ * (fesom_ale_vert_vel_linfs as described at fesom_jax/ale.py:309-336): per edge and
+ * wet triangle level, c = (v dx - u dy) h of the left triangle minus that of the right
+ * one, added to the first node and subtracted from the second; then, per node from the
+ * level above the bottom interface up to the surface, w[k] += w[k+1]; then / area where
+ * area > 0. Interfaces outside [nzmin, nzmax] are zero. */
+int loops_compute_w(const loop_mesh* m, const double* uv, const double* helem, double* w) {
+    const int N = m->nod2D, D = m->edge2D, nl = m->nl;
+    memset(w, 0, (size_t)N * nl * sizeof(double));
+    for (int e = 0; e < D; ++e) {
+        const int a = m->edges[2 * e], b = m->edges[2 * e + 1];
+        const int t1 = m->edge_tri[2 * e], t2 = m->edge_tri[2 * e + 1];
+        const double* cr = m->edge_cross_dxdy + 4 * (size_t)e;
+        for (int k = 0; k < nl; ++k) {
+            double c1 = 0.0, c2 = 0.0;
+            if (t1 >= 0 && k >= m->ulevels[t1] - 1 && k < m->nlevels[t1] - 1) {
+                const size_t i = (size_t)t1 * nl + k;
+                c1 = (uv[2 * i + 1] * cr[0] - uv[2 * i] * cr[1]) * helem[i];
+            }
+            if (t2 >= 0 && k >= m->ulevels[t2] - 1 && k < m->nlevels[t2] - 1) {
+                const size_t i = (size_t)t2 * nl + k;
+                c2 = (uv[2 * i + 1] * cr[2] - uv[2 * i] * cr[3]) * helem[i];
+            }
+            const double c = c1 - c2;
+            w[(size_t)a * nl + k] += c;
+            w[(size_t)b * nl + k] -= c;
+        }
+    }
+    for (int n = 0; n < N; ++n) {
+        const int top = m->ulevels_nod2D[n] - 1, bot = m->nlevels_nod2D[n] - 1;
+        double* col = w + (size_t)n * nl;
+        for (int k = bot - 1; k >= top; --k) col[k] += col[k + 1];
+        for (int k = 0; k < nl; ++k) {
+            if (k < top || k > bot) { col[k] = 0.0; continue; }
+            const double ar = m->area[(size_t)n * nl + k];
+            if (ar > 0.0) col[k] = col[k] / ar;
+        }
+    }
+    return 0;
+}

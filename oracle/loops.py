@@ -34,6 +34,8 @@ def _lib():
         _LIB.fct_loops_step.restype = c_int
         _LIB.fct_loops_step.argtypes = [POINTER(_LoopMesh)] + [POINTER(c_double)] * 7 + [
             c_double, POINTER(c_double)]
+        _LIB.loops_compute_w.restype = c_int
+        _LIB.loops_compute_w.argtypes = [POINTER(_LoopMesh)] + [POINTER(c_double)] * 3
     return _LIB
 
 
@@ -41,8 +43,7 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
-def advect_one_fct(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, *, dt):
-    """T_new of one FCT step (same arguments as the reference's advect_one_fct)."""
+def _mesh_struct(mesh):
     keep = {}
     kw = {}
     for f in _INT_FIELDS:
@@ -53,10 +54,26 @@ def advect_one_fct(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, *, dt):
         kw[f] = keep[f].ctypes.data_as(POINTER(c_double))
     m = _LoopMesh(nod2D=int(mesh.nod2D), elem2D=int(mesh.elem2D), edge2D=int(mesh.edge2D),
                   nl=int(mesh.nl), **kw)
+    return m, keep
+
+
+def advect_one_fct(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, *, dt):
+    """T_new of one FCT step (same arguments as the reference's advect_one_fct)."""
+    m, keep = _mesh_struct(mesh)
     args = [_f64(x) for x in (uv, w_e, helem, hnode, hnode_new, T, T_old)]
     out = np.empty((int(mesh.nod2D), int(mesh.nl)))
     P = lambda a: a.ctypes.data_as(POINTER(c_double))
     rc = _lib().fct_loops_step(ctypes.byref(m), *[P(a) for a in args], float(dt), P(out))
     if rc != 0:
         raise MemoryError("fct_loops_step: allocation failed")
+    return out
+
+
+def compute_w(mesh, uv, helem):
+    """w [nod2D, nl] with the original C kernel's loops (fesom_jax/ale.py:309-365)."""
+    m, keep = _mesh_struct(mesh)
+    uv, helem = _f64(uv), _f64(helem)
+    out = np.empty((int(mesh.nod2D), int(mesh.nl)))
+    P = lambda a: a.ctypes.data_as(POINTER(c_double))
+    _lib().loops_compute_w(ctypes.byref(m), P(uv), P(helem), P(out))
     return out

@@ -96,3 +96,27 @@ def test_planted_wrong_gradient_triangle_is_caught(pi_mesh, pi_case, monkeypatch
     monkeypatch.setattr(O, "fill_up_dn_grad", swapped)
     a, b = _pair(pi_mesh, pi_case)
     assert _rel(a, b, pi_mesh.node_layer_mask) > 1e-9
+
+
+def test_compute_w_matches_c_loops(pi_mesh, pi_case, icosa4_mesh, icosa4_case):
+    """compute_w (fesom_jax/ale.py:309-365) against the original C kernel's loops: edge
+    scatter per wet triangle level, w[k] += w[k+1] from the bottom, / area. Same
+    summation orders, so bit for bit."""
+    for m, c in ((pi_mesh, pi_case), (icosa4_mesh, icosa4_case)):
+        a = O.compute_w(m, c["uv"], c["helem"])
+        b = LP.compute_w(m, c["uv"], c["helem"])
+        assert np.array_equal(a, b)
+        assert np.abs(a).max() > 0
+
+
+def test_planted_compute_w_sign_is_caught(pi_mesh, pi_case, monkeypatch):
+    """Flipping the right triangle's sign in the oracle breaks the agreement."""
+    real_scatter = O.scatter_add
+
+    def flipped(vals, seg, num):
+        return real_scatter(np.stack([vals[:, 0], vals[:, 0]], axis=1), seg, num)
+
+    monkeypatch.setattr(O, "scatter_add", flipped)
+    a = O.compute_w(pi_mesh, pi_case["uv"], pi_case["helem"])
+    b = LP.compute_w(pi_mesh, pi_case["uv"], pi_case["helem"])
+    assert not np.allclose(a, b, rtol=1e-6, atol=0)
