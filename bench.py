@@ -102,12 +102,13 @@ def main():
     ap.add_argument("--cpu-level", type=int, default=7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload",
-                    choices=("plain", "batched", "serve", "adjoint", "vdiff", "compute_w"),
+                    choices=("plain", "batched", "serve", "adjoint", "vdiff", "compute_w", "ice"),
                     default="plain",
                     help="plain: one icosa-9 instance (the metric); batched: 64 uneven "
                          "instances in one batched launch vs 64 sequential plain launches; "
                          "vdiff: implicit vertical diffusion of T and S on the plain mesh; "
-                         "compute_w: vertical velocity from continuity on the plain mesh")
+                         "compute_w: vertical velocity from continuity on the plain mesh; "
+                         "ice: sea-ice FCT of the three ice tracers on the plain mesh")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, run a 1 GiB read + 1 GiB write elementwise "
@@ -143,6 +144,8 @@ def main():
         return run_vdiff(args, dev)
     if args.workload == "compute_w":
         return run_compute_w(args, dev)
+    if args.workload == "ice":
+        return run_ice(args, dev)
     K = args.tracers
 
     t0 = time.time()
@@ -593,6 +596,59 @@ def run_compute_w(args, dev):
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
+
+
+def run_ice(args, dev):
+    """Sea-ice FCT of a_ice, m_ice, m_snow on the plain workload's mesh (surface nodes),
+    ice_dt = 4000 s, velocity up to 0.5 m/s. Algorithmic bytes per call: the five input
+    and three output fields (64 B per node), node area and level range (16 B per node),
+    the element tables (vertices 12 B, gradients 48 B, area and its sqrt factor 16 B,
+    level range 8 B, node->element entries 12 B: 96 B per element)."""
+    import torch
+    from fesom_jax_amd.ice_adv import IceConfig, fct_solve
+    from fesom_jax_amd.inputs import make_ice_case
+    from fesom_jax_amd.meshgen import icosahedral_mesh
+    from fesom_jax_amd.perfmodel import HBM_PEAK_GBS
+    mesh = icosahedral_mesh(args.level, args.nl, seed=args.seed)
+    c = make_ice_case(mesh, seed=args.seed + 5, speed=0.5)
+    f = {k: torch.from_numpy(v).to(dev) for k, v in c.items()}
+    cfg = IceConfig(ice_dt=4000.0)
+    N, E = mesh.nod2D, mesh.elem2D
+    B = 80 * N + 96 * E
+    stream = torch.cuda.current_stream(dev)
+
+    def call():
+        return fct_solve(cfg, mesh, f["a_ice"], f["m_ice"], f["m_snow"], f["u_ice"], f["v_ice"])
+
+    for _ in range(max(1, args.warmup)):
+        out = call()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(args.steps):
+        out = call()
+    end.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / args.steps
+    if not all(bool(torch.isfinite(o).all()) for o in out):
+        raise RuntimeError("non-finite ice tracer")
+    achieved = B / (ms * 1e-3) / 1e9
+    log(f"ice FCT icosa-{args.level}: {ms:.3f} ms/call, {achieved:.0f} GB/s algorithmic")
+    result = {
+        "metric": "FP64 sea-ice FCT node updates/s (3 tracers per node)",
+        "value": 3 * N / (ms * 1e-3), "unit": "node-tracer updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"icosa-{args.level} surface, sea-ice FCT, ice_dt 4000 s",
+                   "nodes": N, "elements": E},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": None, "kernel": "k_ice_* (five launches)", "bytes_per_call": B},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
 
 
 if __name__ == "__main__":

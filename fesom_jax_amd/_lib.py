@@ -44,7 +44,7 @@ EXPORTED_SYMBOLS = ("fct_abi_version", "fct_last_error", "fct_plan_create",
                     "fct_batch_create", "fct_batch_set_tuning", "fct_batch_bind",
                     "fct_batch_launch", "fct_batch_graph_launch", "fct_batch_destroy",
                     "fct_backward", "fct_forward_upwind", "fct_impl_vert_diff",
-                    "fct_compute_w")
+                    "fct_compute_w", "fct_ice_fct_solve")
 
 
 class FctMeshDesc(ctypes.Structure):
@@ -79,6 +79,13 @@ class FctVdiffArgs(ctypes.Structure):
                 ("floor_value", c_double), ("Kv", c_void_p), ("hnode_new", c_void_p),
                 ("Z3d", c_void_p), ("T", POINTER(c_void_p)), ("bc_surf", POINTER(c_void_p)),
                 ("sw_3d", POINTER(c_void_p)), ("T_new", POINTER(c_void_p))]
+
+
+class FctIceArgs(ctypes.Structure):
+    _fields_ = [("ice_dt", c_double), ("ice_diff", c_double), ("ice_gamma_fct", c_double),
+                ("a_ice", c_void_p), ("m_ice", c_void_p), ("m_snow", c_void_p),
+                ("u_ice", c_void_p), ("v_ice", c_void_p), ("a_new", c_void_p),
+                ("m_new", c_void_p), ("ms_new", c_void_p)]
 
 
 class FctPhaseBufs(ctypes.Structure):
@@ -141,6 +148,8 @@ def load(path: Path | None = None, *, require_torch: bool = True):
     lib.fct_batch_destroy.argtypes = [c_void_p]
     lib.fct_forward_upwind.restype = c_int32
     lib.fct_forward_upwind.argtypes = [c_void_p, POINTER(FctStepArgs), c_void_p]
+    lib.fct_ice_fct_solve.restype = c_int32
+    lib.fct_ice_fct_solve.argtypes = [c_void_p, POINTER(FctIceArgs), c_void_p]
     lib.fct_compute_w.restype = c_int32
     lib.fct_compute_w.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
     lib.fct_impl_vert_diff.restype = c_int32

@@ -17,7 +17,8 @@ HERE = Path(__file__).resolve().parent
 PKG = HERE.parent
 ROOT = PKG.parent
 LIB = PKG / "libfesom_fct.so"
-SOURCES = ["fct_api.hip", "fct_phased.hip", "fct_fused.hip", "fct_adjoint.hip", "fct_vdiff.hip", "fct_ale.hip"]
+SOURCES = ["fct_api.hip", "fct_phased.hip", "fct_fused.hip", "fct_adjoint.hip", "fct_vdiff.hip", "fct_ale.hip",
+           "fct_ice.hip"]
 ARCH = os.environ.get("FESOM_FCT_ARCH", "gfx950")
 
 

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <algorithm>
@@ -67,6 +68,9 @@ struct fct_plan {
     // implicit vertical diffusion scratch (allocated on the first call)
     double* vd_scratch = nullptr;
     int vd_grid = 0;
+    // sea-ice FCT workspace and element factors (allocated on the first call)
+    double* ice_work = nullptr;
+    double* ice_sq = nullptr;
 };
 
 namespace {
@@ -622,6 +626,42 @@ int fct_forward_upwind(fct_plan* p, const fct_step_args* a, void* stream) {
     if (p->tab.dm == 0)
         return set_error(FCT_E_UNSUPPORTED, "this mesh cannot take the node-table kernels");
     return launch_upwind(p->m, p->tab, p->ekr, a, p->tune, p->vflux, (hipStream_t)stream);
+}
+
+int fct_ice_fct_solve(fct_plan* p, const fct_ice_args* a, void* stream) {
+    if (!p || !a) return set_error(FCT_E_INVALID, "NULL plan or args");
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != p->device)
+        return set_error(FCT_E_INVALID, "current device %d is not the plan's device %d", cur,
+                         p->device);
+    if (!a->a_ice || !a->m_ice || !a->m_snow || !a->u_ice || !a->v_ice || !a->a_new ||
+        !a->m_new || !a->ms_new)
+        return set_error(FCT_E_INVALID, "NULL ice field");
+    const double* ins[5] = {a->a_ice, a->m_ice, a->m_snow, a->u_ice, a->v_ice};
+    const double* outs[3] = {a->a_new, a->m_new, a->ms_new};
+    for (const double* o : outs)
+        for (const double* i : ins)
+            if (o == i) return set_error(FCT_E_INVALID, "an ice output aliases an input");
+    if (!p->ice_work) {
+        const int E = p->m.E;
+        std::vector<double> h(E);
+        HIPCHK(hipMemcpy(h.data(), p->m.earea, E * sizeof(double), hipMemcpyDeviceToHost));
+        for (int e = 0; e < E; ++e) h[e] = std::sqrt(h[e] / 2.0e8);
+        int rc;
+        const double* sq = nullptr;
+        if ((rc = dev_upload(p, &sq, h.data(), (size_t)E))) return rc;
+        p->ice_sq = const_cast<double*>(sq);
+        if ((rc = dev_alloc(p, &p->ice_work, ice_workspace_doubles(p->m.N)))) return rc;
+    }
+    IceCall c;
+    c.dt = a->ice_dt;
+    c.diff = a->ice_diff;
+    c.gamma = a->ice_gamma_fct;
+    c.u = a->u_ice;
+    c.v = a->v_ice;
+    c.X[0] = a->a_ice; c.X[1] = a->m_ice; c.X[2] = a->m_snow;
+    c.out[0] = a->a_new; c.out[1] = a->m_new; c.out[2] = a->ms_new;
+    return launch_ice_fct(p->m, c, p->ice_work, p->ice_sq, (hipStream_t)stream);
 }
 
 int fct_compute_w(fct_plan* p, const double* uv, const double* helem, double* w,

@@ -149,6 +149,19 @@ inline int64_t adfh_waves(int D, int K, const FusedTuning& t) {
     return (int64_t)((D + t.adfh_epw - 1) / t.adfh_epw) * K;
 }
 
+// Sea-ice FCT of the three ice tracers (fct_ice.hip).
+struct IceCall {
+    double dt = 0.0, diff = 0.0, gamma = 0.0;
+    const double* u = nullptr;
+    const double* v = nullptr;
+    const double* X[3] = {nullptr, nullptr, nullptr};   // a_ice, m_ice, m_snow
+    double* out[3] = {nullptr, nullptr, nullptr};
+};
+size_t ice_workspace_doubles(int N);
+// sq[e] = sqrt(elem_area[e] / 2e8), computed on the host (correctly rounded)
+int launch_ice_fct(const DevMesh& m, const IceCall& c, double* work, const double* sq,
+                   hipStream_t s);
+
 // Vertical velocity from continuity (fct_ale.hip).
 int launch_compute_w(const DevMesh& m, const double* uv, const double* helem, double* w,
                      hipStream_t s);

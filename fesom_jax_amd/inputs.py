@@ -275,3 +275,22 @@ def make_case_device(mesh: Mesh, device, *, seed=0, tracers=("T", "S"), dt=2700.
     return dict(uv=uv.contiguous(), w_e=w.contiguous(), helem=helem.contiguous(),
                 hnode=hnode.contiguous(), hnode_new=hnode.clone(), T=Ts,
                 T_old=[t.clone() for t in Ts], dt=dt, cfl=(ch, cv), vel_scale=fac)
+
+
+def make_ice_case(mesh: Mesh, *, seed=0, speed=0.3, lmax=6):
+    """Seeded sea-ice state on the surface nodes for the ice FCT
+    (fesom_jax/ice_adv.py:177-194): concentration ``a_ice`` in [0, 1] with sharp ice
+    edges, ice volume ``m_ice`` (m) and snow volume ``m_snow`` (m) per unit area, and a
+    smooth nodal ice velocity ``(u_ice, v_ice)`` scaled to ``speed`` m/s at its maximum.
+    Host numpy arrays."""
+    rng = np.random.default_rng(seed)
+    xyz = node_xyz(mesh)
+    edge = random_sph_field(xyz, lmax, rng)
+    a = np.clip(0.5 + 2.0 * edge / np.abs(edge).max(), 0.0, 1.0)
+    thick = 1.0 + 0.5 * random_sph_field(xyz, 4, rng) / 3.0
+    m_ice = a * np.clip(thick, 0.2, None)
+    m_snow = 0.2 * m_ice * (1.0 + 0.2 * np.tanh(random_sph_field(xyz, 3, rng)))
+    u = random_sph_field(xyz, lmax, rng)
+    v = random_sph_field(xyz, lmax, rng)
+    scale = speed / max(np.abs(u).max(), np.abs(v).max())
+    return dict(a_ice=a, m_ice=m_ice, m_snow=m_snow, u_ice=u * scale, v_ice=v * scale)

@@ -175,6 +175,29 @@ int fct_forward_phase(fct_plan* plan, const fct_step_args* args, int32_t tracer,
  * NULL. Needs a mesh that takes the fused path (FCT_PATH_BIT_FUSED). */
 int fct_forward_upwind(fct_plan* plan, const fct_step_args* args, void* stream);
 
+/* ---- sea-ice FCT ---------------------------------------------------------------------
+ * (a_new, m_new, ms_new) = fct_solve(cfg, mesh, a_ice, m_ice, m_snow, u_ice, v_ice)
+ * (fesom_jax/ice_adv.py:177-194, exch=None): Taylor-Galerkin right-hand side,
+ * consistent-mass low- and high-order solves and the element Zalesak limiter of the
+ * three ice tracers on the surface nodes. All fields [nod2D], device; outputs must not
+ * alias inputs. The first call allocates 18 * nod2D doubles of workspace and one factor
+ * per element; later calls allocate nothing. */
+typedef struct fct_ice_args {
+    double ice_dt;                    /* IceConfig.ice_dt (fesom_jax/ice.py:62) */
+    double ice_diff;                  /* IceConfig.ice_diff (:59) */
+    double ice_gamma_fct;             /* IceConfig.ice_gamma_fct (:58) */
+    const double* a_ice;
+    const double* m_ice;
+    const double* m_snow;
+    const double* u_ice;
+    const double* v_ice;
+    double* a_new;
+    double* m_new;
+    double* ms_new;
+} fct_ice_args;
+
+int fct_ice_fct_solve(fct_plan* plan, const fct_ice_args* args, void* stream);
+
 /* ---- vertical velocity ---------------------------------------------------------------
  * w = compute_w(mesh, uv, helem) (fesom_jax/ale.py:309-365), the w_e the step takes:
  * per-level edge divergence into the nodes, cumulative sum from the bottom, divided by

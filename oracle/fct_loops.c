@@ -398,3 +398,148 @@ int loops_compute_w(const loop_mesh* m, const double* uv, const double* helem, d
     }
     return 0;
 }
+
+/* Sea-ice FCT of one tracer set with element loops and scatter-adds, the structure of
+ * the original fesom_ice_fct.c as fesom_jax/ice_adv.py:1-32, 111-194 describes it:
+ * Taylor-Galerkin element matrices, consistent-mass low/high-order solves, element
+ * antidiffusive fluxes, node bounds over the surrounding elements, node ratios, element
+ * limiting by the minimum over its vertices. Elements or nodes under an ice shelf
+ * (ulevels > 1) take no part. */
+typedef struct {
+    double dt, diff, gamma;
+} loop_ice_cfg;
+
+static void ice_mm(const loop_mesh* m, const double* X, double* out) {
+    memset(out, 0, (size_t)m->nod2D * sizeof(double));
+    for (int e = 0; e < m->elem2D; ++e) {
+        if (m->ulevels[e] > 1) continue;
+        const int* v = m->elem_nodes + 3 * (size_t)e;
+        const double s = (X[v[0]] + X[v[1]]) + X[v[2]];
+        const double w = m->elem_area[e] / 12.0;
+        for (int n = 0; n < 3; ++n) out[v[n]] += w * (X[v[n]] + s);
+    }
+}
+
+static void ice_one(const loop_mesh* m, const loop_ice_cfg* c, const double* ent,
+                    const double* X, double* out) {
+    const int N = m->nod2D, E = m->elem2D, nl = m->nl;
+    double* rhs = calloc(N, sizeof(double));
+    double* mm = malloc(N * sizeof(double));
+    double* xl = malloc(N * sizeof(double));
+    double* d = malloc(N * sizeof(double));
+    double* w = malloc(N * sizeof(double));
+    double* hi = malloc(N * sizeof(double));
+    double* lo = malloc(N * sizeof(double));
+    double* fp = calloc(N, sizeof(double));
+    double* fm = calloc(N, sizeof(double));
+    double* pp = malloc(N * sizeof(double));
+    double* pm = malloc(N * sizeof(double));
+    double* acc = calloc(N, sizeof(double));
+    double* flux = malloc(3 * (size_t)E * sizeof(double));
+    /* right-hand side */
+    for (int e = 0; e < E; ++e) {
+        if (m->ulevels[e] > 1) continue;
+        const int* v = m->elem_nodes + 3 * (size_t)e;
+        const double* A = ent + 9 * (size_t)e;
+        for (int n = 0; n < 3; ++n)
+            rhs[v[n]] += (A[3 * n] * X[v[0]] + A[3 * n + 1] * X[v[1]]) + A[3 * n + 2] * X[v[2]];
+    }
+    /* low order: (rhs + gamma mm X) / area + (1 - gamma) X */
+    ice_mm(m, X, mm);
+    for (int n = 0; n < N; ++n) {
+        const double ar = m->area[(size_t)n * nl];
+        const double inv = 1.0 / (ar > 0 ? ar : 1.0);
+        xl[n] = m->ulevels_nod2D[n] <= 1 ? (rhs[n] + c->gamma * mm[n]) * inv + (1.0 - c->gamma) * X[n]
+                                         : X[n];
+        d[n] = m->ulevels_nod2D[n] <= 1 ? rhs[n] * inv : 0.0;
+    }
+    /* high order: two residual corrections with the consistent mass */
+    for (int it = 0; it < 2; ++it) {
+        ice_mm(m, d, mm);
+        for (int n = 0; n < N; ++n) {
+            if (m->ulevels_nod2D[n] > 1) continue;
+            const double ar = m->area[(size_t)n * nl];
+            const double inv = 1.0 / (ar > 0 ? ar : 1.0);
+            d[n] = d[n] + (rhs[n] - mm[n]) * inv;
+        }
+    }
+    /* antidiffusive element fluxes and node bounds */
+    for (int n = 0; n < N; ++n) {
+        w[n] = c->gamma * X[n] + d[n];
+        hi[n] = -INFINITY;
+        lo[n] = INFINITY;
+    }
+    for (int e = 0; e < E; ++e) {
+        const int* v = m->elem_nodes + 3 * (size_t)e;
+        for (int q = 0; q < 3; ++q) flux[3 * (size_t)e + q] = 0.0;
+        if (m->ulevels[e] > 1) continue;
+        const double s = (w[v[0]] + w[v[1]]) + w[v[2]];
+        double eh = -INFINITY, el = INFINITY;
+        for (int q = 0; q < 3; ++q) {
+            const double ar = m->area[(size_t)v[q] * nl];
+            const double inv = 1.0 / (ar > 0 ? ar : 1.0);
+            const double f = -(s - 3.0 * w[v[q]]) * (m->elem_area[e] * inv) / 12.0;
+            flux[3 * (size_t)e + q] = f;
+            fp[v[q]] += f > 0.0 ? f : 0.0;
+            fm[v[q]] += f < 0.0 ? f : 0.0;
+            eh = dmax(eh, dmax(xl[v[q]], X[v[q]]));
+            el = dmin(el, dmin(xl[v[q]], X[v[q]]));
+        }
+        for (int q = 0; q < 3; ++q) {
+            hi[v[q]] = dmax(hi[v[q]], eh);
+            lo[v[q]] = dmin(lo[v[q]], el);
+        }
+    }
+    for (int n = 0; n < N; ++n) {
+        pp[n] = 0.0;
+        pm[n] = 0.0;
+        if (m->ulevels_nod2D[n] > 1) continue;
+        const double h = isfinite(hi[n]) ? hi[n] : 0.0, l = isfinite(lo[n]) ? lo[n] : 0.0;
+        if (fp[n] > 0.0) pp[n] = dmin((h - xl[n]) / (fp[n] > 1e-12 ? fp[n] : 1e-12), 1.0);
+        if (fm[n] < 0.0) pm[n] = dmin((l - xl[n]) / (fm[n] < -1e-12 ? fm[n] : -1e-12), 1.0);
+    }
+    /* limit each element's fluxes by the smallest vertex ratio and scatter them */
+    for (int e = 0; e < E; ++e) {
+        if (m->ulevels[e] > 1) continue;
+        const int* v = m->elem_nodes + 3 * (size_t)e;
+        const double* f = flux + 3 * (size_t)e;
+        double ae = INFINITY;
+        for (int q = 0; q < 3; ++q) ae = dmin(ae, f[q] >= 0.0 ? pp[v[q]] : pm[v[q]]);
+        for (int q = 0; q < 3; ++q) acc[v[q]] += f[q] * ae;
+    }
+    for (int n = 0; n < N; ++n) out[n] = xl[n] + acc[n];
+    free(rhs); free(mm); free(xl); free(d); free(w); free(hi); free(lo); free(fp); free(fm);
+    free(pp); free(pm); free(acc); free(flux);
+}
+
+int loops_ice_fct(const loop_mesh* m, const loop_ice_cfg* c, const double* a_ice,
+                  const double* m_ice, const double* m_snow, const double* u_ice,
+                  const double* v_ice, double* a_out, double* m_out, double* ms_out) {
+    const int E = m->elem2D;
+    double* ent = malloc(9 * (size_t)E * sizeof(double));
+    if (!ent) return -1;
+    /* Taylor-Galerkin element matrices vol dt (a - b - c), rows n, columns q */
+    for (int e = 0; e < E; ++e) {
+        const int* v = m->elem_nodes + 3 * (size_t)e;
+        const double* g = m->gradient_sca + 6 * (size_t)e;
+        const double vol = m->elem_area[e];
+        const double U[3] = {u_ice[v[0]], u_ice[v[1]], u_ice[v[2]]};
+        const double V[3] = {v_ice[v[0]], v_ice[v[1]], v_ice[v[2]]};
+        const double um = (U[0] + U[1]) + U[2], vm = (V[0] + V[1]) + V[2];
+        const double diff = c->diff * sqrt(vol / 2.0e8);
+        double udn[3];
+        for (int n = 0; n < 3; ++n) udn[n] = um * g[n] + vm * g[3 + n];
+        for (int n = 0; n < 3; ++n)
+            for (int q = 0; q < 3; ++q) {
+                const double a = (g[n] * (um + U[q]) + g[3 + n] * (vm + V[q])) / 12.0;
+                const double b = diff * (g[n] * g[q] + g[3 + n] * g[3 + q]);
+                const double cc = 0.5 * c->dt * (udn[n] * udn[q]) / 9.0;
+                ent[9 * (size_t)e + 3 * n + q] = vol * c->dt * (a - b - cc);
+            }
+    }
+    ice_one(m, c, ent, a_ice, a_out);
+    ice_one(m, c, ent, m_ice, m_out);
+    ice_one(m, c, ent, m_snow, ms_out);
+    free(ent);
+    return 0;
+}

@@ -24,6 +24,10 @@ class _LoopMesh(ctypes.Structure):
                 + [(f, POINTER(c_double)) for f in _F64_FIELDS])
 
 
+class _IceCfg(ctypes.Structure):
+    _fields_ = [("dt", c_double), ("diff", c_double), ("gamma", c_double)]
+
+
 _LIB = None
 
 
@@ -34,6 +38,9 @@ def _lib():
         _LIB.fct_loops_step.restype = c_int
         _LIB.fct_loops_step.argtypes = [POINTER(_LoopMesh)] + [POINTER(c_double)] * 7 + [
             c_double, POINTER(c_double)]
+        _LIB.loops_ice_fct.restype = c_int
+        _LIB.loops_ice_fct.argtypes = [POINTER(_LoopMesh), POINTER(_IceCfg)] + [
+            POINTER(c_double)] * 8
         _LIB.loops_compute_w.restype = c_int
         _LIB.loops_compute_w.argtypes = [POINTER(_LoopMesh)] + [POINTER(c_double)] * 3
     return _LIB
@@ -77,3 +84,18 @@ def compute_w(mesh, uv, helem):
     P = lambda a: a.ctypes.data_as(POINTER(c_double))
     _lib().loops_compute_w(ctypes.byref(m), P(uv), P(helem), P(out))
     return out
+
+
+def ice_fct_solve(cfg, mesh, a_ice, m_ice, m_snow, u_ice, v_ice):
+    """(a_new, m_new, ms_new) of one sea-ice FCT step with element loops
+    (fesom_jax/ice_adv.py:177-194)."""
+    m, keep = _mesh_struct(mesh)
+    c = _IceCfg(dt=float(cfg.ice_dt), diff=float(cfg.ice_diff), gamma=float(cfg.ice_gamma_fct))
+    ins = [_f64(x) for x in (a_ice, m_ice, m_snow, u_ice, v_ice)]
+    outs = [np.empty(int(mesh.nod2D)) for _ in range(3)]
+    P = lambda a: a.ctypes.data_as(POINTER(c_double))
+    rc = _lib().loops_ice_fct(ctypes.byref(m), ctypes.byref(c), *[P(a) for a in ins],
+                              *[P(o) for o in outs])
+    if rc != 0:
+        raise MemoryError("loops_ice_fct: allocation failed")
+    return tuple(outs)
