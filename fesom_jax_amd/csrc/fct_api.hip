@@ -400,6 +400,7 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
     if (dm && (L > 128 || !fused_mesh_ok(d, h_eoff, h_ecsr, h_noff, h_ncsr))) dm = 0;
     std::vector<int4> h_thdr;
     std::vector<int4> h_tedge, h_telem;
+    bool vertex_missing = false;   // an element vertex that is not an edge neighbour
     if (dm) {
         h_thdr.resize(N);
         h_tedge.assign((size_t)N * dm, make_int4(0, 0, 0, 0));
@@ -427,10 +428,23 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
             }
             for (int q = 0; q < ne; ++q) {
                 const int el = h_ncsr[h_noff[n] + q];
-                // layer range of element slot q, lo|hi bytes
+                // layer range of element slot q, lo|hi bytes; above them, 4 bits per
+                // vertex: the edge slot whose far end it is, or dm for the node itself
                 const unsigned lo = (unsigned)(d->ulevels[el] - 1);
                 const unsigned hi = (unsigned)std::max(d->nlevels[el] - 1, d->ulevels[el] - 1);
-                h_tedge[(size_t)n * dm + q].w = (int)(lo | (hi << 8));
+                unsigned codes = 0;
+                for (int i = 0; i < 3; ++i) {
+                    const int v = d->elem_nodes[3 * el + i];
+                    unsigned c = (unsigned)dm;
+                    if (v != n) {
+                        c = 0xf;
+                        for (int q2 = 0; q2 < de; ++q2)
+                            if (h_tedge[(size_t)n * dm + q2].y == v) { c = (unsigned)q2; break; }
+                        if (c == 0xf) vertex_missing = true;
+                    }
+                    codes |= c << (4 * i);
+                }
+                h_tedge[(size_t)n * dm + q].w = (int)(lo | (hi << 8) | (codes << 16));
                 h_telem[(size_t)n * dm + q] = make_int4(el, d->elem_nodes[3 * el],
                                                         d->elem_nodes[3 * el + 1],
                                                         d->elem_nodes[3 * el + 2]);
@@ -444,6 +458,7 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
             for (int q = ne; q < dm && ne > 0; ++q)
                 h_telem[(size_t)n * dm + q] = h_telem[(size_t)n * dm];
         }
+        if (vertex_missing) dm = 0;   // the fused kernels read element vertices by slot
     }
     // fused-path edge tables: packed triangle layer ranges and gradient-triangle vertices
     std::vector<int> h_etr;

@@ -245,6 +245,9 @@ __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
         es[q] = ldv(a.tab.edge, (size_t)n * DM + q);
         el[q] = ldv(a.tab.elem, (size_t)n * DM + q);
     }
+    // per wavefront: the neighbour columns at the lane's level, then the node's own
+    __shared__ double s_nb[4][DM + 1][64];
+    double* nb = &s_nb[threadIdx.x >> 6][0][lane_id()];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         if (64 * s > nr.y) break;
@@ -263,9 +266,19 @@ __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
             for (int q = 0; q < DM; ++q) {
                 vf[q] = G(a.vflux)[(size_t)(es[q].x >> 1) * L + k];
                 To[q] = T[(size_t)es[q].y * L + k];
-                tv[q][0] = T[(size_t)el[q].y * L + k];
-                tv[q][1] = T[(size_t)el[q].z * L + k];
-                tv[q][2] = T[(size_t)el[q].w * L + k];
+            }
+            // the element vertices are the node and edge neighbours: read them back
+            // from the lane's own LDS row by slot code (bits 16.. of the element-slot
+            // word: 4 bits per vertex, DM = the node) instead of loading them again
+#pragma unroll
+            for (int q = 0; q < DM; ++q) nb[q * 64] = To[q];
+            nb[DM * 64] = Tk;
+#pragma unroll
+            for (int q = 0; q < DM; ++q) {
+                const unsigned codes = (unsigned)es[q].w >> 16;
+                tv[q][0] = nb[(codes & 0xf) * 64];
+                tv[q][1] = nb[((codes >> 4) & 0xf) * 64];
+                tv[q][2] = nb[((codes >> 8) & 0xf) * 64];
             }
         }
         if (!wet) continue;
