@@ -60,6 +60,7 @@ struct fct_plan {
     int kw = 0;
     FusedTuning tune;
     NodeTab tab;
+    // fpf holds the limiter factors as (plus, minus) pairs; fmf stays null
     double *fLO = nullptr, *fgnode = nullptr, *fadfh = nullptr, *fpf = nullptr, *fmf = nullptr;
     // adjoint workspace (allocated on the first fct_backward)
     bool adj_ready = false;
@@ -532,7 +533,7 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
     }
     if ((rc = dev_alloc(p, &p->fLO, NL * p->kw)) || (rc = dev_alloc(p, &p->fgnode, 2 * NL * p->kw)) ||
         (rc = dev_alloc(p, &p->fadfh, DL * p->kw)) ||
-        (rc = dev_alloc(p, &p->fpf, NL * p->kw)) || (rc = dev_alloc(p, &p->fmf, NL * p->kw))) {
+        (rc = dev_alloc(p, &p->fpf, 2 * NL * p->kw))) {
         free_plan(p);
         return rc;
     }

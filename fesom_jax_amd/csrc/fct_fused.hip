@@ -585,8 +585,10 @@ __device__ __forceinline__ void pass2_body(const FusedArgs& a, int n, int j, int
         const double flux_pos = fplus * a.dt / safe_a / safe_hn + FCT_FLUX_EPS;
         const double flux_neg = fminus * a.dt / safe_a / safe_hn - FCT_FLUX_EPS;
         const bool valid = av > 0.0 && hn > 0.0;
-        G(a.pf)[jo + i] = valid ? fmin(1.0, ttf_max / flux_pos) : 1.0;
-        G(a.mf)[jo + i] = valid ? fmin(1.0, ttf_min / flux_neg) : 1.0;
+        // (plus, minus) factor pairs, interleaved: pass 3 reads both with one load
+        stv(reinterpret_cast<double2*>(a.pf), jo + i,
+            make_double2(valid ? fmin(1.0, ttf_max / flux_pos) : 1.0,
+                         valid ? fmin(1.0, ttf_min / flux_neg) : 1.0));
     }
 }
 
@@ -608,8 +610,7 @@ __device__ __forceinline__ void pass3_body(const FusedArgs& a, int n, int j, int
     const auto T = G(a.T[j]);
     const auto To = G(a.Told[j]);
     const size_t jo = (size_t)j * m.N * L;
-    const auto pf = G(a.pf + jo);
-    const auto mf = G(a.mf + jo);
+    const double2* pm = reinterpret_cast<const double2*>(a.pf) + jo;   // (plus, minus) pairs
     const auto adfh = G(a.adfh + (size_t)j * m.D * L);
     const size_t b0 = (size_t)n * L;
     const int kneed = nr.y + 1 < L ? nr.y + 1 : L;
@@ -632,14 +633,15 @@ __device__ __forceinline__ void pass3_body(const FusedArgs& a, int n, int j, int
 #pragma unroll
             for (int q = 0; q < DM; ++q) {
                 fh[q] = adfh[(size_t)(es[q].x >> 1) * L + k];
-                const size_t io = (size_t)es[q].y * L + k;
-                po[q] = pf[io];
-                mo[q] = mf[io];
+                const double2 f = ldv(pm, (size_t)es[q].y * L + k);
+                po[q] = f.x;
+                mo[q] = f.y;
             }
         }
         if (wet) {
-            Pc[s] = pf[b0 + k];
-            Mc[s] = mf[b0 + k];
+            const double2 f = ldv(pm, b0 + k);
+            Pc[s] = f.x;
+            Mc[s] = f.y;
         }
 #pragma unroll
         for (int q = 0; q < DM; ++q) {
