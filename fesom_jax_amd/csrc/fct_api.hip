@@ -703,9 +703,15 @@ int fct_impl_vert_diff(fct_plan* p, const fct_vdiff_args* a, void* stream) {
         return set_error(FCT_E_INVALID, "NULL Kv, hnode_new, T or T_new");
     if (a->floor_tracer < -1 || a->floor_tracer >= K)
         return set_error(FCT_E_INVALID, "floor_tracer %d outside -1..%d", a->floor_tracer, K - 1);
-    for (int j = 0; j < K; ++j)
+    for (int j = 0; j < K; ++j) {
         if (!a->T[j] || !a->T_new[j])
             return set_error(FCT_E_INVALID, "tracer %d has a NULL field", j);
+        for (int i = 0; i < K; ++i)
+            if (i != j && ((const double*)a->T_new[j] == a->T[i] || a->T_new[j] == a->T_new[i]))
+                return set_error(FCT_E_INVALID,
+                                 "T_new[%d] aliases tracer %d (only T_new[j] == T[j] is allowed)",
+                                 j, i);
+    }
     if (!p->vd_scratch) {
         int cus = 0;
         HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));

@@ -61,7 +61,14 @@ def diffuse_tracers(mesh, Ts, Kv, hnode_new, *, dt: float = DT_DEFAULT, bc_surf=
     if out is None:
         out = [torch.empty(ns, dtype=torch.float64, device=device) for _ in range(K)]
     else:
-        out = [_to_device(o, f"out[{j}]", device, ns) for j, o in enumerate(out)]
+        out = list(out)
+        if len(out) != K:
+            raise ValueError("out must have one tensor per tracer")
+        for j, o in enumerate(out):
+            if not (isinstance(o, torch.Tensor) and o.device == device and
+                    o.dtype == torch.float64 and tuple(o.shape) == ns and o.is_contiguous()):
+                raise ValueError(f"out[{j}] must be a contiguous float64 tensor of shape {ns} "
+                                 f"on {device}")
     if floor_tracer is not None and not 0 <= int(floor_tracer) < K:
         raise ValueError(f"floor_tracer {floor_tracer} outside 0..{K - 1}")
     plan = get_plan(mesh, device)

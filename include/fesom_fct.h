@@ -215,7 +215,8 @@ int fct_compute_w(fct_plan* plan, const double* uv, const double* helem, double*
  * same matrix for every tracer of the call. Optionally clamps one tracer from below on
  * wet layers, the salinity floor of fesom_jax/step.py:456. The first call allocates a
  * scratch bounded by the GPU's resident workgroups (3 * nl * 64 doubles each), later
- * calls allocate nothing. T_new[j] may be T[j] (in place). */
+ * calls allocate nothing. T_new[j] may be T[j] (in place); no other aliasing between
+ * the tracers' fields is allowed (FCT_E_INVALID). */
 #define FCT_VDIFF_MAX_TRACERS 8
 typedef struct fct_vdiff_args {
     int32_t ntracers;                 /* 1 .. FCT_VDIFF_MAX_TRACERS */

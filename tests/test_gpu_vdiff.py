@@ -148,3 +148,11 @@ def test_rejects_bad_arguments(pi_mesh, pi_case, torch_dev):
         diffuse_tracers(m, [t(c["T"][0])], Kv, t(c["hnode_new"]), floor_tracer=1)
     with pytest.raises(ValueError):
         diffuse_tracers(m, [t(c["T"][0])], Kv, t(c["hnode_new"]), bc_surf=[None, None])
+    # an output that is another tracer's input is refused by the library
+    T0, T1 = t(c["T"][0]), t(c["T"][1])
+    with pytest.raises(ValueError):
+        diffuse_tracers(m, [T0, T1], Kv, t(c["hnode_new"]), out=[T1, T0])
+    # a non-contiguous output is refused rather than written through a copy
+    with pytest.raises(ValueError):
+        diffuse_tracers(m, [T0], Kv, t(c["hnode_new"]),
+                        out=[t(np.asfortranarray(c["T"][0])).t().contiguous().t()])
