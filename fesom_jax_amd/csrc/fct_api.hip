@@ -280,6 +280,11 @@ int apply_tuning(FusedTuning& t, int knob, int value) {
             if (value < 1 || value > 64) return set_error(FCT_E_INVALID, "xcd %d out of range", value);
             t.xcd = value;
             return FCT_OK;
+        case FCT_TUNE_XCD_CHUNK:
+            if (value < 0 || value > (1 << 20))
+                return set_error(FCT_E_INVALID, "xcd chunk %d out of range", value);
+            t.xchunk = value;
+            return FCT_OK;
         case FCT_TUNE_WAVES_PER_GROUP:
             if (value != 1 && value != 2 && value != 4)
                 return set_error(FCT_E_INVALID, "waves per workgroup %d not in {1, 2, 4}", value);
@@ -403,12 +408,35 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
     std::vector<int4> h_tedge, h_telem;
     bool vertex_missing = false;   // an element vertex that is not an edge neighbour
     if (dm) {
+        // levels at which the adf_h pass reads a node's averaged gradient gnode: the wet
+        // levels of an incident edge outside its up/dn gradient range (all of them where
+        // that range is empty), as one range [g_lo, g_hi) per node
+        std::vector<int> g_lo(N, L), g_hi(N, 0);
+        for (int e = 0; e < D; ++e) {
+            const int2 kr = h_ekr[e], ur = h_rng[e];
+            if (kr.x >= kr.y) continue;
+            const int tl = std::max(kr.x, ur.x), th = std::min(kr.y, ur.y);
+            int lo = kr.x, hi = kr.y;
+            if (tl < th) {
+                const bool above = tl > kr.x, below = th < kr.y;
+                if (!above && !below) continue;
+                lo = above ? kr.x : th;
+                hi = below ? kr.y : tl;
+            }
+            for (int side = 0; side < 2; ++side) {
+                const int v = side ? h_edges[e].y : h_edges[e].x;
+                g_lo[v] = std::min(g_lo[v], lo);
+                g_hi[v] = std::max(g_hi[v], hi);
+            }
+        }
         h_thdr.resize(N);
         h_tedge.assign((size_t)N * dm, make_int4(0, 0, 0, 0));
         h_telem.assign((size_t)N * dm, make_int4(0, 0, 0, 0));
         for (int n = 0; n < N; ++n) {
             const int de = h_eoff[n + 1] - h_eoff[n], ne = h_noff[n + 1] - h_noff[n];
-            h_thdr[n] = make_int4(d->ulevels_nod2D[n] - 1, d->nlevels_nod2D[n] - 1, de, ne);
+            const int gl = g_lo[n] < g_hi[n] ? g_lo[n] : 0, gh = g_lo[n] < g_hi[n] ? g_hi[n] : 0;
+            h_thdr[n] = make_int4(d->ulevels_nod2D[n] - 1, d->nlevels_nod2D[n] - 1,
+                                  de | (gl << 8) | (gh << 16), ne);
             for (int q = 0; q < de; ++q) {
                 const int ent = h_ecsr[h_eoff[n] + q];
                 const int e = ent >> 1;

@@ -49,7 +49,9 @@ struct DevMesh {
 // Per-node stencil table of the fused path (dm slots per node).
 struct NodeTab {
     int dm = 0;               // slots per node (6 or 8); 0 = mesh not supported
-    const int4* hdr;          // [N] (nzmin, nzmax, incident edges, incident elements)
+    const int4* hdr;          // [N] (nzmin, nzmax, incident edges | glo << 8 | ghi << 16,
+                              //  incident elements); [glo, ghi) = levels at which the
+                              //  adf_h pass reads the node's gnode (empty = none)
     const int4* edge;         // [N*dm] (2*edge + side, other node, layer ranges of the
                               //  edge's triangles as bytes lo1|hi1|lo2|hi2, layer range of
                               //  element slot q as bytes lo|hi), edges in reference order
@@ -106,11 +108,14 @@ int launch_backward(const DevMesh& m, const AdjointWork& w, const fct_step_args*
 // Launch-shape knobs of the fused path (fct_plan_set_tuning / fct_batch_set_tuning);
 // results do not depend on them.
 struct FusedTuning {
-    int xcd = 1;        // remap blocks so each of `xcd` XCDs sweeps one contiguous tile range
-                        // (1 = hardware order, the fastest measured on a single large mesh)
+    int xcd = 1;        // remap blocks so each of `xcd` XCDs sweeps its own tile ranges
+                        // (1 = hardware order)
+    int xchunk = 0;     // tiles per contiguous range of one XCD (0 = one range per XCD)
     int wpb = 4;        // wavefronts per workgroup (1, 2 or 4)
     int vflux_epw = 4;  // edges per wavefront in kf_vflux (2 or 4)
     int adfh_epw = 2;   // edges per wavefront in kf_adfh (1 or 2)
+    // the remap as the kernels take it: XCD count in the low 8 bits, chunk above
+    int remap() const { return xcd | (xchunk << 8); }
 };
 
 struct FusedArgs;

@@ -94,15 +94,24 @@ __device__ __forceinline__ bool in_bytes(int packed, int shift, int k) {
     return k >= lo && k < hi;
 }
 
-// Block -> tile remap so that each XCD works through one contiguous range of tiles
-// (workgroups are dealt to the XCDs round-robin; node and edge order is spatially
-// local, so a contiguous range keeps neighbours in the same L2).
-__device__ __forceinline__ unsigned block_tile(int xcd) {
+// Block -> tile remap. Workgroups are dealt to the X XCDs round-robin (block b to XCD
+// b % X), and node and edge order is spatially local, so giving each XCD runs of C
+// consecutive tiles keeps most neighbours of a tile in the same L2. remap = X | C << 8;
+// C = 0 gives each XCD one contiguous range of the whole grid. With C > 0 the grid's
+// tail beyond the last whole round of X * C blocks keeps hardware order, so the map
+// is a bijection on [0, gridDim.x) for any grid size.
+__device__ __forceinline__ unsigned block_tile(int remap) {
     const unsigned b = blockIdx.x, G = gridDim.x;
-    if (xcd <= 1) return b;
-    const unsigned x = b % (unsigned)xcd, i = b / (unsigned)xcd;
-    const unsigned q = G / (unsigned)xcd, r = G % (unsigned)xcd;
-    return x * q + (x < r ? x : r) + i;
+    const unsigned X = (unsigned)remap & 0xffu, C = (unsigned)remap >> 8;
+    if (X <= 1) return b;
+    const unsigned x = b % X, i = b / X;
+    if (C == 0) {
+        const unsigned q = G / X, r = G % X;
+        return x * q + (x < r ? x : r) + i;
+    }
+    const unsigned span = X * C;
+    if (b >= G - G % span) return b;
+    return ((i / C) * X + x) * C + i % C;
 }
 
 struct FusedArgs {
@@ -118,7 +127,7 @@ struct FusedArgs {
     const double* Z3d;     // optional
     const double* zbar3;   // optional
     double dt;
-    int xcd;               // XCD count for the block -> task remap (1 = identity)
+    int remap;             // block -> task remap, XCD count | chunk << 8 (1 = identity)
     int K;                 // tracers in this launch (<= FCT_KCHUNK)
     const double* T[FCT_KCHUNK];
     const double* Told[FCT_KCHUNK];
@@ -132,8 +141,8 @@ struct FusedArgs {
 
 // Wavefront -> task index. Single-instance launches number the wavefronts of the
 // (remapped) grid; batched launches look up (instance, first wavefront) per workgroup.
-__device__ __forceinline__ bool single_wave(int64_t total, int xcd, int& w) {
-    const int64_t ww = ((int64_t)block_tile(xcd) * blockDim.x + threadIdx.x) >> 6;
+__device__ __forceinline__ bool single_wave(int64_t total, int remap, int& w) {
+    const int64_t ww = ((int64_t)block_tile(remap) * blockDim.x + threadIdx.x) >> 6;
     if (ww >= total) return false;
     w = __builtin_amdgcn_readfirstlane((int)ww);
     return true;
@@ -168,8 +177,8 @@ __device__ __forceinline__ void stv(double2* p, size_t i, double2 v) {
     ((gptr<fct_dv2>)p)[i] = __builtin_bit_cast(fct_dv2, v);
 }
 
-__device__ __forceinline__ int batch_wave(const int2* __restrict__ blk, int xcd, int& inst) {
-    const int2 bi = blk[block_tile(xcd)];
+__device__ __forceinline__ int batch_wave(const int2* __restrict__ blk, int remap, int& inst) {
+    const int2 bi = blk[block_tile(remap)];
     inst = __builtin_amdgcn_readfirstlane(bi.x);
     return __builtin_amdgcn_readfirstlane(bi.y + (int)(threadIdx.x >> 6));
 }
@@ -233,7 +242,10 @@ __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
     const DevMesh& m = a.m;
     const int L = m.L;
     const int4 hd = ldv(a.tab.hdr, n);
-    const int2 nr = make_int2(hd.x, hd.y), cnt = make_int2(hd.z, hd.w);   // levels, counts
+    const int2 nr = make_int2(hd.x, hd.y), cnt = make_int2(hd.z & 0xff, hd.w);   // levels, counts
+    // levels at which the adf_h pass reads this node's gnode, as a range [glo, ghi)
+    const int ghz = __builtin_amdgcn_readfirstlane(hd.z);
+    const int glo = (ghz >> 8) & 0xff, ghi = (ghz >> 16) & 0xff;
     const auto T = G(a.T[j]);
     const size_t b0 = (size_t)n * L;
     const size_t jo = (size_t)j * m.N * L;
@@ -254,6 +266,7 @@ __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
         const int k = 64 * s + lane_id();
         const bool rd = k < kneed;
         const bool wet = in_layer(nr, k);
+        const bool gslot = glo < ghi && glo < 64 * s + 64 && ghi > 64 * s;
         // own column, k-1 and k+1 of T for the vertical fluxes
         const double Tk = rd ? T[b0 + k] : 0.0;
         const double Tm = (rd && k > 0) ? T[b0 + k - 1] : Tk;
@@ -270,15 +283,17 @@ __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
             // the element vertices are the node and edge neighbours: read them back
             // from the lane's own LDS row by slot code (bits 16.. of the element-slot
             // word: 4 bits per vertex, DM = the node) instead of loading them again
+            if (gslot) {
 #pragma unroll
-            for (int q = 0; q < DM; ++q) nb[q * 64] = To[q];
-            nb[DM * 64] = Tk;
+                for (int q = 0; q < DM; ++q) nb[q * 64] = To[q];
+                nb[DM * 64] = Tk;
 #pragma unroll
-            for (int q = 0; q < DM; ++q) {
-                const unsigned codes = (unsigned)es[q].w >> 16;
-                tv[q][0] = nb[(codes & 0xf) * 64];
-                tv[q][1] = nb[((codes >> 4) & 0xf) * 64];
-                tv[q][2] = nb[((codes >> 8) & 0xf) * 64];
+                for (int q = 0; q < DM; ++q) {
+                    const unsigned codes = (unsigned)es[q].w >> 16;
+                    tv[q][0] = nb[(codes & 0xf) * 64];
+                    tv[q][1] = nb[((codes >> 4) & 0xf) * 64];
+                    tv[q][2] = nb[((codes >> 8) & 0xf) * 64];
+                }
             }
         }
         if (!wet) continue;
@@ -311,7 +326,8 @@ __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
         const double numer = Tk * G(a.hnode)[i] + (acc + (fv0 - fv1)) * a.dt / safe_a;
         const double lo = numer / safe_hn;
         G(a.LO)[jo + i] = (hn > 0.0 && av > 0.0) ? lo : 0.0;
-        // gnode: elements in increasing id
+        // gnode: elements in increasing id, only on the levels the adf_h pass reads
+        if (!gslot) continue;
         double nx = 0.0, ny = 0.0, den = 0.0;
 #pragma unroll
         for (int q = 0; q < DM; ++q) {
@@ -326,7 +342,7 @@ __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
             den = den + (use ? ea : 0.0);
         }
         const double d = den > 0.0 ? den : 1.0;
-        stv(a.gnode, jo + i, make_double2(nx / d, ny / d));
+        if (k >= glo && k < ghi) stv(a.gnode, jo + i, make_double2(nx / d, ny / d));
     }
 }
 
@@ -365,6 +381,14 @@ __device__ __forceinline__ void adfh_group(const FusedArgs& a, int g, int j) {
         const int4 va = ldv(m.udv, 2 * (size_t)e[q]), vb = ldv(m.udv, 2 * (size_t)e[q] + 1);
         u[q][0] = va.x; u[q][1] = va.y; u[q][2] = va.z;
         d[q][0] = va.w; d[q][1] = vb.x; d[q][2] = vb.y;
+    }
+    // level ranges as wave-uniform values: which of the gradient sources a slot needs
+#pragma unroll
+    for (int q = 0; q < EPW; ++q) {
+        kr[q].x = __builtin_amdgcn_readfirstlane(kr[q].x);
+        kr[q].y = __builtin_amdgcn_readfirstlane(kr[q].y);
+        ur[q].x = __builtin_amdgcn_readfirstlane(ur[q].x);
+        ur[q].y = __builtin_amdgcn_readfirstlane(ur[q].y);
         kmax = kr[q].y > kmax ? kr[q].y : kmax;
     }
 #pragma unroll
@@ -374,19 +398,31 @@ __device__ __forceinline__ void adfh_group(const FusedArgs& a, int g, int j) {
         double vf[EPW] = {}, T1[EPW] = {}, T2[EPW] = {}, O1[EPW] = {}, O2[EPW] = {};
         double tu[EPW][3] = {}, td[EPW][3] = {};
         double2 ga[EPW], gb[EPW];
+        // per edge: does any wet level of this slot use the up/dn triangle gradients
+        // (nt), does any use the node gradients gnode (ng)
+        bool nt[EPW], ng[EPW];
 #pragma unroll
-        for (int q = 0; q < EPW; ++q) { ga[q] = make_double2(0.0, 0.0); gb[q] = ga[q]; }
+        for (int q = 0; q < EPW; ++q) {
+            ga[q] = make_double2(0.0, 0.0); gb[q] = ga[q];
+            const int lo = kr[q].x > 64 * s ? kr[q].x : 64 * s;
+            const int hi = kr[q].y < 64 * s + 64 ? kr[q].y : 64 * s + 64;
+            const int tl = lo > ur[q].x ? lo : ur[q].x, th = hi < ur[q].y ? hi : ur[q].y;
+            nt[q] = tl < th;
+            ng[q] = lo < hi && !(tl == lo && th == hi);
+        }
         if (k < kmax) {
 #pragma unroll
             for (int q = 0; q < EPW; ++q) {
                 const size_t i1 = (size_t)nd[q].x * L + k, i2 = (size_t)nd[q].y * L + k;
                 vf[q] = G(a.vflux)[(size_t)e[q] * L + k];
                 T1[q] = T[i1]; T2[q] = T[i2]; O1[q] = To[i1]; O2[q] = To[i2];
-                ga[q] = ldv(gn, i1); gb[q] = ldv(gn, i2);
+                if (ng[q]) { ga[q] = ldv(gn, i1); gb[q] = ldv(gn, i2); }
+                if (nt[q]) {
 #pragma unroll
-                for (int v = 0; v < 3; ++v) {
-                    tu[q][v] = T[(size_t)u[q][v] * L + k];
-                    td[q][v] = T[(size_t)d[q][v] * L + k];
+                    for (int v = 0; v < 3; ++v) {
+                        tu[q][v] = T[(size_t)u[q][v] * L + k];
+                        td[q][v] = T[(size_t)d[q][v] * L + k];
+                    }
                 }
             }
         }
@@ -395,10 +431,11 @@ __device__ __forceinline__ void adfh_group(const FusedArgs& a, int g, int j) {
             if (k < kr[q].x || k >= kr[q].y) continue;
             const bool tri = k >= ur[q].x && k < ur[q].y;
             const double A1 = ab2(O1[q], T1[q]), A2 = ab2(O2[q], T2[q]);
-            const auto gsu = G(m.gs) + 6 * (size_t)ud[q].x;
-            const auto gsd = G(m.gs) + 6 * (size_t)ud[q].y;
-            const double2 gu = grad_of(gsu, tu[q][0], tu[q][1], tu[q][2]);
-            const double2 gd = grad_of(gsd, td[q][0], td[q][1], td[q][2]);
+            double2 gu = make_double2(0.0, 0.0), gd = gu;
+            if (nt[q]) {
+                gu = grad_of(G(m.gs) + 6 * (size_t)ud[q].x, tu[q][0], tu[q][1], tu[q][2]);
+                gd = grad_of(G(m.gs) + 6 * (size_t)ud[q].y, td[q][0], td[q][1], td[q][2]);
+            }
             const double g0 = tri ? gu.x : ga[q].x, g1 = tri ? gd.x : gb[q].x;
             const double g2 = tri ? gu.y : ga[q].y, g3 = tri ? gd.y : gb[q].y;
             const double v = vf[q];
@@ -595,7 +632,7 @@ __device__ __forceinline__ void pass2_body(const FusedArgs& a, int n, int j, int
 template <int NS, int DM>
 __device__ __forceinline__ void pass2_task(const FusedArgs& a, int n, int j) {
     const int4 hd = ldv(a.tab.hdr, n);
-    const int2 nr = make_int2(hd.x, hd.y), cnt = make_int2(hd.z, hd.w);
+    const int2 nr = make_int2(hd.x, hd.y), cnt = make_int2(hd.z & 0xff, hd.w);
     if (NS == 1 || nr.y < 64) pass2_body<1, DM>(a, n, j, nr, cnt);
     else pass2_body<NS, DM>(a, n, j, nr, cnt);
 }
@@ -705,7 +742,7 @@ __device__ __forceinline__ void pass3_body(const FusedArgs& a, int n, int j, int
 template <int NS, int DM>
 __device__ __forceinline__ void pass3_task(const FusedArgs& a, int n, int j) {
     const int4 hd = ldv(a.tab.hdr, n);
-    const int2 nr = make_int2(hd.x, hd.y), cnt = make_int2(hd.z, hd.w);
+    const int2 nr = make_int2(hd.x, hd.y), cnt = make_int2(hd.z & 0xff, hd.w);
     if (NS == 1 || nr.y < 64) pass3_body<1, DM>(a, n, j, nr, cnt);
     else pass3_body<NS, DM>(a, n, j, nr, cnt);
 }
@@ -720,7 +757,7 @@ __device__ __forceinline__ void upwind_task(const FusedArgs& a, int n, int j) {
     const DevMesh& m = a.m;
     const int L = m.L;
     const int4 hd = ldv(a.tab.hdr, n);
-    const int2 nr = make_int2(hd.x, hd.y), cnt = make_int2(hd.z, hd.w);
+    const int2 nr = make_int2(hd.x, hd.y), cnt = make_int2(hd.z & 0xff, hd.w);
     const auto T = G(a.T[j]);
     const auto To = G(a.Told[j]);
     const auto Tn = G(a.Tnew[j]);
@@ -795,13 +832,13 @@ __device__ __forceinline__ void upwind_task(const FusedArgs& a, int n, int j) {
 template <int NS, int EPW>
 __global__ void FCT_OCC(8) kf_vflux(FusedArgs a) {
     int w;
-    if (single_wave((a.m.D + EPW - 1) / EPW, a.xcd, w)) vflux_group<NS, EPW>(a, w);
+    if (single_wave((a.m.D + EPW - 1) / EPW, a.remap, w)) vflux_group<NS, EPW>(a, w);
 }
 template <int NS, int EPW>
 __global__ void FCT_OCC(8) kb_vflux(const FusedArgs* __restrict__ d,
-                                                const int2* __restrict__ blk, int xcd) {
+                                                const int2* __restrict__ blk, int remap) {
     int inst;
-    const int w = batch_wave(blk, xcd, inst);
+    const int w = batch_wave(blk, remap, inst);
     const FusedArgs& a = d[inst];
     if (w < (a.m.D + EPW - 1) / EPW) vflux_group<NS, EPW>(a, w);
 }
@@ -809,15 +846,15 @@ __global__ void FCT_OCC(8) kb_vflux(const FusedArgs* __restrict__ d,
 template <int NS, int EPW>
 __global__ void FCT_OCC(6) kf_adfh(FusedArgs a) {
     int w;
-    if (!single_wave((int64_t)((a.m.D + EPW - 1) / EPW) * a.K, a.xcd, w)) return;
+    if (!single_wave((int64_t)((a.m.D + EPW - 1) / EPW) * a.K, a.remap, w)) return;
     const int g = w / a.K;
     adfh_group<NS, EPW>(a, g, w - g * a.K);
 }
 template <int NS, int EPW>
 __global__ void FCT_OCC(6) kb_adfh(const FusedArgs* __restrict__ d,
-                                               const int2* __restrict__ blk, int xcd) {
+                                               const int2* __restrict__ blk, int remap) {
     int inst;
-    const int w = batch_wave(blk, xcd, inst);
+    const int w = batch_wave(blk, remap, inst);
     const FusedArgs& a = d[inst];
     if (w >= ((a.m.D + EPW - 1) / EPW) * a.K) return;
     const int g = w / a.K;
@@ -828,16 +865,16 @@ __global__ void FCT_OCC(6) kb_adfh(const FusedArgs* __restrict__ d,
     template <int NS, int DM>                                                           \
     __global__ void FCT_OCC(DM == 6 ? OCC : 4) kf_##PASS(FusedArgs a) {                 \
         int w;                                                                          \
-        if (!single_wave((int64_t)a.m.N * a.K, a.xcd, w)) return;                       \
+        if (!single_wave((int64_t)a.m.N * a.K, a.remap, w)) return;                       \
         const int n = w / a.K;                                                          \
         PASS##_task<NS, DM>(a, n, w - n * a.K);                                         \
     }                                                                                   \
     template <int NS, int DM>                                                           \
     __global__ void FCT_OCC(DM == 6 ? OCC : 4) kb_##PASS(const FusedArgs* __restrict__ d, \
                                                      const int2* __restrict__ blk,      \
-                                                     int xcd) {                         \
+                                                     int remap) {                         \
         int inst;                                                                       \
-        const int w = batch_wave(blk, xcd, inst);                                       \
+        const int w = batch_wave(blk, remap, inst);                                       \
         const FusedArgs& a = d[inst];                                                   \
         if (w >= a.m.N * a.K) return;                                                   \
         const int n = w / a.K;                                                          \
@@ -911,7 +948,7 @@ static void base_args(FusedArgs& fa, const DevMesh& m, const NodeTab& tab, const
                       double* gnode, double* adfh, double* pf, double* mf) {
     fa.m = m; fa.tab = tab; fa.vflux = vflux; fa.ekr = ekr; fa.uv = a->uv; fa.helem = a->helem;
     fa.w_e = a->w_e; fa.hnode = a->hnode; fa.hnn = a->hnode_new; fa.Z3d = a->Z3d;
-    fa.zbar3 = a->zbar3; fa.dt = a->dt; fa.xcd = tune.xcd; fa.K = 0;
+    fa.zbar3 = a->zbar3; fa.dt = a->dt; fa.remap = tune.remap(); fa.K = 0;
     fa.LO = LO; fa.gnode = (double2*)gnode; fa.adfh = adfh; fa.pf = pf; fa.mf = mf;
 }
 
@@ -986,20 +1023,20 @@ static void launch_batch_t(const BatchGroup& g, const FusedTuning& t, hipStream_
     const dim3 blk(64 * t.wpb);
     if (g.nb_v) {
         if (t.vflux_epw == 2)
-            hipLaunchKernelGGL((kb_vflux<NS, 2>), dim3(g.nb_v), blk, 0, s, g.desc, g.blk_v, t.xcd);
+            hipLaunchKernelGGL((kb_vflux<NS, 2>), dim3(g.nb_v), blk, 0, s, g.desc, g.blk_v, t.remap());
         else
-            hipLaunchKernelGGL((kb_vflux<NS, 4>), dim3(g.nb_v), blk, 0, s, g.desc, g.blk_v, t.xcd);
+            hipLaunchKernelGGL((kb_vflux<NS, 4>), dim3(g.nb_v), blk, 0, s, g.desc, g.blk_v, t.remap());
     }
-    if (g.nb_n) hipLaunchKernelGGL((kb_pass1<NS, DM>), dim3(g.nb_n), blk, 0, s, g.desc, g.blk_n, t.xcd);
+    if (g.nb_n) hipLaunchKernelGGL((kb_pass1<NS, DM>), dim3(g.nb_n), blk, 0, s, g.desc, g.blk_n, t.remap());
     if (g.nb_a) {
         if (t.adfh_epw == 1)
-            hipLaunchKernelGGL((kb_adfh<NS, 1>), dim3(g.nb_a), blk, 0, s, g.desc, g.blk_a, t.xcd);
+            hipLaunchKernelGGL((kb_adfh<NS, 1>), dim3(g.nb_a), blk, 0, s, g.desc, g.blk_a, t.remap());
         else
-            hipLaunchKernelGGL((kb_adfh<NS, 2>), dim3(g.nb_a), blk, 0, s, g.desc, g.blk_a, t.xcd);
+            hipLaunchKernelGGL((kb_adfh<NS, 2>), dim3(g.nb_a), blk, 0, s, g.desc, g.blk_a, t.remap());
     }
     if (g.nb_n) {
-        hipLaunchKernelGGL((kb_pass2<NS, DM>), dim3(g.nb_n), blk, 0, s, g.desc, g.blk_n, t.xcd);
-        hipLaunchKernelGGL((kb_pass3<NS, DM>), dim3(g.nb_n), blk, 0, s, g.desc, g.blk_n, t.xcd);
+        hipLaunchKernelGGL((kb_pass2<NS, DM>), dim3(g.nb_n), blk, 0, s, g.desc, g.blk_n, t.remap());
+        hipLaunchKernelGGL((kb_pass3<NS, DM>), dim3(g.nb_n), blk, 0, s, g.desc, g.blk_n, t.remap());
     }
 }
 

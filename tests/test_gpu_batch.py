@@ -78,7 +78,8 @@ def test_batch_tuning_does_not_change_results(torch_dev):
     results = []
     for cfg in [{}, {_lib.FCT_TUNE_XCD: 1, _lib.FCT_TUNE_WAVES_PER_GROUP: 1},
                 {_lib.FCT_TUNE_VFLUX_EDGES: 2, _lib.FCT_TUNE_ADFH_EDGES: 1,
-                 _lib.FCT_TUNE_WAVES_PER_GROUP: 2}]:
+                 _lib.FCT_TUNE_WAVES_PER_GROUP: 2},
+                {_lib.FCT_TUNE_XCD: 8, _lib.FCT_TUNE_XCD_CHUNK: 5}]:
         _restore(insts, state0, ph0)
         for k, v in cfg.items():
             batch.set_tuning(k, v)
