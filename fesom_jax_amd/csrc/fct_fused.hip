@@ -173,6 +173,25 @@ __device__ __forceinline__ double2 ldv(const double2* p, size_t i) {
 __device__ __forceinline__ double4 ldv(const double4* p, size_t i) {
     return __builtin_bit_cast(double4, ((gptr<const fct_dv4>)p)[i]);
 }
+// Loads at wave-uniform addresses of data that no kernel writes (mesh tables), through
+// the constant address space so that the compiler issues scalar loads: they go through
+// the scalar cache and leave the vector memory pipeline to the column gathers.
+template <class T>
+using cptr = __attribute__((address_space(4))) const T*;
+__device__ __forceinline__ int ldc(const int* p, size_t i) { return ((cptr<int>)p)[i]; }
+__device__ __forceinline__ double ldc(const double* p, size_t i) { return ((cptr<double>)p)[i]; }
+__device__ __forceinline__ int2 ldc(const int2* p, size_t i) {
+    return __builtin_bit_cast(int2, ((cptr<fct_iv2>)p)[i]);
+}
+__device__ __forceinline__ int4 ldc(const int4* p, size_t i) {
+    return __builtin_bit_cast(int4, ((cptr<fct_iv4>)p)[i]);
+}
+__device__ __forceinline__ double2 ldc(const double2* p, size_t i) {
+    return __builtin_bit_cast(double2, ((cptr<fct_dv2>)p)[i]);
+}
+__device__ __forceinline__ double4 ldc(const double4* p, size_t i) {
+    return __builtin_bit_cast(double4, ((cptr<fct_dv4>)p)[i]);
+}
 __device__ __forceinline__ void stv(double2* p, size_t i, double2 v) {
     ((gptr<fct_dv2>)p)[i] = __builtin_bit_cast(fct_dv2, v);
 }
@@ -198,10 +217,10 @@ __device__ __forceinline__ void vflux_group(const FusedArgs& a, int g) {
     for (int q = 0; q < EPW; ++q) {
         const int eq = g * EPW + q;
         e[q] = eq < m.D ? eq : m.D - 1;
-        et[q] = ldv(m.edge_tri, e[q]);
-        kr[q] = eq < m.D ? ldv(a.ekr, e[q]) : make_int2(0, 0);
-        c[q] = ldv(m.cross, e[q]);
-        tr[q] = G(m.etr)[e[q]];
+        et[q] = ldc(m.edge_tri, e[q]);
+        kr[q] = eq < m.D ? ldc(a.ekr, e[q]) : make_int2(0, 0);
+        c[q] = ldc(m.cross, e[q]);
+        tr[q] = ldc(m.etr, e[q]);
         kmax = kr[q].y > kmax ? kr[q].y : kmax;
     }
 #pragma unroll
@@ -241,10 +260,10 @@ template <int NS, int DM>
 __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
     const DevMesh& m = a.m;
     const int L = m.L;
-    const int4 hd = ldv(a.tab.hdr, n);
+    const int4 hd = ldc(a.tab.hdr, n);
     const int2 nr = make_int2(hd.x, hd.y), cnt = make_int2(hd.z & 0xff, hd.w);   // levels, counts
     // levels at which the adf_h pass reads this node's gnode, as a range [glo, ghi)
-    const int ghz = __builtin_amdgcn_readfirstlane(hd.z);
+    const int ghz = hd.z;
     const int glo = (ghz >> 8) & 0xff, ghi = (ghz >> 16) & 0xff;
     const auto T = G(a.T[j]);
     const size_t b0 = (size_t)n * L;
@@ -254,8 +273,8 @@ __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
     int4 el[DM];
 #pragma unroll
     for (int q = 0; q < DM; ++q) {
-        es[q] = ldv(a.tab.edge, (size_t)n * DM + q);
-        el[q] = ldv(a.tab.elem, (size_t)n * DM + q);
+        es[q] = ldc(a.tab.edge, (size_t)n * DM + q);
+        el[q] = ldc(a.tab.elem, (size_t)n * DM + q);
     }
     // per wavefront: the neighbour columns at the lane's level, then the node's own
     __shared__ double s_nb[4][DM + 1][64];
@@ -333,10 +352,12 @@ __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
         for (int q = 0; q < DM; ++q) {
             const int e = el[q].x;
             const bool use = q < cnt.y && in_bytes(es[q].w, 0, k);
-            const auto gs = G(m.gs) + 6 * (size_t)e;
-            const double gx = gs[0] * tv[q][0] + gs[1] * tv[q][1] + gs[2] * tv[q][2];
-            const double gy = gs[3] * tv[q][0] + gs[4] * tv[q][1] + gs[5] * tv[q][2];
-            const double ea = G(m.earea)[e];
+            const size_t g6 = 6 * (size_t)e;
+            const double gx = ldc(m.gs, g6) * tv[q][0] + ldc(m.gs, g6 + 1) * tv[q][1] +
+                              ldc(m.gs, g6 + 2) * tv[q][2];
+            const double gy = ldc(m.gs, g6 + 3) * tv[q][0] + ldc(m.gs, g6 + 4) * tv[q][1] +
+                              ldc(m.gs, g6 + 5) * tv[q][2];
+            const double ea = ldc(m.earea, e);
             nx = nx + (use ? gx * ea : 0.0);
             ny = ny + (use ? gy * ea : 0.0);
             den = den + (use ? ea : 0.0);
@@ -349,11 +370,12 @@ __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
 // ---------------------------------------------------------------------------
 // adf_h(e, k) for EPW consecutive edges, tracer j
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double2 grad_of(gptr<const double> gs, double t0, double t1,
+__device__ __forceinline__ double2 grad_of(const double* gs, int el, double t0, double t1,
                                            double t2) {
+    const size_t g6 = 6 * (size_t)el;
     double2 g;
-    g.x = gs[0] * t0 + gs[1] * t1 + gs[2] * t2;
-    g.y = gs[3] * t0 + gs[4] * t1 + gs[5] * t2;
+    g.x = ldc(gs, g6) * t0 + ldc(gs, g6 + 1) * t1 + ldc(gs, g6 + 2) * t2;
+    g.y = ldc(gs, g6 + 3) * t0 + ldc(gs, g6 + 4) * t1 + ldc(gs, g6 + 5) * t2;
     return g;
 }
 
@@ -373,24 +395,17 @@ __device__ __forceinline__ void adfh_group(const FusedArgs& a, int g, int j) {
     for (int q = 0; q < EPW; ++q) {
         const int eq = g * EPW + q;
         e[q] = eq < m.D ? eq : m.D - 1;
-        kr[q] = eq < m.D ? ldv(a.ekr, e[q]) : make_int2(0, 0);
-        nd[q] = ldv(m.edges, e[q]);
-        ur[q] = ldv(m.utri_rng, e[q]);
-        ud[q] = ldv(m.updn, e[q]);
-        cc[q] = ldv(m.edge_c, e[q]);
-        const int4 va = ldv(m.udv, 2 * (size_t)e[q]), vb = ldv(m.udv, 2 * (size_t)e[q] + 1);
+        kr[q] = eq < m.D ? ldc(a.ekr, e[q]) : make_int2(0, 0);
+        nd[q] = ldc(m.edges, e[q]);
+        ur[q] = ldc(m.utri_rng, e[q]);
+        ud[q] = ldc(m.updn, e[q]);
+        cc[q] = ldc(m.edge_c, e[q]);
+        const int4 va = ldc(m.udv, 2 * (size_t)e[q]), vb = ldc(m.udv, 2 * (size_t)e[q] + 1);
         u[q][0] = va.x; u[q][1] = va.y; u[q][2] = va.z;
         d[q][0] = va.w; d[q][1] = vb.x; d[q][2] = vb.y;
     }
-    // level ranges as wave-uniform values: which of the gradient sources a slot needs
 #pragma unroll
-    for (int q = 0; q < EPW; ++q) {
-        kr[q].x = __builtin_amdgcn_readfirstlane(kr[q].x);
-        kr[q].y = __builtin_amdgcn_readfirstlane(kr[q].y);
-        ur[q].x = __builtin_amdgcn_readfirstlane(ur[q].x);
-        ur[q].y = __builtin_amdgcn_readfirstlane(ur[q].y);
-        kmax = kr[q].y > kmax ? kr[q].y : kmax;
-    }
+    for (int q = 0; q < EPW; ++q) kmax = kr[q].y > kmax ? kr[q].y : kmax;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         if (64 * s >= kmax) break;
@@ -433,8 +448,8 @@ __device__ __forceinline__ void adfh_group(const FusedArgs& a, int g, int j) {
             const double A1 = ab2(O1[q], T1[q]), A2 = ab2(O2[q], T2[q]);
             double2 gu = make_double2(0.0, 0.0), gd = gu;
             if (nt[q]) {
-                gu = grad_of(G(m.gs) + 6 * (size_t)ud[q].x, tu[q][0], tu[q][1], tu[q][2]);
-                gd = grad_of(G(m.gs) + 6 * (size_t)ud[q].y, td[q][0], td[q][1], td[q][2]);
+                gu = grad_of(m.gs, ud[q].x, tu[q][0], tu[q][1], tu[q][2]);
+                gd = grad_of(m.gs, ud[q].y, td[q][0], td[q][1], td[q][2]);
             }
             const double g0 = tri ? gu.x : ga[q].x, g1 = tri ? gd.x : gb[q].x;
             const double g2 = tri ? gu.y : ga[q].y, g3 = tri ? gd.y : gb[q].y;
@@ -538,7 +553,7 @@ __device__ __forceinline__ void pass2_body(const FusedArgs& a, int n, int j, int
     const int kneed = nr.y + 1 < L ? nr.y + 1 : L;
     int4 es[DM];
 #pragma unroll
-    for (int q = 0; q < DM; ++q) es[q] = ldv(a.tab.edge, (size_t)n * DM + q);
+    for (int q = 0; q < DM; ++q) es[q] = ldc(a.tab.edge, (size_t)n * DM + q);
     double Tc[NS], Ac[NS], LOc[NS], tvx[NS], tvn[NS], ap[NS], am[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -631,7 +646,7 @@ __device__ __forceinline__ void pass2_body(const FusedArgs& a, int n, int j, int
 
 template <int NS, int DM>
 __device__ __forceinline__ void pass2_task(const FusedArgs& a, int n, int j) {
-    const int4 hd = ldv(a.tab.hdr, n);
+    const int4 hd = ldc(a.tab.hdr, n);
     const int2 nr = make_int2(hd.x, hd.y), cnt = make_int2(hd.z & 0xff, hd.w);
     if (NS == 1 || nr.y < 64) pass2_body<1, DM>(a, n, j, nr, cnt);
     else pass2_body<NS, DM>(a, n, j, nr, cnt);
@@ -653,7 +668,7 @@ __device__ __forceinline__ void pass3_body(const FusedArgs& a, int n, int j, int
     const int kneed = nr.y + 1 < L ? nr.y + 1 : L;
     int4 es[DM];
 #pragma unroll
-    for (int q = 0; q < DM; ++q) es[q] = ldv(a.tab.edge, (size_t)n * DM + q);
+    for (int q = 0; q < DM; ++q) es[q] = ldc(a.tab.edge, (size_t)n * DM + q);
     double Tc[NS], Ac[NS], Pc[NS], Mc[NS], acc[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -741,7 +756,7 @@ __device__ __forceinline__ void pass3_body(const FusedArgs& a, int n, int j, int
 
 template <int NS, int DM>
 __device__ __forceinline__ void pass3_task(const FusedArgs& a, int n, int j) {
-    const int4 hd = ldv(a.tab.hdr, n);
+    const int4 hd = ldc(a.tab.hdr, n);
     const int2 nr = make_int2(hd.x, hd.y), cnt = make_int2(hd.z & 0xff, hd.w);
     if (NS == 1 || nr.y < 64) pass3_body<1, DM>(a, n, j, nr, cnt);
     else pass3_body<NS, DM>(a, n, j, nr, cnt);
@@ -756,7 +771,7 @@ template <int NS, int DM>
 __device__ __forceinline__ void upwind_task(const FusedArgs& a, int n, int j) {
     const DevMesh& m = a.m;
     const int L = m.L;
-    const int4 hd = ldv(a.tab.hdr, n);
+    const int4 hd = ldc(a.tab.hdr, n);
     const int2 nr = make_int2(hd.x, hd.y), cnt = make_int2(hd.z & 0xff, hd.w);
     const auto T = G(a.T[j]);
     const auto To = G(a.Told[j]);
@@ -764,7 +779,7 @@ __device__ __forceinline__ void upwind_task(const FusedArgs& a, int n, int j) {
     const size_t b0 = (size_t)n * L;
     int4 es[DM];
 #pragma unroll
-    for (int q = 0; q < DM; ++q) es[q] = ldv(a.tab.edge, (size_t)n * DM + q);
+    for (int q = 0; q < DM; ++q) es[q] = ldc(a.tab.edge, (size_t)n * DM + q);
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         const int k = 64 * s + lane_id();
