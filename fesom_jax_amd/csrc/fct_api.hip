@@ -492,6 +492,7 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
     // fused-path edge tables: packed triangle layer ranges and gradient-triangle vertices
     std::vector<int> h_etr;
     std::vector<int4> h_udv;
+    bool updn_off_edge = false;   // an up/dn triangle without the edge's node
     if (dm) {
         h_etr.assign(D, 0);
         h_udv.assign(2 * (size_t)D, make_int4(0, 0, 0, 0));
@@ -505,12 +506,30 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
                 packed |= (lo | (hi << 8)) << (16 * side);
             }
             h_etr[e] = (int)packed;
+            // up triangle: holds the edge's first node at slot cu; down triangle: its
+            // second node at slot cd (the triangles upwind of each end, tracer_adv.py:
+            // 351-363); the kernel reads those two vertices from the edge's own columns
             const int u = h_updn[e].x, w = h_updn[e].y;
-            h_udv[2 * (size_t)e] = make_int4(d->elem_nodes[3 * u], d->elem_nodes[3 * u + 1],
-                                            d->elem_nodes[3 * u + 2], d->elem_nodes[3 * w]);
-            h_udv[2 * (size_t)e + 1] = make_int4(d->elem_nodes[3 * w + 1],
-                                                d->elem_nodes[3 * w + 2], 0, 0);
+            const int* eu = d->elem_nodes + 3 * (size_t)u;
+            const int* ew = d->elem_nodes + 3 * (size_t)w;
+            int cu = 0, cd = 0;
+            if (d->edge_up_dn_tri[2 * e] >= 0 && d->edge_up_dn_tri[2 * e + 1] >= 0) {
+                cu = cd = -1;
+                for (int i = 2; i >= 0; --i) {
+                    if (eu[i] == h_edges[e].x) cu = i;
+                    if (ew[i] == h_edges[e].y) cd = i;
+                }
+                if (cu < 0 || cd < 0) { updn_off_edge = true; cu = cd = 0; }
+            }
+            int ou[2], od[2];
+            for (int i = 0, a = 0, b = 0; i < 3; ++i) {
+                if (i != cu) ou[a++] = eu[i];
+                if (i != cd) od[b++] = ew[i];
+            }
+            h_udv[2 * (size_t)e] = make_int4(ou[0], ou[1], od[0], od[1]);
+            h_udv[2 * (size_t)e + 1] = make_int4(cu | (cd << 2), 0, 0, 0);
         }
+        if (updn_off_edge) dm = 0;   // the fused adf_h pass relies on it
     }
     // static QR4C stencil depths (_z_stencil, tracer_adv.py:228-238)
     std::vector<double> Zp(L), h_Zs(4 * (size_t)L);

@@ -40,8 +40,10 @@ struct DevMesh {
     const double* Zs;         // [4*L] static QR4C stencil depths Z[k], Z[k-1], Z[k-2], Z[k+1]
     const double* zbar;       // [L]
     // fused-path edge tables (nl <= 128): the layer ranges of an edge's two triangles as
-    // bytes lo1|hi1|lo2|hi2 (a missing triangle: empty), and the vertices of its up and
-    // down gradient triangles (u0, u1, u2, d0), (d1, d2, -, -)
+    // bytes lo1|hi1|lo2|hi2 (a missing triangle: empty), and the up and down gradient
+    // triangles: the two vertices other than the edge's first node (up) and second node
+    // (down), in vertex order, (u_a, u_b, d_a, d_b), then (cu | cd << 2, -, -, -) with
+    // cu, cd the vertex slots of the edge's own nodes
     const int* etr;           // [D]
     const int4* udv;          // [2*D]
 };

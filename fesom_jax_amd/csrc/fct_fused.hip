@@ -293,12 +293,18 @@ __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
         // all neighbour columns and edge fluxes at once (one basic block: slot indices
         // past the node's count repeat slot 0 and are ignored when summing)
         double To[DM] = {}, vf[DM] = {}, tv[DM][3] = {};
+        // the node's own fields at k (and k+1 for the lower interface), with the gathers
+        double w0 = 0.0, a0 = 0.0, w1 = 0.0, a1 = 0.0, av = 0.0, hn = 0.0, hd = 0.0;
+        const size_t i = b0 + k, ip = b0 + (k + 1 < L ? k + 1 : k);
         if (wet) {
 #pragma unroll
             for (int q = 0; q < DM; ++q) {
                 vf[q] = G(a.vflux)[(size_t)(es[q].x >> 1) * L + k];
                 To[q] = T[(size_t)es[q].y * L + k];
             }
+            w0 = G(a.w_e)[i]; a0 = G(m.area)[i];
+            w1 = G(a.w_e)[ip]; a1 = G(m.area)[ip];
+            av = G(m.areasvol)[i]; hn = G(a.hnn)[i]; hd = G(a.hnode)[i];
             // the element vertices are the node and edge neighbours: read them back
             // from the lane's own LDS row by slot code (bits 16.. of the element-slot
             // word: 4 bits per vertex, DM = the node) instead of loading them again
@@ -330,19 +336,17 @@ __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
             const bool use = q < cnt.x && v != 0.0;
             acc = acc + (use ? (side ? -f : f) : 0.0);
         }
-        const size_t i = b0 + k;
         double fv0, fv1 = 0.0;
         {
-            const double wk = G(a.w_e)[i], aw = fabs(wk);
-            fv0 = (-0.5 * (Tk * (wk + aw) + Tm * (wk - aw))) * G(m.area)[i];
+            const double aw = fabs(w0);
+            fv0 = (-0.5 * (Tk * (w0 + aw) + Tm * (w0 - aw))) * a0;
         }
         if (in_layer(nr, k + 1)) {
-            const double wk = G(a.w_e)[i + 1], aw = fabs(wk);
-            fv1 = (-0.5 * (Tp * (wk + aw) + Tk * (wk - aw))) * G(m.area)[i + 1];
+            const double aw = fabs(w1);
+            fv1 = (-0.5 * (Tp * (w1 + aw) + Tk * (w1 - aw))) * a1;
         }
-        const double av = G(m.areasvol)[i], hn = G(a.hnn)[i];
         const double safe_a = av > 0.0 ? av : 1.0, safe_hn = hn > 0.0 ? hn : 1.0;
-        const double numer = Tk * G(a.hnode)[i] + (acc + (fv0 - fv1)) * a.dt / safe_a;
+        const double numer = Tk * hd + (acc + (fv0 - fv1)) * a.dt / safe_a;
         const double lo = numer / safe_hn;
         G(a.LO)[jo + i] = (hn > 0.0 && av > 0.0) ? lo : 0.0;
         // gnode: elements in increasing id, only on the levels the adf_h pass reads
@@ -387,7 +391,7 @@ __device__ __forceinline__ void adfh_group(const FusedArgs& a, int g, int j) {
     const auto To = G(a.Told[j]);
     const double2* gn = a.gnode + (size_t)j * m.N * L;
     const auto out = G(a.adfh + (size_t)j * m.D * L);
-    int e[EPW], u[EPW][3], d[EPW][3];
+    int e[EPW], u[EPW][2], d[EPW][2], cu[EPW], cd[EPW];
     int2 kr[EPW], nd[EPW], ur[EPW], ud[EPW];
     double2 cc[EPW];
     int kmax = 0;
@@ -400,9 +404,10 @@ __device__ __forceinline__ void adfh_group(const FusedArgs& a, int g, int j) {
         ur[q] = ldc(m.utri_rng, e[q]);
         ud[q] = ldc(m.updn, e[q]);
         cc[q] = ldc(m.edge_c, e[q]);
-        const int4 va = ldc(m.udv, 2 * (size_t)e[q]), vb = ldc(m.udv, 2 * (size_t)e[q] + 1);
-        u[q][0] = va.x; u[q][1] = va.y; u[q][2] = va.z;
-        d[q][0] = va.w; d[q][1] = vb.x; d[q][2] = vb.y;
+        const int4 va = ldc(m.udv, 2 * (size_t)e[q]);
+        const int vb = ldc(reinterpret_cast<const int*>(m.udv), 8 * (size_t)e[q] + 4);
+        u[q][0] = va.x; u[q][1] = va.y; d[q][0] = va.z; d[q][1] = va.w;
+        cu[q] = vb & 3; cd[q] = (vb >> 2) & 3;
     }
 #pragma unroll
     for (int q = 0; q < EPW; ++q) kmax = kr[q].y > kmax ? kr[q].y : kmax;
@@ -433,11 +438,16 @@ __device__ __forceinline__ void adfh_group(const FusedArgs& a, int g, int j) {
                 T1[q] = T[i1]; T2[q] = T[i2]; O1[q] = To[i1]; O2[q] = To[i2];
                 if (ng[q]) { ga[q] = ldv(gn, i1); gb[q] = ldv(gn, i2); }
                 if (nt[q]) {
-#pragma unroll
-                    for (int v = 0; v < 3; ++v) {
-                        tu[q][v] = T[(size_t)u[q][v] * L + k];
-                        td[q][v] = T[(size_t)d[q][v] * L + k];
-                    }
+                    // the up triangle's vertex at slot cu is the edge's first node, the down
+                    // triangle's at slot cd its second: their T is already loaded
+                    const double a0 = T[(size_t)u[q][0] * L + k], a1 = T[(size_t)u[q][1] * L + k];
+                    const double b0 = T[(size_t)d[q][0] * L + k], b1 = T[(size_t)d[q][1] * L + k];
+                    tu[q][0] = cu[q] == 0 ? T1[q] : a0;
+                    tu[q][1] = cu[q] == 0 ? a0 : (cu[q] == 1 ? T1[q] : a1);
+                    tu[q][2] = cu[q] == 2 ? T1[q] : a1;
+                    td[q][0] = cd[q] == 0 ? T2[q] : b0;
+                    td[q][1] = cd[q] == 0 ? b0 : (cd[q] == 1 ? T2[q] : b1);
+                    td[q][2] = cd[q] == 2 ? T2[q] : b1;
                 }
             }
         }
@@ -470,11 +480,10 @@ __device__ __forceinline__ void adfh_group(const FusedArgs& a, int g, int j) {
 // vertical antidiffusive flux (adv_flux_ver_ho - adv_flux_ver, tracer_adv.py:241-295,
 // 108-120) at interface k from the column's ttfAB at k-2..k+1 and T at k-1, k
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double adfv_at(const DevMesh& m, int k, int2 nr, size_t i, size_t b0,
-                                          double A2, double A1, double A, double Ap, double Tm,
-                                          double Tk, double W, double ar,
-                                          gptr<const double> Z3d, gptr<const double> zbar3) {
-    const int L = m.L;
+__device__ __forceinline__ double adfv_at(int k, int2 nr, double A2, double A1, double A,
+                                          double Ap, double Tm, double Tk, double W, double ar,
+                                          double Znz, double Zm1, double Zm2, double Zp1,
+                                          double zb) {
     double hi = 0.0;
     if (in_iface(nr, k)) {
         if (k == nr.x) {
@@ -484,16 +493,6 @@ __device__ __forceinline__ double adfv_at(const DevMesh& m, int k, int2 nr, size
         } else if (k == nr.x + 1 || k == nr.y - 1) {
             hi = -0.5 * (A1 + A) * W * ar;
         } else if (k >= nr.x + 2 && k <= nr.y - 2) {
-            double Znz, Zm1, Zm2, Zp1, zb;
-            if (Z3d) {
-                const int km1 = k > 0 ? k - 1 : 0, km2 = k > 1 ? k - 2 : 0;
-                const int kp1 = k + 1 < L ? k + 1 : L - 1;
-                Znz = Z3d[i]; Zm1 = Z3d[b0 + km1]; Zm2 = Z3d[b0 + km2]; Zp1 = Z3d[b0 + kp1];
-                zb = zbar3[i];
-            } else {
-                Znz = G(m.Zs)[k]; Zm1 = G(m.Zs)[L + k]; Zm2 = G(m.Zs)[2 * L + k]; Zp1 = G(m.Zs)[3 * L + k];
-                zb = G(m.zbar)[k];
-            }
             const double qc = (A1 - A) / nz_safe(Zm1 - Znz);
             const double qu = (A - Ap) / nz_safe(Znz - Zp1);
             const double qd = (A2 - A1) / nz_safe(Zm2 - Zm1);
@@ -526,12 +525,25 @@ __device__ __forceinline__ void adfv_column(const FusedArgs& a, int n, int2 nr,
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         const int k = 64 * s + lane_id();
-        out[s] = 0.0;
-        if (k <= nr.y && k < L) {
-            const size_t i = b0 + k;
-            out[s] = adfv_at(m, k, nr, i, b0, A2[s], A1[s], Ac[s], Ap[s], Tm[s], Tc[s],
-                             G(a.w_e)[i], G(m.area)[i], G(a.Z3d), G(a.zbar3));
+        // the column's w, area and stencil depths at the lane (clamped below the last
+        // level), loaded together before any branch
+        const int kc = k < L ? k : L - 1;
+        const size_t i = b0 + kc;
+        const double W = G(a.w_e)[i], ar = G(m.area)[i];
+        double Znz, Zm1, Zm2, Zp1, zb;
+        if (a.Z3d) {
+            const int km1 = kc > 0 ? kc - 1 : 0, km2 = kc > 1 ? kc - 2 : 0;
+            const int kp1 = kc + 1 < L ? kc + 1 : L - 1;
+            Znz = G(a.Z3d)[i]; Zm1 = G(a.Z3d)[b0 + km1]; Zm2 = G(a.Z3d)[b0 + km2];
+            Zp1 = G(a.Z3d)[b0 + kp1]; zb = G(a.zbar3)[i];
+        } else {
+            Znz = G(m.Zs)[kc]; Zm1 = G(m.Zs)[L + kc]; Zm2 = G(m.Zs)[2 * L + kc];
+            Zp1 = G(m.Zs)[3 * L + kc]; zb = G(m.zbar)[kc];
         }
+        out[s] = 0.0;
+        if (k <= nr.y && k < L)
+            out[s] = adfv_at(k, nr, A2[s], A1[s], Ac[s], Ap[s], Tm[s], Tc[s], W, ar, Znz, Zm1,
+                             Zm2, Zp1, zb);
     }
 }
 
