@@ -570,7 +570,16 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
     p->host_edge_tri.assign(d->edge_tri, d->edge_tri + 2 * (size_t)D);
     p->host_updn.assign(d->edge_up_dn_tri, d->edge_up_dn_tri + 2 * (size_t)D);
     p->tab.dm = dm;
+    std::vector<int> h_nlist;
+    if (dm && L > 64) {
+        for (int n = 0; n < N; ++n)
+            if (d->nlevels_nod2D[n] - 1 < 64) h_nlist.push_back(n);
+        p->tab.nshallow = (int)h_nlist.size();
+        for (int n = 0; n < N; ++n)
+            if (d->nlevels_nod2D[n] - 1 >= 64) h_nlist.push_back(n);
+    }
     if (dm && ((rc = dev_upload(p, &p->tab.hdr, h_thdr.data(), h_thdr.size())) ||
+               (!h_nlist.empty() && (rc = dev_upload(p, &p->tab.nlist, h_nlist.data(), h_nlist.size()))) ||
                (rc = dev_upload(p, &m.etr, h_etr.data(), h_etr.size())) ||
                (rc = dev_upload(p, &m.udv, h_udv.data(), h_udv.size())) ||
                (rc = dev_upload(p, &p->tab.edge, h_tedge.data(), h_tedge.size())) ||

@@ -58,6 +58,11 @@ struct NodeTab {
                               //  edge's triangles as bytes lo1|hi1|lo2|hi2, layer range of
                               //  element slot q as bytes lo|hi), edges in reference order
     const int4* elem;         // [N*dm] (element, v0, v1, v2), increasing element id
+    // nl > 64 only: the nodes whose wet layers end above level 64 (`nshallow` of them, in
+    // node order), then the others; passes 2 and 3 take each set in a launch of its own,
+    // the first with one 64-level slot per wavefront and fewer registers
+    const int* nlist = nullptr;
+    int nshallow = 0;
     // slots past a node's counts repeat its first slot, so a kernel can load every slot
     // without waiting for the counts
 };

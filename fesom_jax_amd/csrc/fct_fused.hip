@@ -522,28 +522,35 @@ __device__ __forceinline__ void adfv_column(const FusedArgs& a, int n, int2 nr,
     col_prev<NS>(Ac, A1, col_bcast0<NS>(Ac));
     col_prev<NS>(A1, A2, col_bcast0<NS>(Ac));
     col_next<NS>(Ac, Ap, 0.0);
+    // the column's w, area, depth Z at k and interface depth at every lane (clamped
+    // below the last level); the stencil depths at k-1, k-2 and k+1 are lane shifts of Z
+    double W[NS], ar[NS], Zc[NS], zb[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         const int k = 64 * s + lane_id();
-        // the column's w, area and stencil depths at the lane (clamped below the last
-        // level), loaded together before any branch
         const int kc = k < L ? k : L - 1;
         const size_t i = b0 + kc;
-        const double W = G(a.w_e)[i], ar = G(m.area)[i];
-        double Znz, Zm1, Zm2, Zp1, zb;
-        if (a.Z3d) {
-            const int km1 = kc > 0 ? kc - 1 : 0, km2 = kc > 1 ? kc - 2 : 0;
-            const int kp1 = kc + 1 < L ? kc + 1 : L - 1;
-            Znz = G(a.Z3d)[i]; Zm1 = G(a.Z3d)[b0 + km1]; Zm2 = G(a.Z3d)[b0 + km2];
-            Zp1 = G(a.Z3d)[b0 + kp1]; zb = G(a.zbar3)[i];
-        } else {
-            Znz = G(m.Zs)[kc]; Zm1 = G(m.Zs)[L + kc]; Zm2 = G(m.Zs)[2 * L + kc];
-            Zp1 = G(m.Zs)[3 * L + kc]; zb = G(m.zbar)[kc];
-        }
+        W[s] = G(a.w_e)[i];
+        ar[s] = G(m.area)[i];
+        Zc[s] = a.Z3d ? G(a.Z3d)[i] : G(m.Zs)[kc];
+        zb[s] = a.Z3d ? G(a.zbar3)[i] : G(m.zbar)[kc];
+    }
+    // static: Z[k-1], Z[k-2] clamped to Z[0] and Z[k] below level 2 (_z_stencil,
+    // tracer_adv.py:228-238); Z3d: Z3d[max(k-1, 0)], Z3d[max(k-2, 0)]; both Z[min(k+1,
+    // L-1)] above, which the clamped lanes past the last level supply
+    const double z0 = col_bcast0<NS>(Zc);
+    double Zm1[NS], Zm2r[NS], Zp1[NS];
+    col_prev<NS>(Zc, Zm1, z0);
+    col_prev<NS>(Zm1, Zm2r, z0);
+    col_next<NS>(Zc, Zp1, lane_value(Zc[NS - 1], 63));
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int k = 64 * s + lane_id();
+        const double Zm2 = k < 2 ? (a.Z3d ? z0 : Zc[s]) : Zm2r[s];
         out[s] = 0.0;
         if (k <= nr.y && k < L)
-            out[s] = adfv_at(k, nr, A2[s], A1[s], Ac[s], Ap[s], Tm[s], Tc[s], W, ar, Znz, Zm1,
-                             Zm2, Zp1, zb);
+            out[s] = adfv_at(k, nr, A2[s], A1[s], Ac[s], Ap[s], Tm[s], Tc[s], W[s], ar[s], Zc[s],
+                             Zm1[s], Zm2, Zp1[s], zb[s]);
     }
 }
 
@@ -914,6 +921,15 @@ FCT_NODE_KERNELS(pass2, 5)
 FCT_NODE_KERNELS(pass3, 6)
 #undef FCT_NODE_KERNELS
 
+// Passes 2 and 3 over `count` nodes of the node list from `first`, all of whose wet
+// layers fit NSB slots: shallow columns (NSB = 1) run with fewer registers per
+// wavefront and so more wavefronts per SIMD than the two-slot body needs.
+#define FCT_LIST_KERNELS(PASS, OCC1, OCC2)                                                  template <int NSB, int DM>                                                              __global__ void FCT_OCC(DM == 6 ? (NSB == 1 ? OCC1 : OCC2) : 4)                              kl_##PASS(FusedArgs a, int first, int count) {                                          int w;                                                                                  if (!single_wave((int64_t)count * a.K, a.remap, w)) return;                              const int t = w / a.K;                                                                  const int n = ldc(a.tab.nlist, (size_t)first + t);                                      const int4 hd = ldc(a.tab.hdr, n);                                                      PASS##_body<NSB, DM>(a, n, w - t * a.K, make_int2(hd.x, hd.y),                                               make_int2(hd.z & 0xff, hd.w));                                 }
+
+FCT_LIST_KERNELS(pass2, 7, 4)
+FCT_LIST_KERNELS(pass3, 8, 6)
+#undef FCT_LIST_KERNELS
+
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
@@ -940,8 +956,27 @@ static void launch_single(const FusedArgs& fa, const FusedTuning& t, hipStream_t
             else
                 hipLaunchKernelGGL((kf_adfh<NS, 2>), dim3(blocks_for(adfh_waves(m.D, fa.K, t), t.wpb)), blk, 0, s, fa);
             break;
-        case 3: hipLaunchKernelGGL((kf_pass2<NS, DM>), dim3(gn), blk, 0, s, fa); break;
-        case 4: hipLaunchKernelGGL((kf_pass3<NS, DM>), dim3(gn), blk, 0, s, fa); break;
+        case 3:
+        case 4: {
+            const NodeTab& tb = fa.tab;
+            if (NS == 2 && tb.nlist) {
+                const int ns = tb.nshallow, nd = m.N - tb.nshallow;
+                const unsigned g1 = blocks_for(node_waves(ns, fa.K), t.wpb);
+                const unsigned g2 = blocks_for(node_waves(nd, fa.K), t.wpb);
+                if (stage == 3) {
+                    if (ns) hipLaunchKernelGGL((kl_pass2<1, DM>), dim3(g1), blk, 0, s, fa, 0, ns);
+                    if (nd) hipLaunchKernelGGL((kl_pass2<2, DM>), dim3(g2), blk, 0, s, fa, ns, nd);
+                } else {
+                    if (ns) hipLaunchKernelGGL((kl_pass3<1, DM>), dim3(g1), blk, 0, s, fa, 0, ns);
+                    if (nd) hipLaunchKernelGGL((kl_pass3<2, DM>), dim3(g2), blk, 0, s, fa, ns, nd);
+                }
+            } else if (stage == 3) {
+                hipLaunchKernelGGL((kf_pass2<NS, DM>), dim3(gn), blk, 0, s, fa);
+            } else {
+                hipLaunchKernelGGL((kf_pass3<NS, DM>), dim3(gn), blk, 0, s, fa);
+            }
+            break;
+        }
     }
 }
 
