@@ -39,18 +39,27 @@ namespace fct {
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 
-// value of lane l-1 (wave_shr:1) or l+1 (wave_shl:1) of a 64-bit register; lane 0 resp.
-// 63 receives 0. Called with all 64 lanes active.
-__device__ __forceinline__ double wave_shr1(double v) {
+// value of lane l-1 (wave_shr:1) of a 64-bit register, lane 0 keeping `first`, and of
+// lane l+1 (wave_shl:1), lane 63 keeping `last` (the DPP move leaves a lane whose source
+// is out of range at its old value). Called with all 64 lanes active: a lane whose
+// source lane is switched off would keep the old value too. The edge value goes in as
+// the old value rather than through a select on the lane index, so the compiler has no
+// select to turn into a branch around the move (which would switch off the lanes it
+// reads from); the empty asm keeps the move where it is written.
+__device__ __forceinline__ double wave_prev(double v, double first) {
     int2 x = __builtin_bit_cast(int2, v);
-    x.x = __builtin_amdgcn_update_dpp(0, x.x, 0x138, 0xf, 0xf, false);
-    x.y = __builtin_amdgcn_update_dpp(0, x.y, 0x138, 0xf, 0xf, false);
+    const int2 o = __builtin_bit_cast(int2, first);
+    x.x = __builtin_amdgcn_update_dpp(o.x, x.x, 0x138, 0xf, 0xf, false);
+    x.y = __builtin_amdgcn_update_dpp(o.y, x.y, 0x138, 0xf, 0xf, false);
+    asm volatile("" : "+v"(x.x), "+v"(x.y));
     return __builtin_bit_cast(double, x);
 }
-__device__ __forceinline__ double wave_shl1(double v) {
+__device__ __forceinline__ double wave_next(double v, double last) {
     int2 x = __builtin_bit_cast(int2, v);
-    x.x = __builtin_amdgcn_update_dpp(0, x.x, 0x130, 0xf, 0xf, false);
-    x.y = __builtin_amdgcn_update_dpp(0, x.y, 0x130, 0xf, 0xf, false);
+    const int2 o = __builtin_bit_cast(int2, last);
+    x.x = __builtin_amdgcn_update_dpp(o.x, x.x, 0x130, 0xf, 0xf, false);
+    x.y = __builtin_amdgcn_update_dpp(o.y, x.y, 0x130, 0xf, 0xf, false);
+    asm volatile("" : "+v"(x.x), "+v"(x.y));
     return __builtin_bit_cast(double, x);
 }
 __device__ __forceinline__ double lane_value(double v, int l) {
@@ -63,25 +72,17 @@ __device__ __forceinline__ double lane_value(double v, int l) {
 // value at level k-1 of a column held as NS slots; `at0` is used at k = 0
 template <int NS>
 __device__ __forceinline__ void col_prev(const double (&v)[NS], double (&o)[NS], double at0) {
-    const int l = lane_id();
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-        const double up = wave_shr1(v[s]);
-        const double wrap = (s > 0) ? lane_value(v[s > 0 ? s - 1 : 0], 63) : at0;
-        o[s] = (l == 0) ? wrap : up;
-    }
+    for (int s = 0; s < NS; ++s)
+        o[s] = wave_prev(v[s], (s > 0) ? lane_value(v[s > 0 ? s - 1 : 0], 63) : at0);
 }
 
 // value at level k+1; `atend` is used past the last slot lane
 template <int NS>
 __device__ __forceinline__ void col_next(const double (&v)[NS], double (&o)[NS], double atend) {
-    const int l = lane_id();
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-        const double dn = wave_shl1(v[s]);
-        const double wrap = (s + 1 < NS) ? lane_value(v[s + 1 < NS ? s + 1 : s], 0) : atend;
-        o[s] = (l == 63) ? wrap : dn;
-    }
+    for (int s = 0; s < NS; ++s)
+        o[s] = wave_next(v[s], (s + 1 < NS) ? lane_value(v[s + 1 < NS ? s + 1 : s], 0) : atend);
 }
 
 template <int NS>
@@ -257,10 +258,9 @@ __device__ __forceinline__ void vflux_group(const FusedArgs& a, int g) {
 // pass 1 of node n, tracer j: LO and gnode
 // ---------------------------------------------------------------------------
 template <int NS, int DM>
-__device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
+__device__ __forceinline__ void pass1_body(const FusedArgs& a, int n, int j, int4 hd) {
     const DevMesh& m = a.m;
     const int L = m.L;
-    const int4 hd = ldc(a.tab.hdr, n);
     const int2 nr = make_int2(hd.x, hd.y), cnt = make_int2(hd.z & 0xff, hd.w);   // levels, counts
     // levels at which the adf_h pass reads this node's gnode, as a range [glo, ghi)
     const int ghz = hd.z;
@@ -286,15 +286,19 @@ __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
         const bool rd = k < kneed;
         const bool wet = in_layer(nr, k);
         const bool gslot = glo < ghi && glo < 64 * s + 64 && ghi > 64 * s;
-        // own column, k-1 and k+1 of T for the vertical fluxes
+        // own column, k-1 and k+1 of T for the vertical fluxes (the one-slot body takes
+        // k-1 and k+1 from the neighbouring lanes below)
         const double Tk = rd ? T[b0 + k] : 0.0;
-        const double Tm = (rd && k > 0) ? T[b0 + k - 1] : Tk;
-        const double Tp = (k + 1 < kneed) ? T[b0 + k + 1] : 0.0;
+        double Tm = 0.0, Tp = 0.0;
+        if (NS > 1) {
+            Tm = (rd && k > 0) ? T[b0 + k - 1] : Tk;
+            Tp = (k + 1 < kneed) ? T[b0 + k + 1] : 0.0;
+        }
         // all neighbour columns and edge fluxes at once (one basic block: slot indices
         // past the node's count repeat slot 0 and are ignored when summing)
         double To[DM] = {}, vf[DM] = {}, tv[DM][3] = {};
         // the node's own fields at k (and k+1 for the lower interface), with the gathers
-        double w0 = 0.0, a0 = 0.0, w1 = 0.0, a1 = 0.0, av = 0.0, hn = 0.0, hd = 0.0;
+        double w0 = 0.0, a0 = 0.0, w1 = 0.0, a1 = 0.0, av = 0.0, hn = 0.0, hnd = 0.0;
         const size_t i = b0 + k, ip = b0 + (k + 1 < L ? k + 1 : k);
         if (wet) {
 #pragma unroll
@@ -303,8 +307,8 @@ __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
                 To[q] = T[(size_t)es[q].y * L + k];
             }
             w0 = G(a.w_e)[i]; a0 = G(m.area)[i];
-            w1 = G(a.w_e)[ip]; a1 = G(m.area)[ip];
-            av = G(m.areasvol)[i]; hn = G(a.hnn)[i]; hd = G(a.hnode)[i];
+            if (NS > 1) { w1 = G(a.w_e)[ip]; a1 = G(m.area)[ip]; }
+            av = G(m.areasvol)[i]; hn = G(a.hnn)[i]; hnd = G(a.hnode)[i];
             // the element vertices are the node and edge neighbours: read them back
             // from the lane's own LDS row by slot code (bits 16.. of the element-slot
             // word: 4 bits per vertex, DM = the node) instead of loading them again
@@ -320,6 +324,15 @@ __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
                     tv[q][2] = nb[((codes >> 8) & 0xf) * 64];
                 }
             }
+        }
+        if (NS == 1) {
+            // all lanes active here. Lanes past the loaded range hold 0, which is what
+            // the two-slot body reads there; lane 63 is never used for k + 1 in one slot.
+            // Tm is read on wet lanes only, where it is T[k-1], or T[0] at k = 0
+            Tm = wave_prev(Tk, Tk);
+            Tp = wave_next(Tk, 0.0);
+            w1 = wave_next(w0, 0.0);
+            a1 = wave_next(a0, 0.0);
         }
         if (!wet) continue;
         // LO: edges in reference order. Skipped terms add +0.0, which leaves a sum that
@@ -346,7 +359,7 @@ __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
             fv1 = (-0.5 * (Tp * (w1 + aw) + Tk * (w1 - aw))) * a1;
         }
         const double safe_a = av > 0.0 ? av : 1.0, safe_hn = hn > 0.0 ? hn : 1.0;
-        const double numer = Tk * hd + (acc + (fv0 - fv1)) * a.dt / safe_a;
+        const double numer = Tk * hnd + (acc + (fv0 - fv1)) * a.dt / safe_a;
         const double lo = numer / safe_hn;
         G(a.LO)[jo + i] = (hn > 0.0 && av > 0.0) ? lo : 0.0;
         // gnode: elements in increasing id, only on the levels the adf_h pass reads
@@ -369,6 +382,11 @@ __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
         const double d = den > 0.0 ? den : 1.0;
         if (k >= glo && k < ghi) stv(a.gnode, jo + i, make_double2(nx / d, ny / d));
     }
+}
+
+template <int NS, int DM>
+__device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
+    pass1_body<NS, DM>(a, n, j, ldc(a.tab.hdr, n));
 }
 
 // ---------------------------------------------------------------------------
@@ -930,6 +948,16 @@ FCT_LIST_KERNELS(pass2, 7, 4)
 FCT_LIST_KERNELS(pass3, 8, 6)
 #undef FCT_LIST_KERNELS
 
+template <int NSB, int DM>
+__global__ void FCT_OCC(DM == 6 ? 5 : 4) kl_pass1(FusedArgs a, int first,
+                                                                   int count) {
+    int w;
+    if (!single_wave((int64_t)count * a.K, a.remap, w)) return;
+    const int t = w / a.K;
+    const int n = ldc(a.tab.nlist, (size_t)first + t);
+    pass1_body<NSB, DM>(a, n, w - t * a.K, ldc(a.tab.hdr, n));
+}
+
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
@@ -949,7 +977,17 @@ static void launch_single(const FusedArgs& fa, const FusedTuning& t, hipStream_t
             else
                 hipLaunchKernelGGL((kf_vflux<NS, 4>), dim3(blocks_for(vflux_waves(m.D, t), t.wpb)), blk, 0, s, fa);
             break;
-        case 1: hipLaunchKernelGGL((kf_pass1<NS, DM>), dim3(gn), blk, 0, s, fa); break;
+        case 1:
+            if (NS == 2 && fa.tab.nlist) {
+                const int ns = fa.tab.nshallow, nd = m.N - fa.tab.nshallow;
+                if (ns) hipLaunchKernelGGL((kl_pass1<1, DM>), dim3(blocks_for(node_waves(ns, fa.K), t.wpb)),
+                                           blk, 0, s, fa, 0, ns);
+                if (nd) hipLaunchKernelGGL((kl_pass1<2, DM>), dim3(blocks_for(node_waves(nd, fa.K), t.wpb)),
+                                           blk, 0, s, fa, ns, nd);
+            } else {
+                hipLaunchKernelGGL((kf_pass1<NS, DM>), dim3(gn), blk, 0, s, fa);
+            }
+            break;
         case 2:
             if (t.adfh_epw == 1)
                 hipLaunchKernelGGL((kf_adfh<NS, 1>), dim3(blocks_for(adfh_waves(m.D, fa.K, t), t.wpb)), blk, 0, s, fa);
