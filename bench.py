@@ -228,7 +228,7 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": measured_traffic(args, mesh, K),
-                "kernel": "fct_step (the five launches of one step)",
+                "kernel": "fct_step (all launches of one step)",
                 "bytes_per_step": B}
     cpu = None
     if not args.no_cpu_baseline:
