@@ -592,7 +592,7 @@ def run_compute_w(args, dev):
                    "nodes": mesh.nod2D, "levels": mesh.nl, "interfaces": n_in},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": None, "kernel": "k_compute_w", "bytes_per_call": B},
+                     "traffic": None, "kernel": "kf_vflux + k_compute_w_vf", "bytes_per_call": B},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

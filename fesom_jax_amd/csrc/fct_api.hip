@@ -280,6 +280,11 @@ int apply_tuning(FusedTuning& t, int knob, int value) {
             if (value < 1 || value > 64) return set_error(FCT_E_INVALID, "xcd %d out of range", value);
             t.xcd = value;
             return FCT_OK;
+        case FCT_TUNE_W_EDGE_FLUX:
+            if (value != 0 && value != 1)
+                return set_error(FCT_E_INVALID, "compute_w form %d not in {0, 1}", value);
+            t.w_vflux = value;
+            return FCT_OK;
         case FCT_TUNE_XCD_CHUNK:
             if (value < 0 || value > (1 << 20))
                 return set_error(FCT_E_INVALID, "xcd chunk %d out of range", value);
@@ -743,7 +748,13 @@ int fct_compute_w(fct_plan* p, const double* uv, const double* helem, double* w,
     if (hipGetDevice(&cur) != hipSuccess || cur != p->device)
         return set_error(FCT_E_INVALID, "current device %d is not the plan's device %d", cur,
                          p->device);
-    return launch_compute_w(p->m, uv, helem, w, (hipStream_t)stream);
+    const hipStream_t s = (hipStream_t)stream;
+    if (p->tune.w_vflux && p->tab.dm && p->m.L <= 128) {
+        // edge terms once per edge into the plan's vflux workspace, then summed per node
+        int rc = launch_vflux_fused(p->m, p->tab, p->ekr, uv, helem, p->tune, p->vflux, s);
+        return rc ? rc : launch_compute_w_vf(p->m, p->tab, p->ekr, p->vflux, w, s);
+    }
+    return launch_compute_w(p->m, uv, helem, w, s);
 }
 
 int fct_impl_vert_diff(fct_plan* p, const fct_vdiff_args* a, void* stream) {

@@ -121,6 +121,7 @@ struct FusedTuning {
     int wpb = 4;        // wavefronts per workgroup (1, 2 or 4)
     int vflux_epw = 4;  // edges per wavefront in kf_vflux (2 or 4)
     int adfh_epw = 2;   // edges per wavefront in kf_adfh (1 or 2)
+    int w_vflux = 1;    // compute_w from the edge volume flux (1) or per node (0)
     // the remap as the kernels take it: XCD count in the low 8 bits, chunk above
     int remap() const { return xcd | (xchunk << 8); }
 };
@@ -175,6 +176,12 @@ int launch_ice_fct(const DevMesh& m, const IceCall& c, double* work, const doubl
                    hipStream_t s);
 
 // Vertical velocity from continuity (fct_ale.hip).
+// the FCT's edge volume flux alone (kf_vflux) into vflux [D][L]; fused node tables only
+int launch_vflux_fused(const DevMesh& m, const NodeTab& tab, const int2* ekr, const double* uv,
+                       const double* helem, const FusedTuning& t, double* vflux, hipStream_t s);
+// compute_w's divergence from that edge flux (exactly minus the per-edge term)
+int launch_compute_w_vf(const DevMesh& m, const NodeTab& tab, const int2* ekr,
+                        const double* vflux, double* w, hipStream_t s);
 int launch_compute_w(const DevMesh& m, const double* uv, const double* helem, double* w,
                      hipStream_t s);
 

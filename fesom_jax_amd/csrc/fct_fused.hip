@@ -1066,6 +1066,28 @@ int launch_fused(const DevMesh& m, const NodeTab& tab, const int2* ekr, const fc
     return set_error(FCT_E_UNSUPPORTED, "fused path supports node degree <= 8 and nl <= 128");
 }
 
+template <int NS>
+static void launch_vflux_t(const FusedArgs& fa, const FusedTuning& t, hipStream_t s) {
+    const dim3 blk(64 * t.wpb);
+    const unsigned g = blocks_for(vflux_waves(fa.m.D, t), t.wpb);
+    if (t.vflux_epw == 2) hipLaunchKernelGGL((kf_vflux<NS, 2>), dim3(g), blk, 0, s, fa);
+    else hipLaunchKernelGGL((kf_vflux<NS, 4>), dim3(g), blk, 0, s, fa);
+}
+
+int launch_vflux_fused(const DevMesh& m, const NodeTab& tab, const int2* ekr, const double* uv,
+                       const double* helem, const FusedTuning& tune, double* vflux,
+                       hipStream_t s) {
+    if (!tuning_ok(tune)) return set_error(FCT_E_INVALID, "bad launch-shape knobs");
+    if (m.L > 128) return set_error(FCT_E_UNSUPPORTED, "fused edge flux supports nl <= 128");
+    if (!m.D) return FCT_OK;
+    FusedArgs fa{};
+    fa.m = m; fa.tab = tab; fa.ekr = ekr; fa.uv = uv; fa.helem = helem; fa.vflux = vflux;
+    fa.remap = tune.remap(); fa.K = 1;
+    if (m.L <= 64) launch_vflux_t<1>(fa, tune, s);
+    else launch_vflux_t<2>(fa, tune, s);
+    return check_launch("kf_vflux");
+}
+
 template <int NS, int DM>
 static int launch_upwind_t(FusedArgs& fa, const fct_step_args* a, const FusedTuning& t,
                            hipStream_t s) {

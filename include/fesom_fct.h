@@ -115,8 +115,11 @@ enum {
     FCT_TUNE_WAVES_PER_GROUP = 2,   /* wavefronts per workgroup: 1, 2 or 4 (default 4)     */
     FCT_TUNE_VFLUX_EDGES = 3,       /* edges per wavefront in the vflux pass: 2 or 4 (4)   */
     FCT_TUNE_ADFH_EDGES = 4,        /* edges per wavefront in the adf_h pass: 1 or 2 (2)   */
-    FCT_TUNE_XCD_CHUNK = 5          /* workgroups per range of one XCD under FCT_TUNE_XCD:
+    FCT_TUNE_XCD_CHUNK = 5,         /* workgroups per range of one XCD under FCT_TUNE_XCD:
                                        0 = one range per XCD, else 1..2^20 (default 0)    */
+    FCT_TUNE_W_EDGE_FLUX = 6        /* fct_compute_w: 1 = edge terms once per edge (the
+                                       step's edge flux pass), 0 = per node from the
+                                       triangles; same bits either way (default 1)        */
 };
 
 #define FCT_BATCH_SLOTS 4
@@ -204,7 +207,9 @@ int fct_ice_fct_solve(fct_plan* plan, const fct_ice_args* args, void* stream);
  * w = compute_w(mesh, uv, helem) (fesom_jax/ale.py:309-365), the w_e the step takes:
  * per-level edge divergence into the nodes, cumulative sum from the bottom, divided by
  * the interface area; zero outside each node's interfaces [nzmin, nzmax]. uv is
- * [elem2D][nl][2], helem [elem2D][nl], w [nod2D][nl] (device). nl <= 128. */
+ * [elem2D][nl][2], helem [elem2D][nl], w [nod2D][nl] (device). nl <= 128. Uses the
+ * plan's edge-flux workspace, as fct_forward does: calls on one plan are ordered on one
+ * stream (they race if they overlap). No allocation. */
 int fct_compute_w(fct_plan* plan, const double* uv, const double* helem, double* w,
                   void* stream);
 

@@ -15,11 +15,16 @@ def _t(dev):
     return lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
 
 
+@pytest.mark.parametrize("form", [1, 0], ids=["edge_flux", "per_node"])
 @pytest.mark.parametrize("which", ["pi", "icosa4_nl70", "icosa7"])
-def test_compute_w_bitwise(which, pi_mesh, pi_case, torch_dev):
+def test_compute_w_bitwise(which, form, pi_mesh, pi_case, torch_dev):
+    """Both forms of the divergence (FCT_TUNE_W_EDGE_FLUX: the edge terms once per edge
+    from the step's edge flux, or per node from the triangles) against the oracle."""
+    from fesom_jax_amd import _lib
     from fesom_jax_amd.ale import compute_w
     from fesom_jax_amd.inputs import make_case
     from fesom_jax_amd.meshgen import icosahedral_mesh
+    from fesom_jax_amd.tracer_adv import get_plan
     if which == "pi":
         m, c = pi_mesh, pi_case
     elif which == "icosa4_nl70":
@@ -29,9 +34,14 @@ def test_compute_w_bitwise(which, pi_mesh, pi_case, torch_dev):
         m = icosahedral_mesh(7, 70, seed=71)
         c = make_case(m, seed=72, dt=2700.0)
     t = _t(torch_dev)
-    got = compute_w(m, t(c["uv"]), t(c["helem"])).cpu().numpy()
+    plan = get_plan(m, torch_dev)
+    _lib.check(plan.lib.fct_plan_set_tuning(plan.ptr, _lib.FCT_TUNE_W_EDGE_FLUX, form), "tuning")
+    try:
+        got = compute_w(m, t(c["uv"]), t(c["helem"])).cpu().numpy()
+    finally:
+        plan.lib.fct_plan_set_tuning(plan.ptr, _lib.FCT_TUNE_W_EDGE_FLUX, 1)
     want = O.compute_w(m, c["uv"], c["helem"])
-    print(f"[compute_w] {which}: N={m.nod2D} nl={m.nl} differing="
+    print(f"[compute_w] {which} form {form}: N={m.nod2D} nl={m.nl} differing="
           f"{int(np.sum(got != want))}", flush=True)
     assert np.array_equal(got, want)
 
