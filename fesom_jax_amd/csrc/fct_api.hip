@@ -60,8 +60,9 @@ struct fct_plan {
     int kw = 0;
     FusedTuning tune;
     NodeTab tab;
-    // fpf holds the limiter factors as (plus, minus) pairs; fmf stays null
-    double *fLO = nullptr, *fgnode = nullptr, *fadfh = nullptr, *fpf = nullptr, *fmf = nullptr;
+    // fpf holds the limiter factors as (plus, minus) pairs; fadfv the vertical
+    // antidiffusive flux pass 2 computes and pass 3 reads
+    double *fLO = nullptr, *fgnode = nullptr, *fadfh = nullptr, *fpf = nullptr, *fadfv = nullptr;
     // adjoint workspace (allocated on the first fct_backward)
     bool adj_ready = false;
     AdjointWork aw{};
@@ -594,7 +595,8 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
     }
     if ((rc = dev_alloc(p, &p->fLO, NL * p->kw)) || (rc = dev_alloc(p, &p->fgnode, 2 * NL * p->kw)) ||
         (rc = dev_alloc(p, &p->fadfh, DL * p->kw)) ||
-        (rc = dev_alloc(p, &p->fpf, 2 * NL * p->kw))) {
+        (rc = dev_alloc(p, &p->fpf, 2 * NL * p->kw)) ||
+        (rc = dev_alloc(p, &p->fadfv, NL * p->kw))) {
         free_plan(p);
         return rc;
     }
@@ -688,7 +690,7 @@ int fct_forward_ex(fct_plan* p, const fct_step_args* a, int32_t path, void* stre
     if (path != FCT_PATH_AUTO && path != FCT_PATH_FUSED)
         return set_error(FCT_E_INVALID, "unknown path %d", path);
     return launch_fused(p->m, p->tab, p->ekr, a, p->kw, p->tune, p->vflux, p->fLO, p->fgnode,
-                        p->fadfh, p->fpf, p->fmf, s);
+                        p->fadfh, p->fpf, p->fadfv, s);
 }
 
 int fct_forward(fct_plan* p, const fct_step_args* a, void* stream) {
@@ -926,7 +928,7 @@ int build_slot(fct_batch* b, BatchSlot& s, const fct_step_args* args) {
             const int inst = g.count++;
             desc.resize(desc.size() + dsz);
             int rc = fill_fused_args(desc.data() + inst * dsz, p->m, p->tab, p->ekr, &args[i], t,
-                                     p->vflux, p->fLO, p->fgnode, p->fadfh, p->fpf, p->fmf);
+                                     p->vflux, p->fLO, p->fgnode, p->fadfh, p->fpf, p->fadfv);
             if (rc) return rc;
             block_table(bv, inst, vflux_waves(p->m.D, t), t.wpb);
             block_table(bn, inst, node_waves(p->m.N, args[i].ntracers), t.wpb);

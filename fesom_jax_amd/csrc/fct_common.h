@@ -143,7 +143,7 @@ struct BatchGroup {
 // Fused launcher (fct_fused.hip): kw = tracers per chunk (workspace slots).
 int launch_fused(const DevMesh& m, const NodeTab& tab, const int2* ekr, const fct_step_args* a,
                  int kw, const FusedTuning& tune, double* vflux, double* LO, double* gnode,
-                 double* adfh, double* pf, double* mf, hipStream_t s);
+                 double* adfh, double* pf, double* adfv, hipStream_t s);
 
 // Upwind-only step (fct_fused.hip).
 int launch_upwind(const DevMesh& m, const NodeTab& tab, const int2* ekr, const fct_step_args* a,
@@ -153,7 +153,7 @@ int launch_upwind(const DevMesh& m, const NodeTab& tab, const int2* ekr, const f
 size_t fused_args_size();
 int fill_fused_args(void* out, const DevMesh& m, const NodeTab& tab, const int2* ekr,
                     const fct_step_args* a, const FusedTuning& tune, double* vflux,
-                    double* LO, double* gnode, double* adfh, double* pf, double* mf);
+                    double* LO, double* gnode, double* adfh, double* pf, double* adfv);
 int launch_batch_group(const BatchGroup& g, const FusedTuning& tune, hipStream_t s);
 // tasks (wavefronts) of instance with D edges, N nodes, K tracers for each pass kind
 inline int64_t vflux_waves(int D, const FusedTuning& t) { return (D + t.vflux_epw - 1) / t.vflux_epw; }

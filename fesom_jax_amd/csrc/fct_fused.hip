@@ -137,7 +137,7 @@ struct FusedArgs {
     double2* gnode;        // [K][N][L]
     double* adfh;          // [K][D][L]
     double* pf;            // [K][N][L]
-    double* mf;            // [K][N][L]
+    double* adfv;          // [K][N][L] adf_v at each node interface, from pass 2 for pass 3
 };
 
 // Wavefront -> task index. Single-instance launches number the wavefronts of the
@@ -655,6 +655,13 @@ __device__ __forceinline__ void pass2_body(const FusedArgs& a, int n, int j, int
     double fvc[NS], fvb[NS];
     adfv_column<NS>(a, n, nr, Tc, Ac, fvc);
     col_next<NS>(fvc, fvb, 0.0);
+    // for pass 3, which reads it instead of computing it again
+    const auto Fv = G(a.adfv + jo);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int k = 64 * s + lane_id();
+        if (k <= nr.y && k < L) Fv[b0 + k] = fvc[s];
+    }
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         const int k = 64 * s + lane_id();
@@ -697,7 +704,6 @@ __device__ __forceinline__ void pass3_body(const FusedArgs& a, int n, int j, int
     const DevMesh& m = a.m;
     const int L = m.L;
     const auto T = G(a.T[j]);
-    const auto To = G(a.Told[j]);
     const size_t jo = (size_t)j * m.N * L;
     const double2* pm = reinterpret_cast<const double2*>(a.pf) + jo;   // (plus, minus) pairs
     const auto adfh = G(a.adfh + (size_t)j * m.D * L);
@@ -706,16 +712,18 @@ __device__ __forceinline__ void pass3_body(const FusedArgs& a, int n, int j, int
     int4 es[DM];
 #pragma unroll
     for (int q = 0; q < DM; ++q) es[q] = ldc(a.tab.edge, (size_t)n * DM + q);
-    double Tc[NS], Ac[NS], Pc[NS], Mc[NS], acc[NS];
+    // adf_v at the node's interfaces (k <= nzmax) as pass 2 computed it
+    const auto Fv = G(a.adfv + jo);
+    double Tc[NS], fvc[NS], Pc[NS], Mc[NS], acc[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-        Tc[s] = 0.0; Ac[s] = 0.0; Pc[s] = 1.0; Mc[s] = 1.0; acc[s] = 0.0;
+        Tc[s] = 0.0; fvc[s] = 0.0; Pc[s] = 1.0; Mc[s] = 1.0; acc[s] = 0.0;
         if (64 * s >= kneed) continue;
         const int k = 64 * s + lane_id();
         const bool wet = in_layer(nr, k);
         if (k < kneed) {
             Tc[s] = T[b0 + k];
-            Ac[s] = ab2(To[b0 + k], Tc[s]);
+            fvc[s] = Fv[b0 + k];
         }
         double fh[DM] = {}, po[DM] = {}, mo[DM] = {};
         if (wet) {
@@ -743,8 +751,6 @@ __device__ __forceinline__ void pass3_body(const FusedArgs& a, int n, int j, int
             acc[s] = acc[s] + (use ? (side ? -fl : fl) : 0.0);
         }
     }
-    double fvc[NS];
-    adfv_column<NS>(a, n, nr, Tc, Ac, fvc);
     double Pm[NS], Mm[NS];
     col_prev<NS>(Pc, Pm, col_bcast0<NS>(Pc));
     col_prev<NS>(Mc, Mm, col_bcast0<NS>(Mc));
@@ -1045,19 +1051,19 @@ static bool tuning_ok(const FusedTuning& t) {
 
 static void base_args(FusedArgs& fa, const DevMesh& m, const NodeTab& tab, const int2* ekr,
                       const fct_step_args* a, const FusedTuning& tune, double* vflux, double* LO,
-                      double* gnode, double* adfh, double* pf, double* mf) {
+                      double* gnode, double* adfh, double* pf, double* adfv) {
     fa.m = m; fa.tab = tab; fa.vflux = vflux; fa.ekr = ekr; fa.uv = a->uv; fa.helem = a->helem;
     fa.w_e = a->w_e; fa.hnode = a->hnode; fa.hnn = a->hnode_new; fa.Z3d = a->Z3d;
     fa.zbar3 = a->zbar3; fa.dt = a->dt; fa.remap = tune.remap(); fa.K = 0;
-    fa.LO = LO; fa.gnode = (double2*)gnode; fa.adfh = adfh; fa.pf = pf; fa.mf = mf;
+    fa.LO = LO; fa.gnode = (double2*)gnode; fa.adfh = adfh; fa.pf = pf; fa.adfv = adfv;
 }
 
 int launch_fused(const DevMesh& m, const NodeTab& tab, const int2* ekr, const fct_step_args* a,
                  int kw, const FusedTuning& tune, double* vflux, double* LO, double* gnode,
-                 double* adfh, double* pf, double* mf, hipStream_t s) {
+                 double* adfh, double* pf, double* adfv, hipStream_t s) {
     if (!tuning_ok(tune)) return set_error(FCT_E_INVALID, "bad launch-shape knobs");
     FusedArgs fa;
-    base_args(fa, m, tab, ekr, a, tune, vflux, LO, gnode, adfh, pf, mf);
+    base_args(fa, m, tab, ekr, a, tune, vflux, LO, gnode, adfh, pf, adfv);
     if (kw > FCT_KCHUNK) kw = FCT_KCHUNK;
     if (tab.dm == 6 && m.L <= 64) return launch_fused_t<1, 6>(fa, a, kw, tune, s);
     if (tab.dm == 6 && m.L <= 128) return launch_fused_t<2, 6>(fa, a, kw, tune, s);
@@ -1126,11 +1132,11 @@ size_t fused_args_size() { return sizeof(FusedArgs); }
 
 int fill_fused_args(void* out, const DevMesh& m, const NodeTab& tab, const int2* ekr,
                     const fct_step_args* a, const FusedTuning& tune, double* vflux, double* LO,
-                    double* gnode, double* adfh, double* pf, double* mf) {
+                    double* gnode, double* adfh, double* pf, double* adfv) {
     if (a->ntracers > FCT_KCHUNK)
         return set_error(FCT_E_UNSUPPORTED, "batched instances take at most %d tracers", FCT_KCHUNK);
     FusedArgs fa;
-    base_args(fa, m, tab, ekr, a, tune, vflux, LO, gnode, adfh, pf, mf);
+    base_args(fa, m, tab, ekr, a, tune, vflux, LO, gnode, adfh, pf, adfv);
     fa.K = a->ntracers;
     for (int j = 0; j < FCT_KCHUNK; ++j) {
         const int jj = j < fa.K ? j : 0;

@@ -78,7 +78,7 @@ def estimate_bytes(mesh, K, nsteps, adjoint):
     N, E, D, L = mesh.nod2D, mesh.elem2D, mesh.edge2D, mesh.nl
     NL, EL, DL = N * L, E * L, D * L
     topo = 200 * N + 200 * E + 150 * D
-    work = 8 * (DL + K * (5 * NL + DL))
+    work = 8 * (DL + K * (6 * NL + DL))
     fields = 8 * (3 * EL + 3 * NL + 3 * K * NL)
     total = topo + work + fields
     if adjoint:
