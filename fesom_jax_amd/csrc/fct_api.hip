@@ -921,7 +921,8 @@ int build_slot(fct_batch* b, BatchSlot& s, const fct_step_args* args) {
         g.ns = keys[c].first;
         g.dm = keys[c].second;
         std::vector<unsigned char> desc;
-        std::vector<int2> bv, bn, ba;
+        std::vector<int2> bv, bn, ba, bns, bnd;
+        const bool split = g.ns == 2;     // every instance of such a group has node lists
         for (int i = 0; i < n; ++i) {
             if (cls[i] != (int)c) continue;
             fct_plan* p = b->plans[i];
@@ -933,6 +934,12 @@ int build_slot(fct_batch* b, BatchSlot& s, const fct_step_args* args) {
             block_table(bv, inst, vflux_waves(p->m.D, t), t.wpb);
             block_table(bn, inst, node_waves(p->m.N, args[i].ntracers), t.wpb);
             block_table(ba, inst, adfh_waves(p->m.D, args[i].ntracers, t), t.wpb);
+            if (split) {
+                if (!p->tab.nlist) return set_error(FCT_E_UNSUPPORTED, "batch: plan without node lists");
+                block_table(bns, inst, node_waves(p->tab.nshallow, args[i].ntracers), t.wpb);
+                block_table(bnd, inst, node_waves(p->m.N - p->tab.nshallow, args[i].ntracers),
+                            t.wpb);
+            }
         }
         std::vector<unsigned char> dcopy(desc);
         const unsigned char* dd = nullptr;
@@ -940,10 +947,13 @@ int build_slot(fct_batch* b, BatchSlot& s, const fct_step_args* args) {
         if ((rc = slot_upload(s, b->capture, &dd, dcopy)) ||
             (rc = slot_upload(s, b->capture, &g.blk_v, bv)) ||
             (rc = slot_upload(s, b->capture, &g.blk_n, bn)) ||
-            (rc = slot_upload(s, b->capture, &g.blk_a, ba)))
+            (rc = slot_upload(s, b->capture, &g.blk_a, ba)) ||
+            (split && ((rc = slot_upload(s, b->capture, &g.blk_ns, bns)) ||
+                       (rc = slot_upload(s, b->capture, &g.blk_nd, bnd)))))
             return rc;
         g.desc = reinterpret_cast<const FusedArgs*>(dd);
         g.nb_v = (unsigned)bv.size(); g.nb_n = (unsigned)bn.size(); g.nb_a = (unsigned)ba.size();
+        g.nb_ns = (unsigned)bns.size(); g.nb_nd = (unsigned)bnd.size();
         s.groups.push_back(g);
     }
     s.bound = true;

@@ -137,7 +137,11 @@ struct BatchGroup {
     const int2* blk_v = nullptr;   // per workgroup: (instance, first wave) of the vflux pass
     const int2* blk_n = nullptr;   // ... of the node passes
     const int2* blk_a = nullptr;   // ... of the adfh pass
-    unsigned nb_v = 0, nb_n = 0, nb_a = 0;
+    // nl > 64: node passes split by column depth, waves over each instance's shallow
+    // and deep node lists (NodeTab::nlist)
+    const int2* blk_ns = nullptr;
+    const int2* blk_nd = nullptr;
+    unsigned nb_v = 0, nb_n = 0, nb_a = 0, nb_ns = 0, nb_nd = 0;
 };
 
 // Fused launcher (fct_fused.hip): kw = tracers per chunk (workspace slots).

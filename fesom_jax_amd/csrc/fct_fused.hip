@@ -954,6 +954,32 @@ FCT_LIST_KERNELS(pass2, 7, 4)
 FCT_LIST_KERNELS(pass3, 8, 6)
 #undef FCT_LIST_KERNELS
 
+// batched form: the block table maps each workgroup to (instance, first wavefront of the
+// instance's shallow (deep = 0) or deep (deep = 1) node list)
+#define FCT_BATCH_LIST_KERNEL(PASS, OCC1, OCC2, CALL)                                      \
+    template <int NSB, int DM>                                                            \
+    __global__ void FCT_OCC(DM == 6 ? (NSB == 1 ? OCC1 : OCC2) : 4)                        \
+        kbl_##PASS(const FusedArgs* __restrict__ d, const int2* __restrict__ blk, int remap, \
+                   int deep) {                                                            \
+        int inst;                                                                         \
+        const int w = batch_wave(blk, remap, inst);                                       \
+        const FusedArgs& a = d[inst];                                                     \
+        const int ns = a.tab.nshallow;                                                    \
+        const int first = deep ? ns : 0, count = deep ? a.m.N - ns : ns;                  \
+        if (w >= count * a.K) return;                                                     \
+        const int t = w / a.K;                                                            \
+        const int n = ldc(a.tab.nlist, (size_t)first + t);                                \
+        const int4 hd = ldc(a.tab.hdr, n);                                                \
+        CALL;                                                                             \
+    }
+
+FCT_BATCH_LIST_KERNEL(pass1, 5, 5, (pass1_body<NSB, DM>(a, n, w - t * a.K, hd)))
+FCT_BATCH_LIST_KERNEL(pass2, 7, 4, (pass2_body<NSB, DM>(a, n, w - t * a.K, make_int2(hd.x, hd.y),
+                                                        make_int2(hd.z & 0xff, hd.w))))
+FCT_BATCH_LIST_KERNEL(pass3, 8, 6, (pass3_body<NSB, DM>(a, n, w - t * a.K, make_int2(hd.x, hd.y),
+                                                        make_int2(hd.z & 0xff, hd.w))))
+#undef FCT_BATCH_LIST_KERNEL
+
 template <int NSB, int DM>
 __global__ void FCT_OCC(DM == 6 ? 5 : 4) kl_pass1(FusedArgs a, int first,
                                                                    int count) {
@@ -1155,14 +1181,25 @@ static void launch_batch_t(const BatchGroup& g, const FusedTuning& t, hipStream_
         else
             hipLaunchKernelGGL((kb_vflux<NS, 4>), dim3(g.nb_v), blk, 0, s, g.desc, g.blk_v, t.remap());
     }
-    if (g.nb_n) hipLaunchKernelGGL((kb_pass1<NS, DM>), dim3(g.nb_n), blk, 0, s, g.desc, g.blk_n, t.remap());
+    const bool split = NS == 2 && g.blk_ns;
+    if (split) {
+        if (g.nb_ns) hipLaunchKernelGGL((kbl_pass1<1, DM>), dim3(g.nb_ns), blk, 0, s, g.desc, g.blk_ns, t.remap(), 0);
+        if (g.nb_nd) hipLaunchKernelGGL((kbl_pass1<2, DM>), dim3(g.nb_nd), blk, 0, s, g.desc, g.blk_nd, t.remap(), 1);
+    } else if (g.nb_n) {
+        hipLaunchKernelGGL((kb_pass1<NS, DM>), dim3(g.nb_n), blk, 0, s, g.desc, g.blk_n, t.remap());
+    }
     if (g.nb_a) {
         if (t.adfh_epw == 1)
             hipLaunchKernelGGL((kb_adfh<NS, 1>), dim3(g.nb_a), blk, 0, s, g.desc, g.blk_a, t.remap());
         else
             hipLaunchKernelGGL((kb_adfh<NS, 2>), dim3(g.nb_a), blk, 0, s, g.desc, g.blk_a, t.remap());
     }
-    if (g.nb_n) {
+    if (split) {
+        if (g.nb_ns) hipLaunchKernelGGL((kbl_pass2<1, DM>), dim3(g.nb_ns), blk, 0, s, g.desc, g.blk_ns, t.remap(), 0);
+        if (g.nb_nd) hipLaunchKernelGGL((kbl_pass2<2, DM>), dim3(g.nb_nd), blk, 0, s, g.desc, g.blk_nd, t.remap(), 1);
+        if (g.nb_ns) hipLaunchKernelGGL((kbl_pass3<1, DM>), dim3(g.nb_ns), blk, 0, s, g.desc, g.blk_ns, t.remap(), 0);
+        if (g.nb_nd) hipLaunchKernelGGL((kbl_pass3<2, DM>), dim3(g.nb_nd), blk, 0, s, g.desc, g.blk_nd, t.remap(), 1);
+    } else if (g.nb_n) {
         hipLaunchKernelGGL((kb_pass2<NS, DM>), dim3(g.nb_n), blk, 0, s, g.desc, g.blk_n, t.remap());
         hipLaunchKernelGGL((kb_pass3<NS, DM>), dim3(g.nb_n), blk, 0, s, g.desc, g.blk_n, t.remap());
     }
