@@ -3,22 +3,27 @@
 // Mapping: one 64-lane wavefront per task (a node or an edge, times a tracer); lane l of
 // slot s holds level k = 64*s + l of the task's column, so every load of a node or edge
 // column is one coalesced row segment, and vertical neighbours move between lanes with
-// DPP wavefront shifts. A node whose column fits in 64 lanes takes the one-slot body
-// even when nl > 64 (a wave-uniform branch), so deep columns alone pay for the second
-// slot. Task indices are wave-uniform (scalar registers); each node reads its stencil
-// (incident edges in the reference's summation order with the layer ranges of their
-// triangles, incident triangles with their vertices) from a per-node table with a few
-// scalar loads, and then issues the loads of all its neighbours' columns back to back.
-// Only wet lanes are computed; T_new is written on every lane (dry lanes: T + 0, as the
-// reference's ale_reconstruct does).
+// DPP wavefront shifts. For nl > 64 the node passes run as two launches over a node list
+// (kl_*): columns whose wet layers fit 64 lanes in a one-slot kernel, which needs fewer
+// registers, the others in the two-slot kernel. Task indices are wave-uniform (scalar
+// registers); each node reads its stencil (incident edges in the reference's summation
+// order with the layer ranges of their triangles, incident triangles with their
+// vertices) from a per-node table with scalar loads, and then issues the loads of all
+// its neighbours' columns back to back. Only wet lanes are computed; T_new is written
+// on every lane (dry lanes: T + 0, as the reference's ale_reconstruct does). The kernels
+// are bound by the vector memory path (address and data units, L1), so the design
+// removes 64-lane loads wherever it can: tables through scalar loads, k-1/k+1 values by
+// lane shifts, the node gradient only where an edge reads it, adf_v handed from pass 2
+// to pass 3.
 //
 //   kf_vflux  (edge) : vflux on the edge's wet levels (edge_vflux, tracer_adv.py:69-93)
-//   kf_pass1  (node) : LO and the node-averaged gradient gnode
+//   kf_pass1  (node) : LO and the node-averaged gradient gnode (kl_pass1 over a node list)
 //                      (compute_fct_lo :298-313, tracer_gradient_elements :316-326,
 //                       _node_avg_grad :329-341)
 //   kf_adfh   (edge) : adf_h = MFCT high order - upwind low order, once per edge-level
 //                      (:96-105, :196-225, fill_up_dn_grad :344-368)
-//   kf_pass2  (node) : cluster bounds, P+/P-, limiter factors (zalesak_limit :389-451)
+//   kf_pass2  (node) : cluster bounds, P+/P-, limiter factors (zalesak_limit :389-451),
+//                      and adf_v (adv_flux_ver_ho :241-295) stored for pass 3
 //   kf_pass3  (node) : limited fluxes, flux2dtracer_fct, ale_reconstruct
 //                      (:453-466, :469-484, :142-151)
 //
