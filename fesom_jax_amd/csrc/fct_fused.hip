@@ -301,7 +301,7 @@ __device__ __forceinline__ void pass1_body(const FusedArgs& a, int n, int j, int
         }
         // all neighbour columns and edge fluxes at once (one basic block: slot indices
         // past the node's count repeat slot 0 and are ignored when summing)
-        double To[DM] = {}, vf[DM] = {}, tv[DM][3] = {};
+        double To[DM] = {}, vf[DM] = {};
         // the node's own fields at k (and k+1 for the lower interface), with the gathers
         double w0 = 0.0, a0 = 0.0, w1 = 0.0, a1 = 0.0, av = 0.0, hn = 0.0, hnd = 0.0;
         const size_t i = b0 + k, ip = b0 + (k + 1 < L ? k + 1 : k);
@@ -314,20 +314,14 @@ __device__ __forceinline__ void pass1_body(const FusedArgs& a, int n, int j, int
             w0 = G(a.w_e)[i]; a0 = G(m.area)[i];
             if (NS > 1) { w1 = G(a.w_e)[ip]; a1 = G(m.area)[ip]; }
             av = G(m.areasvol)[i]; hn = G(a.hnn)[i]; hnd = G(a.hnode)[i];
-            // the element vertices are the node and edge neighbours: read them back
-            // from the lane's own LDS row by slot code (bits 16.. of the element-slot
-            // word: 4 bits per vertex, DM = the node) instead of loading them again
+            // the element vertices are the node and edge neighbours: the gradient below
+            // reads them back from the lane's own LDS row by slot code (bits 16.. of the
+            // element-slot word: 4 bits per vertex, DM = the node) instead of loading
+            // them again
             if (gslot) {
 #pragma unroll
                 for (int q = 0; q < DM; ++q) nb[q * 64] = To[q];
                 nb[DM * 64] = Tk;
-#pragma unroll
-                for (int q = 0; q < DM; ++q) {
-                    const unsigned codes = (unsigned)es[q].w >> 16;
-                    tv[q][0] = nb[(codes & 0xf) * 64];
-                    tv[q][1] = nb[((codes >> 4) & 0xf) * 64];
-                    tv[q][2] = nb[((codes >> 8) & 0xf) * 64];
-                }
             }
         }
         if (NS == 1) {
@@ -375,10 +369,11 @@ __device__ __forceinline__ void pass1_body(const FusedArgs& a, int n, int j, int
             const int e = el[q].x;
             const bool use = q < cnt.y && in_bytes(es[q].w, 0, k);
             const size_t g6 = 6 * (size_t)e;
-            const double gx = ldc(m.gs, g6) * tv[q][0] + ldc(m.gs, g6 + 1) * tv[q][1] +
-                              ldc(m.gs, g6 + 2) * tv[q][2];
-            const double gy = ldc(m.gs, g6 + 3) * tv[q][0] + ldc(m.gs, g6 + 4) * tv[q][1] +
-                              ldc(m.gs, g6 + 5) * tv[q][2];
+            const unsigned codes = (unsigned)es[q].w >> 16;
+            const double tv0 = nb[(codes & 0xf) * 64], tv1 = nb[((codes >> 4) & 0xf) * 64];
+            const double tv2 = nb[((codes >> 8) & 0xf) * 64];
+            const double gx = ldc(m.gs, g6) * tv0 + ldc(m.gs, g6 + 1) * tv1 + ldc(m.gs, g6 + 2) * tv2;
+            const double gy = ldc(m.gs, g6 + 3) * tv0 + ldc(m.gs, g6 + 4) * tv1 + ldc(m.gs, g6 + 5) * tv2;
             const double ea = ldc(m.earea, e);
             nx = nx + (use ? gx * ea : 0.0);
             ny = ny + (use ? gy * ea : 0.0);
@@ -986,7 +981,7 @@ FCT_BATCH_LIST_KERNEL(pass3, 8, 6, (pass3_body<NSB, DM>(a, n, w - t * a.K, make_
 #undef FCT_BATCH_LIST_KERNEL
 
 template <int NSB, int DM>
-__global__ void FCT_OCC(DM == 6 ? 5 : 4) kl_pass1(FusedArgs a, int first,
+__global__ void FCT_OCC(DM == 6 ? 8 : 4) kl_pass1(FusedArgs a, int first,
                                                                    int count) {
     int w;
     if (!single_wave((int64_t)count * a.K, a.remap, w)) return;
