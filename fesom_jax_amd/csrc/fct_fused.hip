@@ -435,13 +435,12 @@ __device__ __forceinline__ void adfh_group(const FusedArgs& a, int g, int j) {
         const int k = 64 * s + lane_id();
         double vf[EPW] = {}, T1[EPW] = {}, T2[EPW] = {}, O1[EPW] = {}, O2[EPW] = {};
         double tu[EPW][3] = {}, td[EPW][3] = {};
-        double2 ga[EPW], gb[EPW];
         // per edge: does any wet level of this slot use the up/dn triangle gradients
-        // (nt), does any use the node gradients gnode (ng)
+        // (nt), does any use the node gradients gnode (ng; few edges, so those loads
+        // wait until the edge is computed and take no registers before)
         bool nt[EPW], ng[EPW];
 #pragma unroll
         for (int q = 0; q < EPW; ++q) {
-            ga[q] = make_double2(0.0, 0.0); gb[q] = ga[q];
             const int lo = kr[q].x > 64 * s ? kr[q].x : 64 * s;
             const int hi = kr[q].y < 64 * s + 64 ? kr[q].y : 64 * s + 64;
             const int tl = lo > ur[q].x ? lo : ur[q].x, th = hi < ur[q].y ? hi : ur[q].y;
@@ -454,7 +453,6 @@ __device__ __forceinline__ void adfh_group(const FusedArgs& a, int g, int j) {
                 const size_t i1 = (size_t)nd[q].x * L + k, i2 = (size_t)nd[q].y * L + k;
                 vf[q] = G(a.vflux)[(size_t)e[q] * L + k];
                 T1[q] = T[i1]; T2[q] = T[i2]; O1[q] = To[i1]; O2[q] = To[i2];
-                if (ng[q]) { ga[q] = ldv(gn, i1); gb[q] = ldv(gn, i2); }
                 if (nt[q]) {
                     // the up triangle's vertex at slot cu is the edge's first node, the down
                     // triangle's at slot cd its second: their T is already loaded
@@ -479,8 +477,13 @@ __device__ __forceinline__ void adfh_group(const FusedArgs& a, int g, int j) {
                 gu = grad_of(m.gs, ud[q].x, tu[q][0], tu[q][1], tu[q][2]);
                 gd = grad_of(m.gs, ud[q].y, td[q][0], td[q][1], td[q][2]);
             }
-            const double g0 = tri ? gu.x : ga[q].x, g1 = tri ? gd.x : gb[q].x;
-            const double g2 = tri ? gu.y : ga[q].y, g3 = tri ? gd.y : gb[q].y;
+            double2 ga = make_double2(0.0, 0.0), gb = ga;
+            if (ng[q]) {
+                ga = ldv(gn, (size_t)nd[q].x * L + k);
+                gb = ldv(gn, (size_t)nd[q].y * L + k);
+            }
+            const double g0 = tri ? gu.x : ga.x, g1 = tri ? gd.x : gb.x;
+            const double g2 = tri ? gu.y : ga.y, g3 = tri ? gd.y : gb.y;
             const double v = vf[q];
             const double av = fabs(v);
             const double flo = -0.5 * (T1[q] * (v + av) + T2[q] * (v - av));
@@ -902,14 +905,14 @@ __global__ void FCT_OCC(8) kb_vflux(const FusedArgs* __restrict__ d,
 }
 
 template <int NS, int EPW>
-__global__ void FCT_OCC(6) kf_adfh(FusedArgs a) {
+__global__ void FCT_OCC(8) kf_adfh(FusedArgs a) {
     int w;
     if (!single_wave((int64_t)((a.m.D + EPW - 1) / EPW) * a.K, a.remap, w)) return;
     const int g = w / a.K;
     adfh_group<NS, EPW>(a, g, w - g * a.K);
 }
 template <int NS, int EPW>
-__global__ void FCT_OCC(6) kb_adfh(const FusedArgs* __restrict__ d,
+__global__ void FCT_OCC(8) kb_adfh(const FusedArgs* __restrict__ d,
                                                const int2* __restrict__ blk, int remap) {
     int inst;
     const int w = batch_wave(blk, remap, inst);
