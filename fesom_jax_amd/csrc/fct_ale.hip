@@ -29,6 +29,10 @@ template <class T>
 using gp = __attribute__((address_space(1))) T*;
 template <class T>
 __device__ __forceinline__ gp<T> glb(T* p) { return (gp<T>)p; }
+// wave-uniform loads of mesh tables through the constant address space (scalar loads)
+__device__ __forceinline__ int ldc(const int* p, size_t i) {
+    return ((__attribute__((address_space(4))) const int*)p)[i];
+}
 
 // stage 2 and 3 of both kernels for node n (valid: n < nod2D), whose divergence is div.
 // Blocks are FCT_W_WPB wavefronts, one node each; every thread reaches the barriers.
@@ -56,6 +60,7 @@ __device__ __forceinline__ void finish_w(const DevMesh& m, int n, bool valid, in
         if (nn < m.N) {
             const int zmin = nlev[2 * nn];
             double acc = 0.0;
+#pragma unroll 8
             for (int k = L - 1; k >= zmin; --k) {
                 const double d = s_col[c][k];
                 acc = k == L - 1 ? d : acc + d;
@@ -89,20 +94,21 @@ __global__ void __launch_bounds__(256) k_compute_w_vf(DevMesh m, NodeTab tab,
                                                       const int2* ekr_, const double* vflux_,
                                                       double* w_) {
     const int lane = threadIdx.x & 63;
-    const int n0 = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    // the node index is wave-uniform: scalar registers, so the table loads are scalar
+    const int n0 = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     const bool valid = n0 < m.N;
     const int n = valid ? n0 : m.N - 1;
     const int L = m.L;
     const auto vflux = glb(vflux_);
-    const auto ekr = glb(reinterpret_cast<const int*>(ekr_));
-    const auto ent = glb(reinterpret_cast<const int*>(tab.edge));
-    const int cnt = glb(reinterpret_cast<const int*>(tab.hdr))[4 * n + 2] & 0xff;
+    const int* ekr = reinterpret_cast<const int*>(ekr_);
+    const int* ent = reinterpret_cast<const int*>(tab.edge);
+    const int cnt = ldc(reinterpret_cast<const int*>(tab.hdr), 4 * (size_t)n + 2) & 0xff;
     int code[DM], lo[DM], hi[DM];
 #pragma unroll
     for (int q = 0; q < DM; ++q) {
-        code[q] = ent[4 * ((size_t)n * DM + q)];
-        lo[q] = ekr[2 * (code[q] >> 1)];
-        hi[q] = ekr[2 * (code[q] >> 1) + 1];
+        code[q] = ldc(ent, 4 * ((size_t)n * DM + q));
+        lo[q] = ldc(ekr, 2 * (size_t)(code[q] >> 1));
+        hi[q] = ldc(ekr, 2 * (size_t)(code[q] >> 1) + 1);
     }
     double div[NS];
 #pragma unroll
