@@ -279,6 +279,61 @@ def icosphere(level: int):
     return v[order], rank[f]
 
 
+def flip_to_degree_eight(faces: np.ndarray, count: int, rng) -> np.ndarray:
+    """Raise ``count`` degree-6 nodes of a closed triangulation to degree 8. For a node c,
+    flip the two edges opposite c in two opposite triangles of its ring: each quad
+    (c, u, x, v) with diagonal u-v becomes (c, u, x) + (c, x, v). c gains two
+    neighbours (8), the four ring nodes u, v lose one (5), the two far vertices x gain
+    one (7). Every node involved has degree 6 beforehand, and the nodes of one flip
+    pair and their neighbours are not used again, so the flips do not interact."""
+    f = np.array(faces, dtype=np.int64, copy=True)
+    N = int(f.max()) + 1
+    inc = [[] for _ in range(N)]
+    for t, tri in enumerate(f):
+        for v in tri:
+            inc[v].append(t)
+    used = np.zeros(N, bool)
+
+    def ring(c):
+        return sorted(set(f[inc[c]].reshape(-1).tolist()) - {c})
+
+    done = 0
+    for c in rng.permutation(N):
+        if done == count:
+            break
+        c = int(c)
+        if used[c] or len(inc[c]) != 6:
+            continue
+        r = ring(c)
+        # two triangles of c's ring that share no vertex: opposite sides of the hexagon
+        t0 = inc[c][0]
+        e0 = [v for v in f[t0] if v != c]
+        t3 = next(t for t in inc[c] if not set(f[t].tolist()) & set(e0))
+        plan = []
+        for t in (t0, t3):
+            u, v = [int(x) for x in f[t] if x != c]
+            g = next(g for g in inc[u] if g != t and v in f[g])
+            x = int(next(y for y in f[g] if y != u and y != v))
+            plan.append((t, g, u, v, x))
+        nodes = {c, *r} | {p[4] for p in plan}
+        if any(used[n] or len(inc[n]) != 6 for n in nodes):
+            continue
+        if any(p[4] in r for p in plan):
+            continue
+        for t, g, u, v, x in plan:
+            f[t] = (c, u, x)
+            f[g] = (c, x, v)
+            inc[v].remove(t); inc[u].remove(g)
+            inc[x].append(t); inc[c].append(g)
+        for n in list(nodes):
+            used[n] = True
+            used[ring(n)] = True
+        done += 1
+    if done != count:
+        raise ValueError(f"could only raise {done} of {count} nodes to degree 8")
+    return f
+
+
 def _real_sph_harm_basis(xyz, lmax):
     """Real spherical harmonics of degree 1..lmax at unit vectors (columns)."""
     from scipy.special import lpmv
@@ -325,10 +380,15 @@ def default_zbar(nl: int) -> np.ndarray:
 
 def icosahedral_mesh(level: int, nl: int = 70, *, seed: int = 0,
                      land_fraction: float = 0.3, polar_cap_deg: float = 75.0,
-                     min_levels: int = 10, zbar: np.ndarray | None = None) -> Mesh:
-    """Ocean-only icosahedral mesh bundle (see module docstring)."""
+                     min_levels: int = 10, zbar: np.ndarray | None = None,
+                     degree8: int = 0) -> Mesh:
+    """Ocean-only icosahedral mesh bundle (see module docstring). ``degree8`` > 0 flips
+    edges so that that many nodes have 8 neighbours (``flip_to_degree_eight``), as
+    unstructured ocean meshes have; 0 leaves every node at 5 or 6."""
     rng = np.random.default_rng(seed)
     xyz, faces = icosphere(level)
+    if degree8:
+        faces = flip_to_degree_eight(faces, degree8, np.random.default_rng(seed + 7919))
     rlat = np.arcsin(np.clip(xyz[:, 2], -1.0, 1.0))
     rlon = np.arctan2(xyz[:, 1], xyz[:, 0])
 

@@ -16,7 +16,7 @@ def _t(dev):
 
 
 @pytest.mark.parametrize("form", [1, 0], ids=["edge_flux", "per_node"])
-@pytest.mark.parametrize("which", ["pi", "icosa4_nl70", "icosa7"])
+@pytest.mark.parametrize("which", ["pi", "icosa4_nl70", "icosa4_nl70_deg8", "icosa7"])
 def test_compute_w_bitwise(which, form, pi_mesh, pi_case, torch_dev):
     """Both forms of the divergence (FCT_TUNE_W_EDGE_FLUX: the edge terms once per edge
     from the step's edge flux, or per node from the triangles) against the oracle."""
@@ -30,6 +30,9 @@ def test_compute_w_bitwise(which, form, pi_mesh, pi_case, torch_dev):
     elif which == "icosa4_nl70":
         m = icosahedral_mesh(4, 70, seed=41)
         c = make_case(m, seed=42, dt=2700.0)
+    elif which == "icosa4_nl70_deg8":
+        m = icosahedral_mesh(4, 70, seed=43, degree8=30)     # 8 table slots per node
+        c = make_case(m, seed=44, dt=2700.0)
     else:
         m = icosahedral_mesh(7, 70, seed=71)
         c = make_case(m, seed=72, dt=2700.0)

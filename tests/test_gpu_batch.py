@@ -11,12 +11,12 @@ def _instances(dev, specs, pi_mesh=None, pi_case=None):
     from fesom_jax_amd.inputs import make_case
     from fesom_jax_amd.meshgen import icosahedral_mesh
     out = []
-    for level, nl, K, seed in specs:
+    for level, nl, K, seed, *deg8 in specs:
         if level == "pi":
             mesh, case = pi_mesh, dict(pi_case)
             case["T"], case["T_old"] = case["T"][:K], case["T_old"][:K]
         else:
-            mesh = icosahedral_mesh(level, nl, seed=seed)
+            mesh = icosahedral_mesh(level, nl, seed=seed, degree8=deg8[0] if deg8 else 0)
             case = make_case(mesh, seed=seed + 1, dt=2700.0, tracers=("T", "S", "T", "S")[:K])
             # a distinct T_old so that AB2 matters
             case["T_old"] = [0.999 * t for t in case["T"]]
@@ -24,7 +24,8 @@ def _instances(dev, specs, pi_mesh=None, pi_case=None):
     return out
 
 
-SPECS = [("pi", 48, 2, 0), (3, 48, 1, 5), (3, 70, 3, 7), (4, 70, 2, 9)]
+# the last instance has nodes of 8 neighbours (8 table slots) at nl = 70
+SPECS = [("pi", 48, 2, 0), (3, 48, 1, 5), (3, 70, 3, 7), (4, 70, 2, 9), (3, 70, 2, 11, 12)]
 
 
 def _clone_state(insts):

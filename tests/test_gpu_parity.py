@@ -226,6 +226,27 @@ def test_icosa_nl70_two_slots(torch_dev):
     _check("icosa4 nl70 fused sharp", g, _oracle(m, sh, 0), m)
 
 
+def test_degree_eight_nl70_depth_split(torch_dev):
+    """Nodes with 8 neighbours (8 table slots per node) at nl = 70: the node passes split
+    by column depth (one-slot kernels for the columns that end above level 64) on 8-slot
+    tables, against the oracle, smooth and with the limiter active, and against the
+    phased path bit for bit."""
+    from fesom_jax_amd.inputs import make_case
+    from fesom_jax_amd.meshgen import icosahedral_mesh
+    from fesom_jax_amd.tracer_adv import get_plan
+    m = icosahedral_mesh(4, 70, seed=31, degree8=30)
+    case = make_case(m, seed=32, dt=2700.0)
+    assert np.bincount(m.edges.reshape(-1)).max() == 8
+    assert int(m.nlevels_nod2D.max()) > 65 and int(m.nlevels_nod2D.min()) < 60
+    assert get_plan(m, torch_dev).fused_available
+    for name, c in (("smooth", case), ("sharp", sharp_case(m, case, 2.0))):
+        outs = _gpu_step(m, c, torch_dev, path="fused")
+        ph = _gpu_step(m, c, torch_dev, path="phased")
+        for j, g in enumerate(outs):
+            _check(f"icosa4 nl70 degree 8 {name} [{j}]", g, _oracle(m, c, j), m)
+            assert np.array_equal(g, ph[j]), j
+
+
 @pytest.mark.parametrize("knobs", [{"XCD": 8}, {"WAVES_PER_GROUP": 2, "XCD": 3},
                                    {"WAVES_PER_GROUP": 1, "VFLUX_EDGES": 2, "ADFH_EDGES": 1},
                                    {"XCD": 8, "XCD_CHUNK": 16}, {"XCD": 3, "XCD_CHUNK": 1000}])
@@ -252,7 +273,7 @@ def test_launch_shapes_do_not_change_results(knobs, torch_dev):
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("which", ["pi", "icosa4", "nl70"])
+@pytest.mark.parametrize("which", ["pi", "icosa4", "nl70", "nl70_deg8"])
 def test_upwind_step_bitwise(which, pi_mesh, pi_case, icosa4_mesh, icosa4_case, torch_dev):
     """advect_one on the GPU against the oracle (tracer_adv.py:154-165), bit for bit."""
     import torch
@@ -264,7 +285,7 @@ def test_upwind_step_bitwise(which, pi_mesh, pi_case, icosa4_mesh, icosa4_case, 
     elif which == "icosa4":
         m, c = icosa4_mesh, icosa4_case
     else:
-        m = icosahedral_mesh(4, 70, seed=21)
+        m = icosahedral_mesh(4, 70, seed=21, degree8=30 if which == "nl70_deg8" else 0)
         c = make_case(m, seed=22, dt=2700.0)
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(torch_dev)
     for j in range(len(c["T"])):

@@ -63,11 +63,20 @@ def test_icosphere_counts():
         assert np.allclose(np.linalg.norm(v, axis=1), 1.0)
 
 
-@pytest.mark.parametrize("level,nl", [(3, 48), (4, 70)])
-def test_icosahedral_mesh_conventions(level, nl, tmp_path):
-    m = icosahedral_mesh(level, nl, seed=5)
+@pytest.mark.parametrize("level,nl,deg8", [(3, 48, 0), (4, 70, 0), (3, 70, 12)])
+def test_icosahedral_mesh_conventions(level, nl, deg8, tmp_path):
+    m = icosahedral_mesh(level, nl, seed=5, degree8=deg8)
     n0, _, _ = icosa_counts(level)
     assert 0.6 < m.nod2D / n0 < 0.8                 # ~30% land carved out
+    deg = np.bincount(m.edges.reshape(-1), minlength=m.nod2D)
+    assert deg.max() == (8 if deg8 else 6)
+    key = np.sort(m.edges, axis=1) @ np.array([m.nod2D, 1])
+    assert np.unique(key).size == m.edge2D           # no edge twice
+    # every triangle side is an edge
+    en = m.elem_nodes
+    sides = np.sort(np.concatenate([en[:, [0, 1]], en[:, [1, 2]], en[:, [2, 0]]]), axis=1)
+    assert np.isin(sides @ np.array([m.nod2D, 1]), key).all()
+    assert np.all(m.elem_area > 0)
     check_cw_orientation(m.coord_nod2D, m.elem_nodes)
     assert m.nlevels.min() >= 10 and m.nlevels.max() <= nl
     et = m.edge_tri
