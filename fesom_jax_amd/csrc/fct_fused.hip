@@ -501,9 +501,13 @@ __device__ __forceinline__ void adfh_group(const FusedArgs& a, int g, int j) {
 // vertical antidiffusive flux (adv_flux_ver_ho - adv_flux_ver, tracer_adv.py:241-295,
 // 108-120) at interface k from the column's ttfAB at k-2..k+1 and T at k-1, k
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double adfv_at(int k, int2 nr, double A2, double A1, double A,
-                                          double Ap, double Tm, double Tk, double W, double ar,
-                                          double Znz, double Zm1, double Zm2, double Zp1,
+// The three QR4C gradients of interface k are one quotient at three levels:
+// qc(k) = (ttf[k-1] - ttf[k]) / nz_safe(Z[k-1] - Z[k]), qu(k) = qc(k+1), qd(k) = qc(k-1)
+// (the same operands in the same order, so the same bits): the column divides once per
+// level and takes qu, qd by lane shifts.
+__device__ __forceinline__ double adfv_at(int k, int2 nr, double A1, double A, double Tm,
+                                          double Tk, double W, double ar, double Znz,
+                                          double Zm1, double qc, double qu, double qd,
                                           double zb) {
     double hi = 0.0;
     if (in_iface(nr, k)) {
@@ -514,9 +518,6 @@ __device__ __forceinline__ double adfv_at(int k, int2 nr, double A2, double A1, 
         } else if (k == nr.x + 1 || k == nr.y - 1) {
             hi = -0.5 * (A1 + A) * W * ar;
         } else if (k >= nr.x + 2 && k <= nr.y - 2) {
-            const double qc = (A1 - A) / nz_safe(Zm1 - Znz);
-            const double qu = (A - Ap) / nz_safe(Znz - Zp1);
-            const double qd = (A2 - A1) / nz_safe(Zm2 - Zm1);
             const double Tmean1 = A + (2.0 * qc + qu) * (zb - Znz) / 3.0;
             const double Tmean2 = A1 + (2.0 * qc + qd) * (zb - Zm1) / 3.0;
             hi = -0.5 * (Tmean1 + Tmean2) * W * ar;
@@ -538,13 +539,11 @@ __device__ __forceinline__ void adfv_column(const FusedArgs& a, int n, int2 nr,
     const DevMesh& m = a.m;
     const int L = m.L;
     const size_t b0 = (size_t)n * L;
-    double Tm[NS], A1[NS], A2[NS], Ap[NS];
+    double Tm[NS], A1[NS];
     col_prev<NS>(Tc, Tm, col_bcast0<NS>(Tc));
     col_prev<NS>(Ac, A1, col_bcast0<NS>(Ac));
-    col_prev<NS>(A1, A2, col_bcast0<NS>(Ac));
-    col_next<NS>(Ac, Ap, 0.0);
     // the column's w, area, depth Z at k and interface depth at every lane (clamped
-    // below the last level); the stencil depths at k-1, k-2 and k+1 are lane shifts of Z
+    // below the last level)
     double W[NS], ar[NS], Zc[NS], zb[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -556,22 +555,25 @@ __device__ __forceinline__ void adfv_column(const FusedArgs& a, int n, int2 nr,
         Zc[s] = a.Z3d ? G(a.Z3d)[i] : G(m.Zs)[kc];
         zb[s] = a.Z3d ? G(a.zbar3)[i] : G(m.zbar)[kc];
     }
-    // static: Z[k-1], Z[k-2] clamped to Z[0] and Z[k] below level 2 (_z_stencil,
-    // tracer_adv.py:228-238); Z3d: Z3d[max(k-1, 0)], Z3d[max(k-2, 0)]; both Z[min(k+1,
-    // L-1)] above, which the clamped lanes past the last level supply
+    // The reference's stencil depths (_z_stencil, tracer_adv.py:228-238) clamp Z[k-1] and
+    // Z[k-2] at the top and Z[k+1] at the bottom. qc is needed at levels nr.x+1 .. nr.y-1
+    // only (qd at nr.x+2 reads nr.x+1, qu at nr.y-2 reads nr.y-1): there 1 <= k <= L-2,
+    // no clamp applies, and Z[k-1] is the plain lane shift of Z.
     const double z0 = col_bcast0<NS>(Zc);
-    double Zm1[NS], Zm2r[NS], Zp1[NS];
+    double Zm1[NS], qc[NS], qu[NS], qd[NS];
     col_prev<NS>(Zc, Zm1, z0);
-    col_prev<NS>(Zm1, Zm2r, z0);
-    col_next<NS>(Zc, Zp1, lane_value(Zc[NS - 1], 63));
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+        qc[s] = 64 * s <= nr.y ? (A1[s] - Ac[s]) / nz_safe(Zm1[s] - Zc[s]) : 0.0;
+    col_next<NS>(qc, qu, 0.0);
+    col_prev<NS>(qc, qd, 0.0);
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         const int k = 64 * s + lane_id();
-        const double Zm2 = k < 2 ? (a.Z3d ? z0 : Zc[s]) : Zm2r[s];
         out[s] = 0.0;
         if (k <= nr.y && k < L)
-            out[s] = adfv_at(k, nr, A2[s], A1[s], Ac[s], Ap[s], Tm[s], Tc[s], W[s], ar[s], Zc[s],
-                             Zm1[s], Zm2, Zp1[s], zb[s]);
+            out[s] = adfv_at(k, nr, A1[s], Ac[s], Tm[s], Tc[s], W[s], ar[s], Zc[s], Zm1[s],
+                             qc[s], qu[s], qd[s], zb[s]);
     }
 }
 
@@ -684,10 +686,16 @@ __device__ __forceinline__ void pass2_body(const FusedArgs& a, int n, int j, int
         const double flux_pos = fplus * a.dt / safe_a / safe_hn + FCT_FLUX_EPS;
         const double flux_neg = fminus * a.dt / safe_a / safe_hn - FCT_FLUX_EPS;
         const bool valid = av > 0.0 && hn > 0.0;
+        // fmin(1, x / y) is 1 wherever x / y >= 1 (rounding is monotone and 1 is a
+        // double), which the signs and one comparison decide: only lanes where the
+        // limiter may bind divide, and a wavefront with none skips the divisions
+        const bool bp = valid && !(flux_pos > 0.0 && ttf_max >= flux_pos);
+        const bool bm = valid && !(flux_neg < 0.0 && ttf_min <= flux_neg);
+        double fp = 1.0, fm = 1.0;
+        if (bp) fp = fmin(1.0, ttf_max / flux_pos);
+        if (bm) fm = fmin(1.0, ttf_min / flux_neg);
         // (plus, minus) factor pairs, interleaved: pass 3 reads both with one load
-        stv(reinterpret_cast<double2*>(a.pf), jo + i,
-            make_double2(valid ? fmin(1.0, ttf_max / flux_pos) : 1.0,
-                         valid ? fmin(1.0, ttf_min / flux_neg) : 1.0));
+        stv(reinterpret_cast<double2*>(a.pf), jo + i, make_double2(fp, fm));
     }
 }
 
