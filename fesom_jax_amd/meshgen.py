@@ -397,6 +397,9 @@ def icosahedral_mesh(level: int, nl: int = 70, *, seed: int = 0,
     q = np.quantile(landf[~cap], 1.0 - max(land_fraction - cap.mean(), 0.0) / max(1e-12, (~cap).mean()))
     land = cap | (landf > q)
     ocean_face = ~land[faces].any(axis=1)
+    if not ocean_face.any():
+        raise ValueError(f"no ocean triangle left at level {level}, land fraction "
+                         f"{land_fraction}, polar caps {polar_cap_deg} deg")
     faces = faces[ocean_face]
     used = np.zeros(xyz.shape[0], bool); used[faces.reshape(-1)] = True
     remap = np.full(xyz.shape[0], -1, np.int64); remap[used] = np.arange(used.sum())

@@ -226,6 +226,29 @@ def test_icosa_nl70_two_slots(torch_dev):
     _check("icosa4 nl70 fused sharp", g, _oracle(m, sh, 0), m)
 
 
+@pytest.mark.parametrize("shape", [(1, 0.85, 2), (1, 0.7, 1), (1, 0.7, 0), (2, 0.9, 2)],
+                         ids=["one_triangle", "two_triangles", "six_triangles", "nine_triangles"])
+def test_tiny_meshes(shape, torch_dev):
+    """The smallest meshes: one triangle (three coastal edges, no up/down triangles), two
+    triangles (one interior edge), columns of one to three layers (nl = 4), on the
+    default and the phased path, against the oracle, smooth and with the limiter
+    active."""
+    from fesom_jax_amd.inputs import make_case
+    from fesom_jax_amd.meshgen import icosahedral_mesh
+    from fesom_jax_amd.tracer_adv import get_plan
+    lev, lf, seed = shape
+    m = icosahedral_mesh(lev, 4, seed=seed, land_fraction=lf, min_levels=2,
+                         polar_cap_deg=60.0)
+    c = make_case(m, seed=seed + 50, dt=2700.0)
+    print(f"[parity] tiny mesh N={m.nod2D} E={m.elem2D} D={m.edge2D} "
+          f"fused={get_plan(m, torch_dev).fused_available}", flush=True)
+    for name, case in (("smooth", c), ("sharp", sharp_case(m, c, 2.0))):
+        for path in ("auto", "phased"):
+            outs = _gpu_step(m, case, torch_dev, path=path)
+            for j, g in enumerate(outs):
+                _check(f"tiny N={m.nod2D} {name} {path} [{j}]", g, _oracle(m, case, j), m)
+
+
 def test_degree_eight_nl70_depth_split(torch_dev):
     """Nodes with 8 neighbours (8 table slots per node) at nl = 70: the node passes split
     by column depth (one-slot kernels for the columns that end above level 64) on 8-slot

@@ -55,6 +55,19 @@ def test_orientation_gate_rejects_ccw(pi_mesh):
         check_cw_orientation(pi_mesh.coord_nod2D, en)
 
 
+def test_tiny_and_empty_meshes():
+    """The generator's smallest outputs (one triangle with three coastal edges, two
+    triangles sharing one edge) keep the conventions; an all-land draw is an error."""
+    m = icosahedral_mesh(1, 4, seed=2, land_fraction=0.85, min_levels=2, polar_cap_deg=60.0)
+    assert (m.nod2D, m.elem2D, m.edge2D, m.edge2D_in) == (3, 1, 3, 0)
+    check_cw_orientation(m.coord_nod2D, m.elem_nodes)
+    m = icosahedral_mesh(1, 4, seed=1, land_fraction=0.7, min_levels=2, polar_cap_deg=60.0)
+    assert (m.nod2D, m.elem2D, m.edge2D, m.edge2D_in) == (4, 2, 5, 1)
+    assert np.all(m.edge_tri[:1, 1] >= 0) and np.all(m.edge_tri[1:, 1] < 0)
+    with pytest.raises(ValueError, match="no ocean"):
+        icosahedral_mesh(1, 4, seed=0, land_fraction=0.9, min_levels=2, polar_cap_deg=60.0)
+
+
 def test_icosphere_counts():
     for lev in range(4):
         v, f = icosphere(lev)
