@@ -48,6 +48,7 @@ struct IceArgs {
     double* d1;             // [3][N] (pong)
     double* pp;             // [3][N] ratios
     double* pm;
+    double* inv;            // [N] 1 / surface area (k_ice_rhs), as inv_area computes it
 };
 
 __device__ __forceinline__ double inv_area(const IceArgs& a, int n) {
@@ -113,6 +114,7 @@ __global__ void __launch_bounds__(256) k_ice_rhs(IceArgs a) {
         }
     }
     const double inv = inv_area(a, n);
+    glb(a.inv)[n] = inv;
     const bool nc = node_open(a, n);
     for (int t = 0; t < ICE_K; ++t) {
         const size_t i = (size_t)t * a.N + n;
@@ -152,12 +154,21 @@ __global__ void __launch_bounds__(256) k_ice_hi(IceArgs a, const double* din, do
     }
 }
 
-// antidiffusive flux of element e at its vertex q for one tracer (ice_adv.py:125-130)
+// antidiffusive flux of element e at its vertex q for one tracer (ice_adv.py:125-130):
+// ice_flux = ice_flux_num / 12
+__device__ __forceinline__ double ice_flux_num(const double (&w)[3], int q, double vol,
+                                               double inv_q) {
+    const double s = sum3(w[0], w[1], w[2]) - 3.0 * w[q];
+    return -s * (vol * inv_q);
+}
 __device__ __forceinline__ double ice_flux(double g, const double (&w)[3], int q, double vol,
                                            double inv_q) {
-    const double s = sum3(w[0], w[1], w[2]) - 3.0 * w[q];
-    return -s * (vol * inv_q) / 12.0;
+    return ice_flux_num(w, q, vol, inv_q) / 12.0;
 }
+// ice_flux(...) >= 0.0 from its numerator g, without the division: g / 12 rounds to a
+// value >= 0 (or to -0) exactly when g >= -6 * 2^-1074 (|g / 12| <= 2^-1075 rounds to
+// zero, ties to even); NaN compares false either way
+__device__ __forceinline__ bool ice_flux_nonneg(double g) { return g >= -0x6p-1074; }
 
 // pass 3: flux sums, cluster bounds and clipping ratios of every node (ice_adv.py:132-160)
 __global__ void __launch_bounds__(256) k_ice_ratio(IceArgs a, const double* dv) {
@@ -226,18 +237,21 @@ __global__ void __launch_bounds__(256) k_ice_apply(IceArgs a, const double* dv) 
         const bool on = elem_open(a, e);
         const double vol = glb(a.vol)[e];
         double invj[3];
-        for (int j = 0; j < 3; ++j) invj[j] = inv_area(a, vv[j]);
+        for (int j = 0; j < 3; ++j) invj[j] = glb(a.inv)[vv[j]];
         for (int t = 0; t < ICE_K; ++t) {
             const size_t o = (size_t)t * a.N;
             const auto X = glb(a.X[t]);
             double w[3];
             for (int j = 0; j < 3; ++j) w[j] = a.gamma * X[vv[j]] + D[o + vv[j]];
-            double f[3], ae = INFINITY;
+            // the element's factor takes each vertex's ratio by the sign of that vertex's
+            // flux (0 when the element is closed); only the node's own flux is divided
+            double ae = INFINITY, gq = 0.0;
             for (int j = 0; j < 3; ++j) {
-                f[j] = on ? ice_flux(a.gamma, w, j, vol, invj[j]) : 0.0;
-                ae = fmin(ae, f[j] >= 0.0 ? PP[o + vv[j]] : PM[o + vv[j]]);
+                const double g = ice_flux_num(w, j, vol, invj[j]);
+                gq = j == q ? g : gq;
+                ae = fmin(ae, !on || ice_flux_nonneg(g) ? PP[o + vv[j]] : PM[o + vv[j]]);
             }
-            acc[t] = acc[t] + (on ? f[q] * ae : 0.0);
+            acc[t] = acc[t] + (on ? (gq / 12.0) * ae : 0.0);
         }
     }
     for (int t = 0; t < ICE_K; ++t) {
@@ -248,7 +262,7 @@ __global__ void __launch_bounds__(256) k_ice_apply(IceArgs a, const double* dv) 
 
 }  // namespace
 
-size_t ice_workspace_doubles(int N) { return (size_t)18 * N; }
+size_t ice_workspace_doubles(int N) { return (size_t)19 * N; }
 
 int launch_ice_fct(const DevMesh& m, const IceCall& c, double* work, const double* sq,
                    hipStream_t s) {
@@ -281,6 +295,7 @@ int launch_ice_fct(const DevMesh& m, const IceCall& c, double* work, const doubl
     a.d1 = work + 3 * N3;
     a.pp = work + 4 * N3;
     a.pm = work + 5 * N3;
+    a.inv = work + 6 * N3;
     const dim3 grid((m.N + 255) / 256), block(256);
     k_ice_rhs<<<grid, block, 0, s>>>(a);
     k_ice_hi<<<grid, block, 0, s>>>(a, a.d0, a.d1);
