@@ -6,9 +6,10 @@
 // the tracers share every geometry load). Each node gathers over its elements in
 // increasing element id (the node -> element CSR), which is the order of the
 // reference's scatter-add over (element, vertex); sums over an element's three
-// vertices run ((x0 + x1) + x2). Four launches, each a gather pass over the elements
-// around every node: right-hand side + low-order solve, two high-order corrections,
-// fluxes + bounds + ratios, limited fluxes + update.
+// vertices run ((x0 + x1) + x2). Five launches, each a gather pass over the elements
+// around every node: right-hand side + low-order solve, two high-order corrections (the
+// second also stores each node's gamma * X + d, the vertex value of the antidiffusive
+// fluxes), fluxes + bounds + ratios, limited fluxes + update.
 
 #include <hip/hip_runtime.h>
 
@@ -49,6 +50,7 @@ struct IceArgs {
     double* pp;             // [3][N] ratios
     double* pm;
     double* inv;            // [N] 1 / surface area (k_ice_rhs), as inv_area computes it
+    double* w;              // [3][N] gamma * X + d with the final d (second k_ice_hi)
 };
 
 __device__ __forceinline__ double inv_area(const IceArgs& a, int n) {
@@ -124,8 +126,10 @@ __global__ void __launch_bounds__(256) k_ice_rhs(IceArgs a) {
     }
 }
 
-// pass 2 (twice): d <- d + (rhs - mm d) / area (ice_adv.py:102-104)
-__global__ void __launch_bounds__(256) k_ice_hi(IceArgs a, const double* din, double* dout) {
+// pass 2 (twice): d <- d + (rhs - mm d) / area (ice_adv.py:102-104); the second pass also
+// stores gamma * X + d (wout), the vertex value of the antidiffusive fluxes
+__global__ void __launch_bounds__(256) k_ice_hi(IceArgs a, const double* din, double* dout,
+                                                double* wout) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= a.N) return;
     const auto en = glb(a.en);
@@ -150,7 +154,9 @@ __global__ void __launch_bounds__(256) k_ice_hi(IceArgs a, const double* din, do
     for (int t = 0; t < ICE_K; ++t) {
         const size_t i = (size_t)t * a.N + n;
         const double d = D[i];
-        glb(dout)[i] = nc ? d + (glb(a.rhs)[i] - mm[t]) * inv : d;
+        const double dn = nc ? d + (glb(a.rhs)[i] - mm[t]) * inv : d;
+        glb(dout)[i] = dn;
+        if (wout) glb(wout)[i] = a.gamma * glb(a.X[t])[n] + dn;
     }
 }
 
@@ -171,11 +177,11 @@ __device__ __forceinline__ double ice_flux(double g, const double (&w)[3], int q
 __device__ __forceinline__ bool ice_flux_nonneg(double g) { return g >= -0x6p-1074; }
 
 // pass 3: flux sums, cluster bounds and clipping ratios of every node (ice_adv.py:132-160)
-__global__ void __launch_bounds__(256) k_ice_ratio(IceArgs a, const double* dv) {
+__global__ void __launch_bounds__(256) k_ice_ratio(IceArgs a) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= a.N) return;
     const auto en = glb(a.en);
-    const auto D = glb(dv);
+    const auto W = glb(a.w);
     const auto XL = glb(a.xl);
     const double inv = inv_area(a, n);
     double fp[ICE_K] = {0.0, 0.0, 0.0}, fm[ICE_K] = {0.0, 0.0, 0.0};
@@ -194,7 +200,7 @@ __global__ void __launch_bounds__(256) k_ice_ratio(IceArgs a, const double* dv) 
             double w[3], eh = -INFINITY, el = INFINITY;
             for (int j = 0; j < 3; ++j) {
                 const double x = X[vv[j]], l = XL[o + vv[j]];
-                w[j] = a.gamma * x + D[o + vv[j]];
+                w[j] = W[o + vv[j]];
                 eh = fmax(eh, fmax(l, x));
                 el = fmin(el, fmin(l, x));
             }
@@ -222,11 +228,11 @@ __global__ void __launch_bounds__(256) k_ice_ratio(IceArgs a, const double* dv) 
 
 // pass 4: each element's fluxes limited by the smallest ratio of its vertices, summed
 // into the node, added to the low-order solution (ice_adv.py:162-169)
-__global__ void __launch_bounds__(256) k_ice_apply(IceArgs a, const double* dv) {
+__global__ void __launch_bounds__(256) k_ice_apply(IceArgs a) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= a.N) return;
     const auto en = glb(a.en);
-    const auto D = glb(dv);
+    const auto W = glb(a.w);
     const auto PP = glb(a.pp), PM = glb(a.pm);
     double acc[ICE_K] = {0.0, 0.0, 0.0};
     const int s1 = glb(a.ncsr_off)[n + 1];
@@ -240,9 +246,8 @@ __global__ void __launch_bounds__(256) k_ice_apply(IceArgs a, const double* dv) 
         for (int j = 0; j < 3; ++j) invj[j] = glb(a.inv)[vv[j]];
         for (int t = 0; t < ICE_K; ++t) {
             const size_t o = (size_t)t * a.N;
-            const auto X = glb(a.X[t]);
             double w[3];
-            for (int j = 0; j < 3; ++j) w[j] = a.gamma * X[vv[j]] + D[o + vv[j]];
+            for (int j = 0; j < 3; ++j) w[j] = W[o + vv[j]];
             // the element's factor takes each vertex's ratio by the sign of that vertex's
             // flux (0 when the element is closed); only the node's own flux is divided
             double ae = INFINITY, gq = 0.0;
@@ -262,7 +267,7 @@ __global__ void __launch_bounds__(256) k_ice_apply(IceArgs a, const double* dv) 
 
 }  // namespace
 
-size_t ice_workspace_doubles(int N) { return (size_t)19 * N; }
+size_t ice_workspace_doubles(int N) { return (size_t)22 * N; }
 
 int launch_ice_fct(const DevMesh& m, const IceCall& c, double* work, const double* sq,
                    hipStream_t s) {
@@ -296,12 +301,13 @@ int launch_ice_fct(const DevMesh& m, const IceCall& c, double* work, const doubl
     a.pp = work + 4 * N3;
     a.pm = work + 5 * N3;
     a.inv = work + 6 * N3;
+    a.w = work + 6 * N3 + m.N;
     const dim3 grid((m.N + 255) / 256), block(256);
     k_ice_rhs<<<grid, block, 0, s>>>(a);
-    k_ice_hi<<<grid, block, 0, s>>>(a, a.d0, a.d1);
-    k_ice_hi<<<grid, block, 0, s>>>(a, a.d1, a.d0);
-    k_ice_ratio<<<grid, block, 0, s>>>(a, a.d0);
-    k_ice_apply<<<grid, block, 0, s>>>(a, a.d0);
+    k_ice_hi<<<grid, block, 0, s>>>(a, a.d0, a.d1, nullptr);
+    k_ice_hi<<<grid, block, 0, s>>>(a, a.d1, a.d0, a.w);
+    k_ice_ratio<<<grid, block, 0, s>>>(a);
+    k_ice_apply<<<grid, block, 0, s>>>(a);
     return check_launch("k_ice");
 }
 

@@ -185,7 +185,7 @@ int fct_forward_upwind(fct_plan* plan, const fct_step_args* args, void* stream);
  * (fesom_jax/ice_adv.py:177-194, exch=None): Taylor-Galerkin right-hand side,
  * consistent-mass low- and high-order solves and the element Zalesak limiter of the
  * three ice tracers on the surface nodes. All fields [nod2D], device; outputs must not
- * alias inputs. The first call allocates 19 * nod2D doubles of workspace and one factor
+ * alias inputs. The first call allocates 22 * nod2D doubles of workspace and one factor
  * per element; later calls allocate nothing. */
 typedef struct fct_ice_args {
     double ice_dt;                    /* IceConfig.ice_dt (fesom_jax/ice.py:62) */
