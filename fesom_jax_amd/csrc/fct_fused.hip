@@ -732,10 +732,10 @@ __device__ __forceinline__ void pass3_body(const FusedArgs& a, int n, int j, int
         if (64 * s >= kneed) continue;
         const int k = 64 * s + lane_id();
         const bool wet = in_layer(nr, k);
-        if (k < kneed) {
-            Tc[s] = T[b0 + k];
-            fvc[s] = Fv[b0 + k];
-        }
+        // T on every level of the slot (one load for the wet levels and the dry ones
+        // below, which are copied)
+        if (k < L) Tc[s] = T[b0 + k];
+        if (k < kneed) fvc[s] = Fv[b0 + k];
         double fh[DM] = {}, po[DM] = {}, mo[DM] = {};
         if (wet) {
 #pragma unroll
@@ -788,7 +788,7 @@ __device__ __forceinline__ void pass3_body(const FusedArgs& a, int n, int j, int
         if (k >= L) continue;
         const size_t i = b0 + k;
         if (!in_layer(nr, k)) {
-            const double Ti = (k < kneed) ? Tc[s] : T[i];
+            const double Ti = 64 * s < kneed ? Tc[s] : T[i];
             Tn[i] = Ti + 0.0;
             continue;
         }
