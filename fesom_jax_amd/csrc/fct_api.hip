@@ -548,6 +548,9 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
     }
 
     const size_t NL = (size_t)N * L, DL = (size_t)D * L, EL = (size_t)E * L;
+    std::vector<double2> h_aav(NL), h_zzb(L);
+    for (size_t i = 0; i < NL; ++i) h_aav[i] = make_double2(d->area[i], d->areasvol[i]);
+    for (int k = 0; k < L; ++k) h_zzb[k] = make_double2(h_Zs[k], d->zbar[k]);
     if ((rc = dev_upload(p, &m.edges, h_edges.data(), D)) ||
         (rc = dev_upload(p, &m.edge_tri, h_et.data(), D)) ||
         (rc = dev_upload(p, &m.updn, h_updn.data(), D)) ||
@@ -567,6 +570,8 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
         (rc = dev_upload(p, &m.areasvol, d->areasvol, NL)) ||
         (rc = dev_upload(p, &m.Zs, h_Zs.data(), h_Zs.size())) ||
         (rc = dev_upload(p, &m.zbar, d->zbar, (size_t)L)) ||
+        (rc = dev_upload(p, &m.aav, h_aav.data(), NL)) ||
+        (rc = dev_upload(p, &m.zzb, h_zzb.data(), (size_t)L)) ||
         (rc = dev_alloc(p, &p->vflux, DL)) ||
         (rc = dev_upload(p, &p->ekr, h_ekr.data(), D))) {
         free_plan(p);

@@ -39,6 +39,11 @@ struct DevMesh {
     const double* areasvol;   // [N*L]
     const double* Zs;         // [4*L] static QR4C stencil depths Z[k], Z[k-1], Z[k-2], Z[k+1]
     const double* zbar;       // [L]
+    // fused path: (area, areasvol) at every node level and (Z[k], zbar[k]) of the static
+    // stencil, for the passes that need both of a pair: one 16-byte load instead of two
+    // 8-byte ones
+    const double2* aav;       // [N*L]
+    const double2* zzb;       // [L]
     // fused-path edge tables (nl <= 128): the layer ranges of an edge's two triangles as
     // bytes lo1|hi1|lo2|hi2 (a missing triangle: empty), and the up and down gradient
     // triangles: the two vertices other than the edge's first node (up) and second node

@@ -311,9 +311,10 @@ __device__ __forceinline__ void pass1_body(const FusedArgs& a, int n, int j, int
                 vf[q] = G(a.vflux)[(size_t)(es[q].x >> 1) * L + k];
                 To[q] = T[(size_t)es[q].y * L + k];
             }
-            w0 = G(a.w_e)[i]; a0 = G(m.area)[i];
+            const double2 aa = ldv(m.aav, i);
+            w0 = G(a.w_e)[i]; a0 = aa.x; av = aa.y;
             if (NS > 1) { w1 = G(a.w_e)[ip]; a1 = G(m.area)[ip]; }
-            av = G(m.areasvol)[i]; hn = G(a.hnn)[i]; hnd = G(a.hnode)[i];
+            hn = G(a.hnn)[i]; hnd = G(a.hnode)[i];
             // the element vertices are the node and edge neighbours: the gradient below
             // reads them back from the lane's own LDS row by slot code (bits 16.. of the
             // element-slot word: 4 bits per vertex, DM = the node) instead of loading
@@ -532,10 +533,11 @@ __device__ __forceinline__ double adfv_at(int k, int2 nr, double A1, double A, d
 }
 
 // Column of adf_v for the node: every lane k <= nzmax (0 elsewhere). All lanes active.
+// Also hands back areasvol at every level (loaded with the area, clamped like it).
 template <int NS>
 __device__ __forceinline__ void adfv_column(const FusedArgs& a, int n, int2 nr,
                                             const double (&Tc)[NS], const double (&Ac)[NS],
-                                            double (&out)[NS]) {
+                                            double (&out)[NS], double (&avc)[NS]) {
     const DevMesh& m = a.m;
     const int L = m.L;
     const size_t b0 = (size_t)n * L;
@@ -551,9 +553,17 @@ __device__ __forceinline__ void adfv_column(const FusedArgs& a, int n, int2 nr,
         const int kc = k < L ? k : L - 1;
         const size_t i = b0 + kc;
         W[s] = G(a.w_e)[i];
-        ar[s] = G(m.area)[i];
-        Zc[s] = a.Z3d ? G(a.Z3d)[i] : G(m.Zs)[kc];
-        zb[s] = a.Z3d ? G(a.zbar3)[i] : G(m.zbar)[kc];
+        const double2 aa = ldv(m.aav, i);
+        ar[s] = aa.x;
+        avc[s] = aa.y;
+        if (a.Z3d) {
+            Zc[s] = G(a.Z3d)[i];
+            zb[s] = G(a.zbar3)[i];
+        } else {
+            const double2 z = ldv(m.zzb, kc);
+            Zc[s] = z.x;
+            zb[s] = z.y;
+        }
     }
     // The reference's stencil depths (_z_stencil, tracer_adv.py:228-238) clamp Z[k-1] and
     // Z[k-2] at the top and Z[k+1] at the bottom. qc is needed at levels nr.x+1 .. nr.y-1
@@ -657,8 +667,8 @@ __device__ __forceinline__ void pass2_body(const FusedArgs& a, int n, int j, int
     col_next<NS>(tvx, tvxp, 0.0);
     col_prev<NS>(tvn, tvnm, col_bcast0<NS>(tvn));
     col_next<NS>(tvn, tvnp, 0.0);
-    double fvc[NS], fvb[NS];
-    adfv_column<NS>(a, n, nr, Tc, Ac, fvc);
+    double fvc[NS], fvb[NS], avc[NS];
+    adfv_column<NS>(a, n, nr, Tc, Ac, fvc, avc);
     col_next<NS>(fvc, fvb, 0.0);
     // for pass 3, which reads it instead of computing it again
     const auto Fv = G(a.adfv + jo);
@@ -681,7 +691,7 @@ __device__ __forceinline__ void pass2_body(const FusedArgs& a, int n, int j, int
         const double neg_v = fmin(fvc[s], 0.0) + fmin(-fvb[s], 0.0);
         const double fplus = pos_v + ap[s];
         const double fminus = neg_v + am[s];
-        const double av = G(m.areasvol)[i], hn = G(a.hnn)[i];
+        const double av = avc[s], hn = G(a.hnn)[i];
         const double safe_a = av > 0.0 ? av : 1.0, safe_hn = hn > 0.0 ? hn : 1.0;
         const double flux_pos = fplus * a.dt / safe_a / safe_hn + FCT_FLUX_EPS;
         const double flux_neg = fminus * a.dt / safe_a / safe_hn - FCT_FLUX_EPS;

@@ -77,7 +77,7 @@ def estimate_bytes(mesh, K, nsteps, adjoint):
     (adjoint) the stored trajectory plus the adjoint workspace."""
     N, E, D, L = mesh.nod2D, mesh.elem2D, mesh.edge2D, mesh.nl
     NL, EL, DL = N * L, E * L, D * L
-    topo = 200 * N + 200 * E + 150 * D
+    topo = 200 * N + 200 * E + 150 * D + 8 * 4 * NL     # + area, areasvol and their pairs
     work = 8 * (DL + K * (6 * NL + DL))
     fields = 8 * (3 * EL + 3 * NL + 3 * K * NL)
     total = topo + work + fields
