@@ -385,6 +385,107 @@ __device__ __forceinline__ void pass1_body(const FusedArgs& a, int n, int j, int
     }
 }
 
+// Pass 1 of a shallow node (wet layers end above level 64, one 64-level slot) for its KT
+// tracers in one wavefront: the tracer-independent loads (the edge fluxes, w, area,
+// areasvol, hnode, hnode_new) are made once; per tracer the arithmetic is pass1_body's.
+// KT is a compile-time count so that the tracer pointers are read at constant offsets.
+template <int DM, int KT>
+__device__ __forceinline__ void pass1_shallow_all(const FusedArgs& a, int n, int4 hd) {
+    const DevMesh& m = a.m;
+    const int L = m.L;
+    const int2 nr = make_int2(hd.x, hd.y), cnt = make_int2(hd.z & 0xff, hd.w);
+    const int glo = (hd.z >> 8) & 0xff, ghi = (hd.z >> 16) & 0xff;
+    const size_t b0 = (size_t)n * L;
+    const int kneed = nr.y + 1 < L ? nr.y + 1 : L;
+    int4 es[DM];
+    int4 el[DM];
+#pragma unroll
+    for (int q = 0; q < DM; ++q) {
+        es[q] = ldc(a.tab.edge, (size_t)n * DM + q);
+        el[q] = ldc(a.tab.elem, (size_t)n * DM + q);
+    }
+    __shared__ double s_nb[4][DM + 1][64];
+    double* nb = &s_nb[threadIdx.x >> 6][0][lane_id()];
+    const int k = lane_id();
+    const bool rd = k < kneed;
+    const bool wet = in_layer(nr, k);
+    const bool gslot = glo < ghi;
+    const size_t i = b0 + k;
+    double vf[DM] = {};
+    double w0 = 0.0, a0 = 0.0, av = 0.0, hn = 0.0, hnd = 0.0;
+    if (wet) {
+#pragma unroll
+        for (int q = 0; q < DM; ++q) vf[q] = G(a.vflux)[(size_t)(es[q].x >> 1) * L + k];
+        const double2 aa = ldv(m.aav, i);
+        w0 = G(a.w_e)[i]; a0 = aa.x; av = aa.y;
+        hn = G(a.hnn)[i]; hnd = G(a.hnode)[i];
+    }
+    const double w1 = wave_next(w0, 0.0), a1 = wave_next(a0, 0.0);
+#pragma unroll
+    for (int j = 0; j < KT; ++j) {
+        const auto T = G(a.T[j]);
+        const size_t jo = (size_t)j * m.N * L;
+        const double Tk = rd ? T[b0 + k] : 0.0;
+        double To[DM] = {};
+        if (wet) {
+#pragma unroll
+            for (int q = 0; q < DM; ++q) To[q] = T[(size_t)es[q].y * L + k];
+            if (gslot) {
+#pragma unroll
+                for (int q = 0; q < DM; ++q) nb[q * 64] = To[q];
+                nb[DM * 64] = Tk;
+            }
+        }
+        const double Tm = wave_prev(Tk, Tk);
+        const double Tp = wave_next(Tk, 0.0);
+        if (!wet) continue;
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < DM; ++q) {
+            const double v = vf[q];
+            const bool side = es[q].x & 1;
+            const double T1 = side ? To[q] : Tk;
+            const double T2 = side ? Tk : To[q];
+            const double aq = fabs(v);
+            const double f = -0.5 * (T1 * (v + aq) + T2 * (v - aq));
+            const bool use = q < cnt.x && v != 0.0;
+            acc = acc + (use ? (side ? -f : f) : 0.0);
+        }
+        double fv0, fv1 = 0.0;
+        {
+            const double aw = fabs(w0);
+            fv0 = (-0.5 * (Tk * (w0 + aw) + Tm * (w0 - aw))) * a0;
+        }
+        if (in_layer(nr, k + 1)) {
+            const double aw = fabs(w1);
+            fv1 = (-0.5 * (Tp * (w1 + aw) + Tk * (w1 - aw))) * a1;
+        }
+        const double safe_a = av > 0.0 ? av : 1.0, safe_hn = hn > 0.0 ? hn : 1.0;
+        const double numer = Tk * hnd + (acc + (fv0 - fv1)) * a.dt / safe_a;
+        const double lo = numer / safe_hn;
+        G(a.LO)[jo + i] = (hn > 0.0 && av > 0.0) ? lo : 0.0;
+        if (!gslot) continue;
+        double nx = 0.0, ny = 0.0, den = 0.0;
+#pragma unroll
+        for (int q = 0; q < DM; ++q) {
+            const int e = el[q].x;
+            const bool use = q < cnt.y && in_bytes(es[q].w, 0, k);
+            const size_t g6 = 6 * (size_t)e;
+            const unsigned codes = (unsigned)es[q].w >> 16;
+            const double tv0 = nb[(codes & 0xf) * 64], tv1 = nb[((codes >> 4) & 0xf) * 64];
+            const double tv2 = nb[((codes >> 8) & 0xf) * 64];
+            const double gx = ldc(m.gs, g6) * tv0 + ldc(m.gs, g6 + 1) * tv1 + ldc(m.gs, g6 + 2) * tv2;
+            const double gy = ldc(m.gs, g6 + 3) * tv0 + ldc(m.gs, g6 + 4) * tv1 + ldc(m.gs, g6 + 5) * tv2;
+            const double ea = ldc(m.earea, e);
+            nx = nx + (use ? gx * ea : 0.0);
+            ny = ny + (use ? gy * ea : 0.0);
+            den = den + (use ? ea : 0.0);
+        }
+        const double d = den > 0.0 ? den : 1.0;
+        if (k >= glo && k < ghi) stv(a.gnode, jo + i, make_double2(nx / d, ny / d));
+    }
+}
+
 template <int NS, int DM>
 __device__ __forceinline__ void pass1_task(const FusedArgs& a, int n, int j) {
     pass1_body<NS, DM>(a, n, j, ldc(a.tab.hdr, n));
@@ -1001,6 +1102,15 @@ FCT_BATCH_LIST_KERNEL(pass3, 8, 6, (pass3_body<NSB, DM>(a, n, w - t * a.K, make_
                                                         make_int2(hd.z & 0xff, hd.w))))
 #undef FCT_BATCH_LIST_KERNEL
 
+// one wavefront per shallow node, KT = K tracers
+template <int DM, int KT>
+__global__ void FCT_OCC(DM == 6 ? 5 : 4) kl_pass1_all(FusedArgs a, int count) {
+    int w;
+    if (!single_wave((int64_t)count, a.remap, w)) return;
+    const int n = ldc(a.tab.nlist, (size_t)w);
+    pass1_shallow_all<DM, KT>(a, n, ldc(a.tab.hdr, n));
+}
+
 template <int NSB, int DM>
 __global__ void FCT_OCC(DM == 6 ? 8 : 4) kl_pass1(FusedArgs a, int first,
                                                                    int count) {
@@ -1033,8 +1143,14 @@ static void launch_single(const FusedArgs& fa, const FusedTuning& t, hipStream_t
         case 1:
             if (NS == 2 && fa.tab.nlist) {
                 const int ns = fa.tab.nshallow, nd = m.N - fa.tab.nshallow;
-                if (ns) hipLaunchKernelGGL((kl_pass1<1, DM>), dim3(blocks_for(node_waves(ns, fa.K), t.wpb)),
-                                           blk, 0, s, fa, 0, ns);
+                // two tracers (temperature and salinity) share the tracer-independent
+                // loads of a shallow node in one wavefront
+                if (ns && DM == 6 && fa.K == 2)
+                    hipLaunchKernelGGL((kl_pass1_all<DM, 2>), dim3(blocks_for(ns, t.wpb)),
+                                       blk, 0, s, fa, ns);
+                else if (ns)
+                    hipLaunchKernelGGL((kl_pass1<1, DM>), dim3(blocks_for(node_waves(ns, fa.K), t.wpb)),
+                                       blk, 0, s, fa, 0, ns);
                 if (nd) hipLaunchKernelGGL((kl_pass1<2, DM>), dim3(blocks_for(node_waves(nd, fa.K), t.wpb)),
                                            blk, 0, s, fa, ns, nd);
             } else {
