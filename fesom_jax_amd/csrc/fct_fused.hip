@@ -919,6 +919,119 @@ __device__ __forceinline__ void pass3_body(const FusedArgs& a, int n, int j, int
     for (int k = 64 * NS + lane_id(); k < L; k += 64) Tn[b0 + k] = T[b0 + k] + 0.0;
 }
 
+// Pass 3 of a shallow node (one 64-level slot) for its KT tracers in one wavefront:
+// areasvol, hnode_new and hnode are loaded once; per tracer the arithmetic is
+// pass3_body's.
+template <int DM, int KT>
+__device__ __forceinline__ void pass3_shallow_all(const FusedArgs& a, int n, int2 nr, int2 cnt) {
+    constexpr int NS = 1;
+    const DevMesh& m = a.m;
+    const int L = m.L;
+    int4 es[DM];
+#pragma unroll
+    for (int q = 0; q < DM; ++q) es[q] = ldc(a.tab.edge, (size_t)n * DM + q);
+    const int kl = lane_id();
+    const bool wetl = in_layer(nr, kl);
+    double av_s = 0.0, hn_s = 0.0, hnd_s = 0.0;
+    if (wetl) {
+        av_s = G(m.areasvol)[(size_t)n * L + kl];
+        hn_s = G(a.hnn)[(size_t)n * L + kl];
+        hnd_s = G(a.hnode)[(size_t)n * L + kl];
+    }
+#pragma unroll
+    for (int j = 0; j < KT; ++j) {
+        const auto T = G(a.T[j]);
+        const size_t jo = (size_t)j * m.N * L;
+        const double2* pm = reinterpret_cast<const double2*>(a.pf) + jo;   // (plus, minus) pairs
+        const auto adfh = G(a.adfh + (size_t)j * m.D * L);
+        const size_t b0 = (size_t)n * L;
+        const int kneed = nr.y + 1 < L ? nr.y + 1 : L;
+        // adf_v at the node's interfaces (k <= nzmax) as pass 2 computed it
+        const auto Fv = G(a.adfv + jo);
+        double Tc[NS], fvc[NS], Pc[NS], Mc[NS], acc[NS];
+    #pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            Tc[s] = 0.0; fvc[s] = 0.0; Pc[s] = 1.0; Mc[s] = 1.0; acc[s] = 0.0;
+            if (64 * s >= kneed) continue;
+            const int k = 64 * s + lane_id();
+            const bool wet = in_layer(nr, k);
+            // T on every level of the slot (one load for the wet levels and the dry ones
+            // below, which are copied)
+            if (k < L) Tc[s] = T[b0 + k];
+            if (k < kneed) fvc[s] = Fv[b0 + k];
+            double fh[DM] = {}, po[DM] = {}, mo[DM] = {};
+            if (wet) {
+    #pragma unroll
+                for (int q = 0; q < DM; ++q) {
+                    fh[q] = adfh[(size_t)(es[q].x >> 1) * L + k];
+                    const double2 f = ldv(pm, (size_t)es[q].y * L + k);
+                    po[q] = f.x;
+                    mo[q] = f.y;
+                }
+            }
+            if (wet) {
+                const double2 f = ldv(pm, b0 + k);
+                Pc[s] = f.x;
+                Mc[s] = f.y;
+            }
+    #pragma unroll
+            for (int q = 0; q < DM; ++q) {
+                const double f = fh[q];
+                const bool side = es[q].x & 1;
+                const double ae = side ? (f >= 0.0 ? fmin(po[q], Mc[s]) : fmin(mo[q], Pc[s]))
+                                       : (f >= 0.0 ? fmin(Pc[s], mo[q]) : fmin(Mc[s], po[q]));
+                const double fl = ae * f;
+                const bool use = wet && q < cnt.x && f != 0.0;
+                acc[s] = acc[s] + (use ? (side ? -fl : fl) : 0.0);
+            }
+        }
+        double Pm[NS], Mm[NS];
+        col_prev<NS>(Pc, Pm, col_bcast0<NS>(Pc));
+        col_prev<NS>(Mc, Mm, col_bcast0<NS>(Mc));
+        double vl[NS];
+    #pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int k = 64 * s + lane_id();
+            vl[s] = 0.0;
+            if (in_iface(nr, k) && k < L) {
+                const double f = fvc[s];
+                double ae;
+                if (k == nr.x) ae = f >= 0.0 ? Pc[s] : Mc[s];
+                else ae = f >= 0.0 ? fmin(Mm[s], Pc[s]) : fmin(Pm[s], Mc[s]);
+                vl[s] = ae * f;
+            }
+        }
+        double vl1[NS];
+        col_next<NS>(vl, vl1, 0.0);
+        const auto LO = G(a.LO + jo);
+        const auto Tn = G(a.Tnew[j]);
+    #pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int k = 64 * s + lane_id();
+            if (k >= L) continue;
+            const size_t i = b0 + k;
+            if (!in_layer(nr, k)) {
+                const double Ti = 64 * s < kneed ? Tc[s] : T[i];
+                Tn[i] = Ti + 0.0;
+                continue;
+            }
+            const double av = av_s, hn = hn_s, hnd = hnd_s;
+            const double safe_a = av > 0.0 ? av : 1.0, safe_hn = hn > 0.0 ? hn : 1.0;
+            const double Ti = Tc[s];
+            const double lo_trans = -Ti * hnd + LO[i] * hn;
+            const double vdiv = av > 0.0 ? (vl[s] - vl1[s]) * a.dt / safe_a : 0.0;
+            const double dttf_v = lo_trans + vdiv;
+            const double dttf_h = acc[s] * a.dt / safe_a;
+            double del = dttf_h + dttf_v;
+            del = del + Ti * (hnd - hn);
+            const double dT = hn > 0.0 ? del / safe_hn : 0.0;
+            Tn[i] = Ti + dT;
+        }
+        // levels past the slots (a shallow column taking the one-slot body): dry copies
+        for (int k = 64 * NS + lane_id(); k < L; k += 64) Tn[b0 + k] = T[b0 + k] + 0.0;
+    }
+}
+
 template <int NS, int DM>
 __device__ __forceinline__ void pass3_task(const FusedArgs& a, int n, int j) {
     const int4 hd = ldc(a.tab.hdr, n);
@@ -1102,6 +1215,15 @@ FCT_BATCH_LIST_KERNEL(pass3, 8, 6, (pass3_body<NSB, DM>(a, n, w - t * a.K, make_
                                                         make_int2(hd.z & 0xff, hd.w))))
 #undef FCT_BATCH_LIST_KERNEL
 
+template <int DM, int KT>
+__global__ void FCT_OCC(DM == 6 ? 7 : 4) kl_pass3_all(FusedArgs a, int count) {
+    int w;
+    if (!single_wave((int64_t)count, a.remap, w)) return;
+    const int n = ldc(a.tab.nlist, (size_t)w);
+    const int4 hd = ldc(a.tab.hdr, n);
+    pass3_shallow_all<DM, KT>(a, n, make_int2(hd.x, hd.y), make_int2(hd.z & 0xff, hd.w));
+}
+
 // one wavefront per shallow node, KT = K tracers
 template <int DM, int KT>
 __global__ void FCT_OCC(DM == 6 ? 5 : 4) kl_pass1_all(FusedArgs a, int count) {
@@ -1174,7 +1296,11 @@ static void launch_single(const FusedArgs& fa, const FusedTuning& t, hipStream_t
                     if (ns) hipLaunchKernelGGL((kl_pass2<1, DM>), dim3(g1), blk, 0, s, fa, 0, ns);
                     if (nd) hipLaunchKernelGGL((kl_pass2<2, DM>), dim3(g2), blk, 0, s, fa, ns, nd);
                 } else {
-                    if (ns) hipLaunchKernelGGL((kl_pass3<1, DM>), dim3(g1), blk, 0, s, fa, 0, ns);
+                    if (ns && DM == 6 && fa.K == 2)
+                        hipLaunchKernelGGL((kl_pass3_all<DM, 2>), dim3(blocks_for(ns, t.wpb)), blk, 0,
+                                           s, fa, ns);
+                    else if (ns)
+                        hipLaunchKernelGGL((kl_pass3<1, DM>), dim3(g1), blk, 0, s, fa, 0, ns);
                     if (nd) hipLaunchKernelGGL((kl_pass3<2, DM>), dim3(g2), blk, 0, s, fa, ns, nd);
                 }
             } else if (stage == 3) {
