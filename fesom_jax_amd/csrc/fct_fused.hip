@@ -635,19 +635,15 @@ __device__ __forceinline__ double adfv_at(int k, int2 nr, double A1, double A, d
 
 // Column of adf_v for the node: every lane k <= nzmax (0 elsewhere). All lanes active.
 // Also hands back areasvol at every level (loaded with the area, clamped like it).
+// The column's w, area, areasvol, depth Z at k and interface depth at every lane
+// (clamped below the last level): the tracer-independent inputs of adfv_column.
 template <int NS>
-__device__ __forceinline__ void adfv_column(const FusedArgs& a, int n, int2 nr,
-                                            const double (&Tc)[NS], const double (&Ac)[NS],
-                                            double (&out)[NS], double (&avc)[NS]) {
+__device__ __forceinline__ void adfv_inputs(const FusedArgs& a, int n, double (&W)[NS],
+                                            double (&ar)[NS], double (&avc)[NS],
+                                            double (&Zc)[NS], double (&zb)[NS]) {
     const DevMesh& m = a.m;
     const int L = m.L;
     const size_t b0 = (size_t)n * L;
-    double Tm[NS], A1[NS];
-    col_prev<NS>(Tc, Tm, col_bcast0<NS>(Tc));
-    col_prev<NS>(Ac, A1, col_bcast0<NS>(Ac));
-    // the column's w, area, depth Z at k and interface depth at every lane (clamped
-    // below the last level)
-    double W[NS], ar[NS], Zc[NS], zb[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         const int k = 64 * s + lane_id();
@@ -666,6 +662,18 @@ __device__ __forceinline__ void adfv_column(const FusedArgs& a, int n, int2 nr,
             zb[s] = z.y;
         }
     }
+}
+
+template <int NS>
+__device__ __forceinline__ void adfv_compute(const FusedArgs& a, int2 nr,
+                                             const double (&Tc)[NS], const double (&Ac)[NS],
+                                             const double (&W)[NS], const double (&ar)[NS],
+                                             const double (&Zc)[NS], const double (&zb)[NS],
+                                             double (&out)[NS]) {
+    const int L = a.m.L;
+    double Tm[NS], A1[NS];
+    col_prev<NS>(Tc, Tm, col_bcast0<NS>(Tc));
+    col_prev<NS>(Ac, A1, col_bcast0<NS>(Ac));
     // The reference's stencil depths (_z_stencil, tracer_adv.py:228-238) clamp Z[k-1] and
     // Z[k-2] at the top and Z[k+1] at the bottom. qc is needed at levels nr.x+1 .. nr.y-1
     // only (qd at nr.x+2 reads nr.x+1, qu at nr.y-2 reads nr.y-1): there 1 <= k <= L-2,
@@ -686,6 +694,15 @@ __device__ __forceinline__ void adfv_column(const FusedArgs& a, int n, int2 nr,
             out[s] = adfv_at(k, nr, A1[s], Ac[s], Tm[s], Tc[s], W[s], ar[s], Zc[s], Zm1[s],
                              qc[s], qu[s], qd[s], zb[s]);
     }
+}
+
+template <int NS>
+__device__ __forceinline__ void adfv_column(const FusedArgs& a, int n, int2 nr,
+                                            const double (&Tc)[NS], const double (&Ac)[NS],
+                                            double (&out)[NS], double (&avc)[NS]) {
+    double W[NS], ar[NS], Zc[NS], zb[NS];
+    adfv_inputs<NS>(a, n, W, ar, avc, Zc, zb);
+    adfv_compute<NS>(a, nr, Tc, Ac, W, ar, Zc, zb, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -807,6 +824,139 @@ __device__ __forceinline__ void pass2_body(const FusedArgs& a, int n, int j, int
         if (bm) fm = fmin(1.0, ttf_min / flux_neg);
         // (plus, minus) factor pairs, interleaved: pass 3 reads both with one load
         stv(reinterpret_cast<double2*>(a.pf), jo + i, make_double2(fp, fm));
+    }
+}
+
+// Pass 2 of a shallow node (one 64-level slot) for its KT tracers in one wavefront: w,
+// area, areasvol, the stencil depths and hnode_new are loaded once; per tracer the
+// arithmetic is pass2_body's.
+template <int DM, int KT>
+__device__ __forceinline__ void pass2_shallow_all(const FusedArgs& a, int n, int2 nr, int2 cnt) {
+    constexpr int NS = 1;
+    const DevMesh& m = a.m;
+    const int L = m.L;
+    double Ws[NS], ars[NS], avs[NS], Zcs[NS], zbs[NS];
+    adfv_inputs<NS>(a, n, Ws, ars, avs, Zcs, zbs);
+    double hns[NS];
+    {
+        const int kl = lane_id();
+        hns[0] = in_layer(nr, kl) ? G(a.hnn)[(size_t)n * L + kl] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < KT; ++j) {
+        const size_t jo = (size_t)j * m.N * L;
+        const size_t b0 = (size_t)n * L;
+        const auto Tb = G(a.T[j]);
+        const auto LOb = G(a.LO + jo);
+        const auto Tr = G(Tb + b0);
+        const auto Or = G(a.Told[j] + b0);
+        const auto LOr = G(LOb + b0);
+        const auto adfh = G(a.adfh + (size_t)j * m.D * L);
+        const int kneed = nr.y + 1 < L ? nr.y + 1 : L;
+        int4 es[DM];
+#pragma unroll
+        for (int q = 0; q < DM; ++q) es[q] = ldc(a.tab.edge, (size_t)n * DM + q);
+        double Tc[NS], Ac[NS], LOc[NS], tvx[NS], tvn[NS], ap[NS], am[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            Tc[s] = 0.0; Ac[s] = 0.0; LOc[s] = 0.0;
+            tvx[s] = -INFINITY; tvn[s] = INFINITY; ap[s] = 0.0; am[s] = 0.0;
+            if (64 * s >= kneed) continue;
+            const int k = 64 * s + lane_id();
+            const bool rd = k < kneed;
+            const bool wet = in_layer(nr, k);
+            // own column and the neighbours' LO, T and the edges' antidiffusive fluxes,
+            // loaded in one basic block
+            double lnb[DM] = {}, tnb[DM] = {}, fh[DM] = {};
+            double lo_own = 0.0;
+            if (rd) {
+                Tc[s] = Tr[k];
+                Ac[s] = ab2(Or[k], Tc[s]);
+                lo_own = LOr[k];
+#pragma unroll
+                for (int q = 0; q < DM; ++q) {
+                    const size_t r = (size_t)es[q].y * L + k;
+                    lnb[q] = LOb[r];
+                    tnb[q] = Tb[r];
+                    fh[q] = adfh[(size_t)(es[q].x >> 1) * L + k];
+                }
+            }
+            LOc[s] = wet ? lo_own : 0.0;
+            // a1-a3: max/min of (LO, T) over the vertices of the node's triangles that are
+            // wet at k, and the -+1e3 pad when one of them is dry at k
+            bool act = false, inact = false;
+#pragma unroll
+            for (int q = 0; q < DM; ++q) {
+                const bool in = in_bytes(es[q].w, 0, k);
+                act = act || (q < cnt.y && in);
+                inact = inact || (q < cnt.y && !in);
+            }
+            double mx = act ? fmax(LOc[s], Tc[s]) : -INFINITY;
+            double mn = act ? fmin(LOc[s], Tc[s]) : INFINITY;
+#pragma unroll
+            for (int q = 0; q < DM; ++q) {
+                const bool inc = q < cnt.x && (in_bytes(es[q].z, 0, k) || in_bytes(es[q].z, 16, k));
+                mx = inc ? fmax(mx, fmax(lnb[q], tnb[q])) : mx;
+                mn = inc ? fmin(mn, fmin(lnb[q], tnb[q])) : mn;
+            }
+            mx = inact ? fmax(mx, -FCT_BIG) : mx;
+            mn = inact ? fmin(mn, FCT_BIG) : mn;
+            tvx[s] = rd ? mx : -INFINITY;
+            tvn[s] = rd ? mn : INFINITY;
+            // b1: P+/P- from the incident edges in reference order
+#pragma unroll
+            for (int q = 0; q < DM; ++q) {
+                const double f = (es[q].x & 1) ? -fh[q] : fh[q];
+                const bool use = wet && q < cnt.x;
+                ap[s] = ap[s] + (use ? fmax(f, 0.0) : 0.0);
+                am[s] = am[s] + (use ? fmin(f, 0.0) : 0.0);
+            }
+        }
+        double tvxm[NS], tvxp[NS], tvnm[NS], tvnp[NS];
+        col_prev<NS>(tvx, tvxm, col_bcast0<NS>(tvx));
+        col_next<NS>(tvx, tvxp, 0.0);
+        col_prev<NS>(tvn, tvnm, col_bcast0<NS>(tvn));
+        col_next<NS>(tvn, tvnp, 0.0);
+        double fvc[NS], fvb[NS];
+        adfv_compute<NS>(a, nr, Tc, Ac, Ws, ars, Zcs, zbs, fvc);
+        col_next<NS>(fvc, fvb, 0.0);
+        // for pass 3, which reads it instead of computing it again
+        const auto Fv = G(a.adfv + jo);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int k = 64 * s + lane_id();
+            if (k <= nr.y && k < L) Fv[b0 + k] = fvc[s];
+        }
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int k = 64 * s + lane_id();
+            if (!in_layer(nr, k)) continue;
+            const size_t i = b0 + k;
+            const bool use_own = (k == nr.x) || (k == nr.y - 1);
+            const double cmax = use_own ? tvx[s] : fmax(fmax(tvxm[s], tvx[s]), tvxp[s]);
+            const double cmin = use_own ? tvn[s] : fmin(fmin(tvnm[s], tvn[s]), tvnp[s]);
+            const double ttf_max = cmax - LOc[s];
+            const double ttf_min = cmin - LOc[s];
+            const double pos_v = fmax(fvc[s], 0.0) + fmax(-fvb[s], 0.0);
+            const double neg_v = fmin(fvc[s], 0.0) + fmin(-fvb[s], 0.0);
+            const double fplus = pos_v + ap[s];
+            const double fminus = neg_v + am[s];
+            const double av = avs[s], hn = hns[s];
+            const double safe_a = av > 0.0 ? av : 1.0, safe_hn = hn > 0.0 ? hn : 1.0;
+            const double flux_pos = fplus * a.dt / safe_a / safe_hn + FCT_FLUX_EPS;
+            const double flux_neg = fminus * a.dt / safe_a / safe_hn - FCT_FLUX_EPS;
+            const bool valid = av > 0.0 && hn > 0.0;
+            // fmin(1, x / y) is 1 wherever x / y >= 1 (rounding is monotone and 1 is a
+            // double), which the signs and one comparison decide: only lanes where the
+            // limiter may bind divide, and a wavefront with none skips the divisions
+            const bool bp = valid && !(flux_pos > 0.0 && ttf_max >= flux_pos);
+            const bool bm = valid && !(flux_neg < 0.0 && ttf_min <= flux_neg);
+            double fp = 1.0, fm = 1.0;
+            if (bp) fp = fmin(1.0, ttf_max / flux_pos);
+            if (bm) fm = fmin(1.0, ttf_min / flux_neg);
+            // (plus, minus) factor pairs, interleaved: pass 3 reads both with one load
+            stv(reinterpret_cast<double2*>(a.pf), jo + i, make_double2(fp, fm));
+        }
     }
 }
 
@@ -949,7 +1099,7 @@ __device__ __forceinline__ void pass3_shallow_all(const FusedArgs& a, int n, int
         // adf_v at the node's interfaces (k <= nzmax) as pass 2 computed it
         const auto Fv = G(a.adfv + jo);
         double Tc[NS], fvc[NS], Pc[NS], Mc[NS], acc[NS];
-    #pragma unroll
+#pragma unroll
         for (int s = 0; s < NS; ++s) {
             Tc[s] = 0.0; fvc[s] = 0.0; Pc[s] = 1.0; Mc[s] = 1.0; acc[s] = 0.0;
             if (64 * s >= kneed) continue;
@@ -961,7 +1111,7 @@ __device__ __forceinline__ void pass3_shallow_all(const FusedArgs& a, int n, int
             if (k < kneed) fvc[s] = Fv[b0 + k];
             double fh[DM] = {}, po[DM] = {}, mo[DM] = {};
             if (wet) {
-    #pragma unroll
+#pragma unroll
                 for (int q = 0; q < DM; ++q) {
                     fh[q] = adfh[(size_t)(es[q].x >> 1) * L + k];
                     const double2 f = ldv(pm, (size_t)es[q].y * L + k);
@@ -974,7 +1124,7 @@ __device__ __forceinline__ void pass3_shallow_all(const FusedArgs& a, int n, int
                 Pc[s] = f.x;
                 Mc[s] = f.y;
             }
-    #pragma unroll
+#pragma unroll
             for (int q = 0; q < DM; ++q) {
                 const double f = fh[q];
                 const bool side = es[q].x & 1;
@@ -989,7 +1139,7 @@ __device__ __forceinline__ void pass3_shallow_all(const FusedArgs& a, int n, int
         col_prev<NS>(Pc, Pm, col_bcast0<NS>(Pc));
         col_prev<NS>(Mc, Mm, col_bcast0<NS>(Mc));
         double vl[NS];
-    #pragma unroll
+#pragma unroll
         for (int s = 0; s < NS; ++s) {
             const int k = 64 * s + lane_id();
             vl[s] = 0.0;
@@ -1005,7 +1155,7 @@ __device__ __forceinline__ void pass3_shallow_all(const FusedArgs& a, int n, int
         col_next<NS>(vl, vl1, 0.0);
         const auto LO = G(a.LO + jo);
         const auto Tn = G(a.Tnew[j]);
-    #pragma unroll
+#pragma unroll
         for (int s = 0; s < NS; ++s) {
             const int k = 64 * s + lane_id();
             if (k >= L) continue;
@@ -1216,6 +1366,15 @@ FCT_BATCH_LIST_KERNEL(pass3, 8, 6, (pass3_body<NSB, DM>(a, n, w - t * a.K, make_
 #undef FCT_BATCH_LIST_KERNEL
 
 template <int DM, int KT>
+__global__ void FCT_OCC(DM == 6 ? 6 : 4) kl_pass2_all(FusedArgs a, int count) {
+    int w;
+    if (!single_wave((int64_t)count, a.remap, w)) return;
+    const int n = ldc(a.tab.nlist, (size_t)w);
+    const int4 hd = ldc(a.tab.hdr, n);
+    pass2_shallow_all<DM, KT>(a, n, make_int2(hd.x, hd.y), make_int2(hd.z & 0xff, hd.w));
+}
+
+template <int DM, int KT>
 __global__ void FCT_OCC(DM == 6 ? 7 : 4) kl_pass3_all(FusedArgs a, int count) {
     int w;
     if (!single_wave((int64_t)count, a.remap, w)) return;
@@ -1293,7 +1452,11 @@ static void launch_single(const FusedArgs& fa, const FusedTuning& t, hipStream_t
                 const unsigned g1 = blocks_for(node_waves(ns, fa.K), t.wpb);
                 const unsigned g2 = blocks_for(node_waves(nd, fa.K), t.wpb);
                 if (stage == 3) {
-                    if (ns) hipLaunchKernelGGL((kl_pass2<1, DM>), dim3(g1), blk, 0, s, fa, 0, ns);
+                    if (ns && DM == 6 && fa.K == 2)
+                        hipLaunchKernelGGL((kl_pass2_all<DM, 2>), dim3(blocks_for(ns, t.wpb)), blk, 0,
+                                           s, fa, ns);
+                    else if (ns)
+                        hipLaunchKernelGGL((kl_pass2<1, DM>), dim3(g1), blk, 0, s, fa, 0, ns);
                     if (nd) hipLaunchKernelGGL((kl_pass2<2, DM>), dim3(g2), blk, 0, s, fa, ns, nd);
                 } else {
                     if (ns && DM == 6 && fa.K == 2)
