@@ -5,7 +5,9 @@
 // column is one coalesced row segment, and vertical neighbours move between lanes with
 // DPP wavefront shifts. For nl > 64 the node passes run as two launches over a node list
 // (kl_*): columns whose wet layers fit 64 lanes in a one-slot kernel, which needs fewer
-// registers, the others in the two-slot kernel. Task indices are wave-uniform (scalar
+// registers, the others in the two-slot kernel. With two tracers on a 6-slot mesh the
+// one-slot launches take one wavefront per node for both tracers (kl_pass*_all), so the
+// tracer-independent loads are made once. Task indices are wave-uniform (scalar
 // registers); each node reads its stencil (incident edges in the reference's summation
 // order with the layer ranges of their triangles, incident triangles with their
 // vertices) from a per-node table with scalar loads, and then issues the loads of all
