@@ -281,6 +281,11 @@ int apply_tuning(FusedTuning& t, int knob, int value) {
             if (value < 1 || value > 64) return set_error(FCT_E_INVALID, "xcd %d out of range", value);
             t.xcd = value;
             return FCT_OK;
+        case FCT_TUNE_PAIR_TRACERS:
+            if (value != 0 && value != 1)
+                return set_error(FCT_E_INVALID, "pair tracers %d not in {0, 1}", value);
+            t.pair = value;
+            return FCT_OK;
         case FCT_TUNE_W_EDGE_FLUX:
             if (value != 0 && value != 1)
                 return set_error(FCT_E_INVALID, "compute_w form %d not in {0, 1}", value);

@@ -127,6 +127,7 @@ struct FusedTuning {
     int vflux_epw = 4;  // edges per wavefront in kf_vflux (2 or 4)
     int adfh_epw = 2;   // edges per wavefront in kf_adfh (1 or 2)
     int w_vflux = 1;    // compute_w from the edge volume flux (1) or per node (0)
+    int pair = 1;       // two tracers: shallow node passes take both in one wavefront
     // the remap as the kernels take it: XCD count in the low 8 bits, chunk above
     int remap() const { return xcd | (xchunk << 8); }
 };

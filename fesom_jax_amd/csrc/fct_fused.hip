@@ -1428,7 +1428,7 @@ static void launch_single(const FusedArgs& fa, const FusedTuning& t, hipStream_t
                 const int ns = fa.tab.nshallow, nd = m.N - fa.tab.nshallow;
                 // two tracers (temperature and salinity) share the tracer-independent
                 // loads of a shallow node in one wavefront
-                if (ns && DM == 6 && fa.K == 2)
+                if (ns && DM == 6 && fa.K == 2 && t.pair)
                     hipLaunchKernelGGL((kl_pass1_all<DM, 2>), dim3(blocks_for(ns, t.wpb)),
                                        blk, 0, s, fa, ns);
                 else if (ns)
@@ -1454,14 +1454,14 @@ static void launch_single(const FusedArgs& fa, const FusedTuning& t, hipStream_t
                 const unsigned g1 = blocks_for(node_waves(ns, fa.K), t.wpb);
                 const unsigned g2 = blocks_for(node_waves(nd, fa.K), t.wpb);
                 if (stage == 3) {
-                    if (ns && DM == 6 && fa.K == 2)
+                    if (ns && DM == 6 && fa.K == 2 && t.pair)
                         hipLaunchKernelGGL((kl_pass2_all<DM, 2>), dim3(blocks_for(ns, t.wpb)), blk, 0,
                                            s, fa, ns);
                     else if (ns)
                         hipLaunchKernelGGL((kl_pass2<1, DM>), dim3(g1), blk, 0, s, fa, 0, ns);
                     if (nd) hipLaunchKernelGGL((kl_pass2<2, DM>), dim3(g2), blk, 0, s, fa, ns, nd);
                 } else {
-                    if (ns && DM == 6 && fa.K == 2)
+                    if (ns && DM == 6 && fa.K == 2 && t.pair)
                         hipLaunchKernelGGL((kl_pass3_all<DM, 2>), dim3(blocks_for(ns, t.wpb)), blk, 0,
                                            s, fa, ns);
                     else if (ns)

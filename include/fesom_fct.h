@@ -117,9 +117,13 @@ enum {
     FCT_TUNE_ADFH_EDGES = 4,        /* edges per wavefront in the adf_h pass: 1 or 2 (2)   */
     FCT_TUNE_XCD_CHUNK = 5,         /* workgroups per range of one XCD under FCT_TUNE_XCD:
                                        0 = one range per XCD, else 1..2^20 (default 0)    */
-    FCT_TUNE_W_EDGE_FLUX = 6        /* fct_compute_w: 1 = edge terms once per edge (the
+    FCT_TUNE_W_EDGE_FLUX = 6,       /* fct_compute_w: 1 = edge terms once per edge (the
                                        step's edge flux pass), 0 = per node from the
                                        triangles; same bits either way (default 1)        */
+    FCT_TUNE_PAIR_TRACERS = 7       /* nl > 64, two tracers, node degree <= 6: 1 = the
+                                       shallow columns' node passes take both tracers
+                                       in one wavefront, 0 = one wavefront per tracer
+                                       (default 1)                                         */
 };
 
 #define FCT_BATCH_SLOTS 4

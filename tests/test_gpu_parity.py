@@ -272,7 +272,8 @@ def test_degree_eight_nl70_depth_split(torch_dev):
 
 @pytest.mark.parametrize("knobs", [{"XCD": 8}, {"WAVES_PER_GROUP": 2, "XCD": 3},
                                    {"WAVES_PER_GROUP": 1, "VFLUX_EDGES": 2, "ADFH_EDGES": 1},
-                                   {"XCD": 8, "XCD_CHUNK": 16}, {"XCD": 3, "XCD_CHUNK": 1000}])
+                                   {"XCD": 8, "XCD_CHUNK": 16}, {"XCD": 3, "XCD_CHUNK": 1000},
+                                   {"PAIR_TRACERS": 0}])
 def test_launch_shapes_do_not_change_results(knobs, torch_dev):
     """Other launch shapes give the default launch's bits."""
     from fesom_jax_amd import _lib
@@ -290,7 +291,7 @@ def test_launch_shapes_do_not_change_results(knobs, torch_dev):
         got = _gpu_step(m, case, torch_dev, path="fused")
     finally:
         for k, v in (("XCD", 1), ("XCD_CHUNK", 0), ("WAVES_PER_GROUP", 4), ("VFLUX_EDGES", 4),
-                     ("ADFH_EDGES", 2)):
+                     ("ADFH_EDGES", 2), ("PAIR_TRACERS", 1)):
             plan.lib.fct_plan_set_tuning(plan.ptr, getattr(_lib, "FCT_TUNE_" + k), v)
     for a, b in zip(got, ref):
         assert np.array_equal(a, b)
