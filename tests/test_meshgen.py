@@ -66,6 +66,9 @@ def test_tiny_and_empty_meshes():
     assert np.all(m.edge_tri[:1, 1] >= 0) and np.all(m.edge_tri[1:, 1] < 0)
     with pytest.raises(ValueError, match="no ocean"):
         icosahedral_mesh(1, 4, seed=0, land_fraction=0.9, min_levels=2, polar_cap_deg=60.0)
+    # more degree-8 nodes than a 42-node sphere can hold without overlapping flips
+    with pytest.raises(ValueError, match="degree 8"):
+        icosahedral_mesh(1, 4, seed=0, degree8=100)
 
 
 def test_icosphere_counts():
