@@ -50,3 +50,15 @@ def test_plan_create_rejects_bad_mesh_without_gpu():
     rc = lib.fct_plan_create(None, ctypes.byref(out))
     assert rc == _lib.FCT_E_INVALID
     assert b"NULL" in lib.fct_last_error()
+
+
+def test_enum_constants_match_header():
+    """Every FCT_TUNE_*, FCT_PATH_* and FCT_E_* value in the header has the same value
+    in the Python binding."""
+    text = HEADER.read_text()
+    pairs = dict(re.findall(r"\b(FCT_(?:TUNE|PATH|E)_\w+)\s*=\s*(-?\d+)", text))
+    pairs.update(re.findall(r"#define\s+(FCT_PATH_BIT_\w+)\s+(\d+)", text))
+    assert any(k.startswith("FCT_TUNE_") for k in pairs)
+    for name, value in pairs.items():
+        assert hasattr(_lib, name), name
+        assert getattr(_lib, name) == int(value), name
