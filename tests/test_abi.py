@@ -58,6 +58,8 @@ def test_enum_constants_match_header():
     text = HEADER.read_text()
     pairs = dict(re.findall(r"\b(FCT_(?:TUNE|PATH|E)_\w+)\s*=\s*(-?\d+)", text))
     pairs.update(re.findall(r"#define\s+(FCT_PATH_BIT_\w+)\s+(\d+)", text))
+    pairs.update(re.findall(r"#define\s+(FCT_OK|FCT_E_\w+)\s+\(?(-?\d+)\)?", text))
+    assert "FCT_E_UNSUPPORTED" in pairs and "FCT_PATH_BIT_FUSED" in pairs
     assert any(k.startswith("FCT_TUNE_") for k in pairs)
     for name, value in pairs.items():
         assert hasattr(_lib, name), name
