@@ -33,6 +33,14 @@ _AB_OLD = -(0.5 + _EPS_AB)                # fesom_jax/tracer_adv.py:45
 _AB_NEW = (1.5 + _EPS_AB)                 # fesom_jax/tracer_adv.py:46
 _FCT_FLUX_EPS = 1e-16                     # fesom_jax/tracer_adv.py:192
 _FCT_BIG = 1e3                            # fesom_jax/tracer_adv.py:193
+# Coefficients of the two high-order reconstructions, named so that the pinning tests
+# can plant a wrong value (tests/test_oracle_pins.py): MFCT edge means
+# ``T1 + (2*diff + ...)/6`` (fesom_jax/tracer_adv.py:222-223) and QR4C interface means
+# ``T + (2*qc + qu)*(zbar - Z)/3`` (fesom_jax/tracer_adv.py:282-283).
+_MFCT_DIFF_W = 2.0
+_MFCT_DEN = 6.0
+_QR4C_QC_W = 2.0
+_QR4C_DEN = 3.0
 
 
 # --------------------------------------------------------------------------
@@ -232,8 +240,8 @@ def adv_flux_hor_ho(mesh, uv, helem, ttf, eud):
     a = a[:, None]
     edx = mesh.edge_dxdy[:, 0:1]
     edy = mesh.edge_dxdy[:, 1:2]
-    Tmean1 = T1 + (2.0 * diff + edx * a * g0 + edy * R_EARTH * g2) / 6.0
-    Tmean2 = T2 - (2.0 * diff + edx * a * g1 + edy * R_EARTH * g3) / 6.0
+    Tmean1 = T1 + (_MFCT_DIFF_W * diff + edx * a * g0 + edy * R_EARTH * g2) / _MFCT_DEN
+    Tmean2 = T2 - (_MFCT_DIFF_W * diff + edx * a * g1 + edy * R_EARTH * g3) / _MFCT_DEN
     av = np.abs(vflux)
     return -0.5 * ((vflux + av) * Tmean1 + (vflux - av) * Tmean2)
 
@@ -275,8 +283,8 @@ def adv_flux_ver_ho(mesh, w_e, ttf, Z3d=None, zbar3=None):
         qc = (Tm1 - T) / _sf(Z_m1 - Z_nz)
         qu = (T - Tp1) / _sf(Z_nz - Z_p1)
         qd = (Tm2 - Tm1) / _sf(Z_m2 - Z_m1)
-        Tmean1 = T + (2.0 * qc + qu) * (zb - Z_nz) / 3.0
-        Tmean2 = Tm1 + (2.0 * qc + qd) * (zb - Z_m1) / 3.0
+        Tmean1 = T + (_QR4C_QC_W * qc + qu) * (zb - Z_nz) / _QR4C_DEN
+        Tmean2 = Tm1 + (_QR4C_QC_W * qc + qd) * (zb - Z_m1) / _QR4C_DEN
         quad = -0.5 * (Tmean1 + Tmean2) * W * area
     k = np.arange(mesh.nl)[None, :]
     nzmin = (mesh.ulevels_nod2D.astype(np.int64) - 1)[:, None]
