@@ -110,6 +110,9 @@ def main():
                          "compute_w: vertical velocity from continuity on the plain mesh; "
                          "ice: sea-ice FCT of the three ice tracers on the plain mesh")
     ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--shuffle", type=int, default=None, metavar="SEED",
+                    help="plain: renumber nodes, triangles and edges at random (the plan's "
+                         "locality order must make up for a numbering without locality)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, run a 1 GiB read + 1 GiB write elementwise "
                          "kernel (the reference for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -150,8 +153,11 @@ def main():
 
     t0 = time.time()
     mesh = icosahedral_mesh(args.level, args.nl, seed=args.seed)
-    log(f"mesh icosa-{args.level}: N={mesh.nod2D} E={mesh.elem2D} D={mesh.edge2D} "
-        f"nl={mesh.nl} ({time.time() - t0:.1f}s)")
+    if args.shuffle is not None:
+        from fesom_jax_amd.meshgen import shuffled
+        mesh = shuffled(mesh, args.shuffle)
+    log(f"mesh icosa-{args.level}{' (shuffled numbering)' if args.shuffle is not None else ''}: "
+        f"N={mesh.nod2D} E={mesh.elem2D} D={mesh.edge2D} nl={mesh.nl} ({time.time() - t0:.1f}s)")
     t0 = time.time()
     kinds = ("T", "S", "T", "S")[:K]
     case = make_case_device(mesh, dev, seed=args.seed + 1, dt=args.dt, tracers=kinds)
@@ -240,7 +246,8 @@ def main():
         "value": value, "unit": "node-level updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"icosa-{args.level} x nl={mesh.nl}, {K} tracers, plain FCT step",
+        "config": {"workload": f"icosa-{args.level} x nl={mesh.nl}, {K} tracers, plain FCT step"
+                               + (", shuffled numbering" if args.shuffle is not None else ""),
                    "nodes": mesh.nod2D, "elements": mesh.elem2D, "edges": mesh.edge2D,
                    "levels": mesh.nl, "tracers": K, "updates_per_step": U, "dt": args.dt,
                    "parallelism": "single GPU"},

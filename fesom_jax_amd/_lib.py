@@ -33,6 +33,8 @@ FCT_TUNE_ADFH_EDGES = 4
 FCT_TUNE_XCD_CHUNK = 5
 FCT_TUNE_W_EDGE_FLUX = 6
 FCT_TUNE_PAIR_TRACERS = 7
+FCT_TUNE_ADFH_CHUNK = 8
+FCT_TUNE_ADFH_THREADS = 9
 FCT_BATCH_SLOTS = 4
 FCT_VDIFF_MAX_TRACERS = 8
 

@@ -18,7 +18,7 @@ PKG = HERE.parent
 ROOT = PKG.parent
 LIB = PKG / "libfesom_fct.so"
 SOURCES = ["fct_api.hip", "fct_phased.hip", "fct_fused.hip", "fct_adjoint.hip", "fct_vdiff.hip", "fct_ale.hip",
-           "fct_ice.hip"]
+           "fct_ice.hip", "fct_locality.hip"]
 ARCH = os.environ.get("FESOM_FCT_ARCH", "gfx950")
 
 

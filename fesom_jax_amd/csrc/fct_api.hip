@@ -306,6 +306,16 @@ int apply_tuning(FusedTuning& t, int knob, int value) {
                 return set_error(FCT_E_INVALID, "vflux edges per wave %d not in {2, 4}", value);
             t.vflux_epw = value;
             return FCT_OK;
+        case FCT_TUNE_ADFH_CHUNK:
+            if (value != 0 && value != 4 && value != 8)
+                return set_error(FCT_E_INVALID, "adf_h level chunk %d not in {0, 4, 8}", value);
+            t.adfh_kc = value;
+            return FCT_OK;
+        case FCT_TUNE_ADFH_THREADS:
+            if (value != 1 && value != 2 && value != 4)
+                return set_error(FCT_E_INVALID, "adf_h threads per edge %d not in {1, 2, 4}", value);
+            t.adfh_tpe = value;
+            return FCT_OK;
         case FCT_TUNE_ADFH_EDGES:
             if (value != 1 && value != 2)
                 return set_error(FCT_E_INVALID, "adf_h edges per wave %d not in {1, 2}", value);
@@ -586,13 +596,35 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
     p->host_edge_tri.assign(d->edge_tri, d->edge_tri + 2 * (size_t)D);
     p->host_updn.assign(d->edge_up_dn_tri, d->edge_up_dn_tri + 2 * (size_t)D);
     p->tab.dm = dm;
+    // locality order and edge patches of the adf_h patch kernel
+    LocalityHost loc;
+    if (dm) {
+        std::vector<unsigned char> has_updn(D);
+        for (int e = 0; e < D; ++e)
+            has_updn[e] = d->edge_up_dn_tri[2 * e] >= 0 && d->edge_up_dn_tri[2 * e + 1] >= 0;
+        if ((rc = build_locality(N, D, h_edges.data(), h_eoff.data(), h_ecsr.data(), h_udv.data(),
+                                 has_updn.data(), h_ekr.data(), h_rng.data(), loc))) {
+            free_plan(p);
+            return rc;
+        }
+        p->tab.ep.np = (int)loc.ptab.size();
+    }
+    // node passes for nl > 64: shallow columns, then deep ones, each in the locality
+    // order (patch by patch)
     std::vector<int> h_nlist;
     if (dm && L > 64) {
-        for (int n = 0; n < N; ++n)
+        for (int n : loc.node_order)
             if (d->nlevels_nod2D[n] - 1 < 64) h_nlist.push_back(n);
         p->tab.nshallow = (int)h_nlist.size();
-        for (int n = 0; n < N; ++n)
+        for (int n : loc.node_order)
             if (d->nlevels_nod2D[n] - 1 >= 64) h_nlist.push_back(n);
+    }
+    if (dm && D && ((rc = dev_upload(p, &p->tab.ep.ptab, loc.ptab.data(), loc.ptab.size())) ||
+                    (rc = dev_upload(p, &p->tab.ep.pnodes, loc.pnodes.data(), loc.pnodes.size())) ||
+                    (rc = dev_upload(p, &p->tab.ep.elist, loc.elist.data(), loc.elist.size())) ||
+                    (rc = dev_upload(p, &p->tab.ep.eslot, loc.eslot.data(), loc.eslot.size())))) {
+        free_plan(p);
+        return rc;
     }
     if (dm && ((rc = dev_upload(p, &p->tab.hdr, h_thdr.data(), h_thdr.size())) ||
                (!h_nlist.empty() && (rc = dev_upload(p, &p->tab.nlist, h_nlist.data(), h_nlist.size()))) ||
@@ -769,7 +801,11 @@ int fct_compute_w(fct_plan* p, const double* uv, const double* helem, double* w,
     return launch_compute_w(p->m, uv, helem, w, s);
 }
 
-int fct_impl_vert_diff(fct_plan* p, const fct_vdiff_args* a, void* stream) {
+}  // extern "C"
+
+namespace {
+
+int check_vdiff(const fct_plan* p, const fct_vdiff_args* a) {
     if (!p || !a) return set_error(FCT_E_INVALID, "NULL plan or args");
     int cur = -1;
     if (hipGetDevice(&cur) != hipSuccess || cur != p->device)
@@ -791,14 +827,14 @@ int fct_impl_vert_diff(fct_plan* p, const fct_vdiff_args* a, void* stream) {
                                  "T_new[%d] aliases tracer %d (only T_new[j] == T[j] is allowed)",
                                  j, i);
     }
-    if (!p->vd_scratch) {
-        int cus = 0;
-        HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
-        int grid = 0, rc;
-        if ((rc = vdiff_max_grid(cus, &grid))) return rc;
-        if ((rc = dev_alloc(p, &p->vd_scratch, vdiff_scratch_doubles(p->m.L, grid)))) return rc;
-        p->vd_grid = grid;
-    }
+    return FCT_OK;
+}
+
+// the solves of a checked call, tracers two at a time, on `grid` workgroups whose
+// scratch slots start at `scratch`
+int run_vdiff(const fct_plan* p, const fct_vdiff_args* a, int grid, double* scratch,
+              hipStream_t s) {
+    const int K = a->ntracers;
     for (int j0 = 0; j0 < K; j0 += 2) {
         VdiffCall c;
         c.K = std::min(2, K - j0);
@@ -816,10 +852,28 @@ int fct_impl_vert_diff(fct_plan* p, const fct_vdiff_args* a, void* stream) {
             c.floor_j = a->floor_tracer - j0;
             c.floor_v = a->floor_value;
         }
-        const int rc = launch_vdiff(p->m, c, p->vd_grid, p->vd_scratch, (hipStream_t)stream);
+        const int rc = launch_vdiff(p->m, c, grid, scratch, s);
         if (rc) return rc;
     }
     return FCT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fct_impl_vert_diff(fct_plan* p, const fct_vdiff_args* a, void* stream) {
+    int rc = check_vdiff(p, a);
+    if (rc) return rc;
+    if (!p->vd_scratch) {
+        int cus = 0;
+        HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
+        int grid = 0;
+        if ((rc = vdiff_max_grid(cus, &grid))) return rc;
+        if ((rc = dev_alloc(p, &p->vd_scratch, vdiff_scratch_doubles(p->m.L, grid)))) return rc;
+        p->vd_grid = grid;
+    }
+    return run_vdiff(p, a, p->vd_grid, p->vd_scratch, (hipStream_t)stream);
 }
 
 int fct_forward_phase(fct_plan* p, const fct_step_args* a, int32_t j, int32_t phase,

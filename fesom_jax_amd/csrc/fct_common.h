@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "fesom_fct.h"
 
 namespace fct {
@@ -53,6 +55,27 @@ struct DevMesh {
     const int4* udv;          // [2*D]
 };
 
+// Edge patches of the fused path. The plan groups the nodes into compact patches grown
+// breadth-first over the mesh graph (whatever the caller's numbering), and each patch
+// takes the edges whose first node it holds: at most FCT_EP edges and FCT_PU distinct
+// nodes (the edges' ends and their up/down triangle vertices) per patch. A patch
+// kernel loads the columns of its node list once per level chunk into LDS and runs
+// one edge per thread over the chunk's levels.
+constexpr int FCT_EP = 256;   // edges per patch (threads per workgroup)
+constexpr int FCT_PU = 168;   // node slots per patch
+struct EdgePatches {
+    int np = 0;                   // patches
+    const int4* ptab = nullptr;   // [np] (first task, node-list offset,
+                                  //  edges | nodes << 16, klo | khi << 8 | kg << 16):
+                                  //  [klo, khi) = the union of the patch edges' wet level
+                                  //  ranges, kg = the first level at which an edge reads
+                                  //  node-averaged gradients (khi if none does)
+    const int* pnodes = nullptr;  // node list of each patch
+    const int* elist = nullptr;   // [D] edge of each task, patch by patch
+    const int2* eslot = nullptr;  // [D] per task: LDS slots of (n1, n2, u_a, u_b) as bytes;
+                                  //  (d_a, d_b) as bytes | cu << 16 | cd << 18
+};
+
 // Per-node stencil table of the fused path (dm slots per node).
 struct NodeTab {
     int dm = 0;               // slots per node (6 or 8); 0 = mesh not supported
@@ -70,7 +93,22 @@ struct NodeTab {
     int nshallow = 0;
     // slots past a node's counts repeat its first slot, so a kernel can load every slot
     // without waiting for the counts
+    EdgePatches ep;           // edge patches of the adf_h patch kernel
 };
+
+// Host side of the locality order (fct_locality.hip): `node_order` lists the nodes
+// patch by patch; the edge-patch tables are as in EdgePatches. `udv` and `has_updn`
+// are the up/down triangle vertex table and whether the edge has both triangles, `utri`
+// the level range in which an edge uses its up/down triangles' gradients.
+struct LocalityHost {
+    std::vector<int> node_order;
+    std::vector<int4> ptab;
+    std::vector<int> pnodes, elist;
+    std::vector<int2> eslot;
+};
+int build_locality(int N, int D, const int2* edges, const int* eoff, const int* ecsr,
+                   const int4* udv, const unsigned char* has_updn, const int2* ekr,
+                   const int2* utri, LocalityHost& out);
 
 __device__ __forceinline__ bool in_layer(int2 r, int k) { return k >= r.x && k < r.y; }
 __device__ __forceinline__ bool in_iface(int2 r, int k) { return k >= r.x && k <= r.y; }
@@ -126,6 +164,10 @@ struct FusedTuning {
     int wpb = 4;        // wavefronts per workgroup (1, 2 or 4)
     int vflux_epw = 4;  // edges per wavefront in kf_vflux (2 or 4)
     int adfh_epw = 2;   // edges per wavefront in kf_adfh (1 or 2)
+    int adfh_kc = 8;    // adf_h pass: levels per chunk of the edge-patch kernel (4 or 8),
+                        // 0 = one wavefront per edge column (kf_adfh)
+    int adfh_tpe = 1;   // edge-patch kernel: threads per edge (1, 2 or 4), which split
+                        // the chunk's levels
     int w_vflux = 1;    // compute_w from the edge volume flux (1) or per node (0)
     int pair = 1;       // two tracers: shallow node passes take both in one wavefront
     // the remap as the kernels take it: XCD count in the low 8 bits, chunk above

@@ -223,8 +223,11 @@ __device__ __forceinline__ void vflux_group(const FusedArgs& a, int g) {
     int kmax = 0;
 #pragma unroll
     for (int q = 0; q < EPW; ++q) {
+        // edges in the plan's locality order where it has one (so that a triangle's
+        // three edges are computed close together in time, whatever the numbering)
         const int eq = g * EPW + q;
-        e[q] = eq < m.D ? eq : m.D - 1;
+        const int ec = eq < m.D ? eq : m.D - 1;
+        e[q] = a.tab.ep.elist ? ldc(a.tab.ep.elist, ec) : ec;
         et[q] = ldc(m.edge_tri, e[q]);
         kr[q] = eq < m.D ? ldc(a.ekr, e[q]) : make_int2(0, 0);
         c[q] = ldc(m.cross, e[q]);
@@ -599,6 +602,171 @@ __device__ __forceinline__ void adfh_group(const FusedArgs& a, int g, int j) {
             out[(size_t)e[q] * L + k] = r;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// adf_h over edge patches: KT tracers, KC levels per chunk, TPE threads per edge
+// ---------------------------------------------------------------------------
+// A workgroup takes one edge patch (the edges a compact node patch owns). For each chunk
+// of KC levels it loads (T, ttfAB) of the patch's node list into LDS, KC lanes per node
+// column so that every load instruction reads whole 8*KC-byte row segments, and the
+// patch's vflux likewise; then each edge's TPE threads compute its chunk levels from
+// LDS (the same expressions in the same order as adfh_group, so the same bits) and leave
+// adf_h in LDS, from where it is stored as row segments on the edge's wet levels. The
+// loads of the next chunk are issued before the current chunk is computed, so their
+// latency overlaps the arithmetic. Each node column of the patch is read from memory
+// once per chunk instead of once per incident edge; only the node-averaged gradients
+// (read on the few levels where an edge does not use its up/down triangles) are still
+// gathered per edge.
+template <int KT, int KC, int TPE>
+__device__ __forceinline__ void adfh_patch(const FusedArgs& a, int p, int j0) {
+    constexpr int NT = FCT_EP * TPE;              // threads
+    constexpr int PS = FCT_PU + 1;                // row pitches padded against bank conflicts
+    constexpr int ES = FCT_EP + 1;
+    constexpr int NI = (FCT_PU * KC + NT - 1) / NT;   // node-column items per thread
+    constexpr int NV = (FCT_EP * KC + NT - 1) / NT;   // edge-column items per thread
+    constexpr int LPT = KC / TPE;                 // chunk levels per thread
+    __shared__ double2 ta[KT][KC][PS];            // (T, ttfAB) of the node list
+    __shared__ double vo[KT][KC][ES];             // vflux in (tracer 0's rows), adf_h out
+    __shared__ int2 ekr_s[FCT_EP];
+    const DevMesh& m = a.m;
+    const int L = m.L;
+    const int4 pt = ldc(a.tab.ep.ptab, p);
+    const int first = pt.x, noff = pt.y, cnt = pt.z & 0xffff, U = pt.z >> 16;
+    const int klo = pt.w & 0xff, khi = (pt.w >> 8) & 0xff;
+    const int t = threadIdx.x, q = t % FCT_EP, kq = (t / FCT_EP) * LPT;
+    int2 kr = make_int2(0, 0), ur = kr, nd = kr, sl = kr;
+    double2 cc = make_double2(0.0, 0.0);
+    double gu[6] = {}, gd[6] = {};
+    if (q < cnt) {
+        const int e = G(a.tab.ep.elist)[first + q];
+        kr = ldv(a.ekr, e);
+        ur = ldv(m.utri_rng, e);
+        nd = ldv(m.edges, e);
+        sl = ldv(a.tab.ep.eslot, (size_t)first + q);
+        cc = ldv(m.edge_c, e);
+        if (ur.x < ur.y) {
+            const int2 ud = ldv(m.updn, e);
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                gu[i] = G(m.gs)[6 * (size_t)ud.x + i];
+                gd[i] = G(m.gs)[6 * (size_t)ud.y + i];
+            }
+        }
+        if (t < FCT_EP) ekr_s[q] = kr;
+    }
+    const int s1 = sl.x & 0xff, s2 = (sl.x >> 8) & 0xff;
+    const int sua = (sl.x >> 16) & 0xff, sub = (sl.x >> 24) & 0xff;
+    const int sda = sl.y & 0xff, sdb = (sl.y >> 8) & 0xff;
+    const int cu = (sl.y >> 16) & 3, cd = (sl.y >> 18) & 3;
+    // row offsets of this thread's items: node columns (slot s, level kk) and edge
+    // columns (edge q, level kk), i = t + r * NT, s or q = i / KC, kk = i % KC
+    size_t nrow[NI], erow[NV];
+#pragma unroll
+    for (int r = 0; r < NI; ++r) {
+        const int i = t + r * NT, s = i / KC;
+        nrow[r] = s < U ? (size_t)G(a.tab.ep.pnodes)[noff + s] * L : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < NV; ++r) {
+        const int i = t + r * NT, qq = i / KC;
+        erow[r] = qq < cnt ? (size_t)G(a.tab.ep.elist)[first + qq] * L : 0;
+    }
+    double pT[NI][KT], pO[NI][KT], pv[NV];
+#define FCT_PREFETCH(K0)                                                                    \
+    do {                                                                                    \
+        _Pragma("unroll") for (int r = 0; r < NI; ++r) {                                    \
+            const int i = t + r * NT, s = i / KC, k = (K0) + i % KC;                        \
+            const bool ok = s < U && k < L;                                                 \
+            _Pragma("unroll") for (int jj = 0; jj < KT; ++jj) {                             \
+                pT[r][jj] = ok ? G(a.T[j0 + jj])[nrow[r] + k] : 0.0;                        \
+                pO[r][jj] = ok ? G(a.Told[j0 + jj])[nrow[r] + k] : 0.0;                     \
+            }                                                                               \
+        }                                                                                   \
+        _Pragma("unroll") for (int r = 0; r < NV; ++r) {                                    \
+            const int i = t + r * NT, qq = i / KC, k = (K0) + i % KC;                       \
+            pv[r] = (qq < cnt && k < L) ? G(a.vflux)[erow[r] + k] : 0.0;                    \
+        }                                                                                   \
+    } while (0)
+    FCT_PREFETCH(klo);
+    for (int k0 = klo; k0 < khi; k0 += KC) {
+#pragma unroll
+        for (int r = 0; r < NI; ++r) {
+            const int i = t + r * NT, s = i / KC, kk = i % KC;
+            if (s < U) {
+#pragma unroll
+                for (int jj = 0; jj < KT; ++jj)
+                    ta[jj][kk][s] = make_double2(pT[r][jj], ab2(pO[r][jj], pT[r][jj]));
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < NV; ++r) {
+            const int i = t + r * NT, qq = i / KC, kk = i % KC;
+            if (qq < cnt) vo[0][kk][qq] = pv[r];
+        }
+        __syncthreads();
+        if (k0 + KC < khi) FCT_PREFETCH(k0 + KC);
+        if (q < cnt) {
+#pragma unroll
+            for (int kl = 0; kl < LPT; ++kl) {
+                const int kk = kq + kl, k = k0 + kk;
+                const double v = vo[0][kk][q];
+                const bool wet = k >= kr.x && k < kr.y;
+                const bool tri = k >= ur.x && k < ur.y;
+#pragma unroll
+                for (int jj = 0; jj < KT; ++jj) {
+                    double r = 0.0;
+                    if (wet) {
+                        const double2 e1 = ta[jj][kk][s1], e2 = ta[jj][kk][s2];
+                        const double T1 = e1.x, T2 = e2.x, A1 = e1.y, A2 = e2.y;
+                        double g0, g1, g2, g3;
+                        if (tri) {
+                            const double a0 = ta[jj][kk][sua].x, a1 = ta[jj][kk][sub].x;
+                            const double b0 = ta[jj][kk][sda].x, b1 = ta[jj][kk][sdb].x;
+                            const double u0 = cu == 0 ? T1 : a0;
+                            const double u1 = cu == 0 ? a0 : (cu == 1 ? T1 : a1);
+                            const double u2 = cu == 2 ? T1 : a1;
+                            const double d0 = cd == 0 ? T2 : b0;
+                            const double d1 = cd == 0 ? b0 : (cd == 1 ? T2 : b1);
+                            const double d2 = cd == 2 ? T2 : b1;
+                            g0 = gu[0] * u0 + gu[1] * u1 + gu[2] * u2;
+                            g2 = gu[3] * u0 + gu[4] * u1 + gu[5] * u2;
+                            g1 = gd[0] * d0 + gd[1] * d1 + gd[2] * d2;
+                            g3 = gd[3] * d0 + gd[4] * d1 + gd[5] * d2;
+                        } else {
+                            const double2* gn = a.gnode + (size_t)(j0 + jj) * m.N * L;
+                            const double2 ga = ldv(gn, (size_t)nd.x * L + k);
+                            const double2 gb = ldv(gn, (size_t)nd.y * L + k);
+                            g0 = ga.x; g1 = gb.x; g2 = ga.y; g3 = gb.y;
+                        }
+                        const double av = fabs(v);
+                        const double flo = -0.5 * (T1 * (v + av) + T2 * (v - av));
+                        const double diff = A2 - A1;
+                        const double Tm1 = A1 + (2.0 * diff + cc.x * g0 + cc.y * g2) / 6.0;
+                        const double Tm2 = A2 - (2.0 * diff + cc.x * g1 + cc.y * g3) / 6.0;
+                        const double ho = -0.5 * ((v + av) * Tm1 + (v - av) * Tm2);
+                        r = v != 0.0 ? ho - flo : 0.0;
+                    }
+                    vo[jj][kk][q] = r;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < NV; ++r) {
+            const int i = t + r * NT, qq = i / KC, kk = i % KC, k = k0 + kk;
+            if (qq < cnt) {
+                const int2 rr = ekr_s[qq];
+                if (k >= rr.x && k < rr.y) {
+#pragma unroll
+                    for (int jj = 0; jj < KT; ++jj)
+                        G(a.adfh + (size_t)(j0 + jj) * m.D * L)[erow[r] + k] = vo[jj][kk][qq];
+                }
+            }
+        }
+        __syncthreads();
+    }
+#undef FCT_PREFETCH
 }
 
 // ---------------------------------------------------------------------------
@@ -1306,6 +1474,14 @@ __global__ void FCT_OCC(8) kb_adfh(const FusedArgs* __restrict__ d,
     adfh_group<NS, EPW>(a, g, w - g * a.K);
 }
 
+// adf_h over edge patches; KT tracers from slot j0 of the launch's tracer chunk
+// at most 128 registers per thread
+template <int KT, int KC, int TPE>
+__global__ void __launch_bounds__(FCT_EP * TPE) __attribute__((amdgpu_waves_per_eu(4)))
+ke_adfh(FusedArgs a, int j0) {
+    adfh_patch<KT, KC, TPE>(a, (int)block_tile(a.remap), j0);
+}
+
 #define FCT_NODE_KERNELS(PASS, OCC)                                                     \
     template <int NS, int DM>                                                           \
     __global__ void FCT_OCC(DM == 6 ? OCC : 4) kf_##PASS(FusedArgs a) {                 \
@@ -1441,7 +1617,27 @@ static void launch_single(const FusedArgs& fa, const FusedTuning& t, hipStream_t
             }
             break;
         case 2:
-            if (t.adfh_epw == 1)
+            if (t.adfh_kc && fa.tab.ep.np) {
+                const dim3 g(fa.tab.ep.np);
+                for (int j = 0; j < fa.K; j += 2) {
+                    const bool two = fa.K - j >= 2;
+#define FCT_KE(KC, TPE)                                                                         \
+    do {                                                                                        \
+        if (two) hipLaunchKernelGGL((ke_adfh<2, KC, TPE>), g, dim3(FCT_EP * TPE), 0, s, fa, j); \
+        else hipLaunchKernelGGL((ke_adfh<1, KC, TPE>), g, dim3(FCT_EP * TPE), 0, s, fa, j);     \
+    } while (0)
+                    if (t.adfh_kc == 4) {
+                        if (t.adfh_tpe == 1) FCT_KE(4, 1);
+                        else if (t.adfh_tpe == 2) FCT_KE(4, 2);
+                        else FCT_KE(4, 4);
+                    } else {
+                        if (t.adfh_tpe == 1) FCT_KE(8, 1);
+                        else if (t.adfh_tpe == 2) FCT_KE(8, 2);
+                        else FCT_KE(8, 4);
+                    }
+#undef FCT_KE
+                }
+            } else if (t.adfh_epw == 1)
                 hipLaunchKernelGGL((kf_adfh<NS, 1>), dim3(blocks_for(adfh_waves(m.D, fa.K, t), t.wpb)), blk, 0, s, fa);
             else
                 hipLaunchKernelGGL((kf_adfh<NS, 2>), dim3(blocks_for(adfh_waves(m.D, fa.K, t), t.wpb)), blk, 0, s, fa);
@@ -1500,7 +1696,9 @@ static int launch_fused_t(FusedArgs& fa, const fct_step_args* a, int kw, const F
 
 static bool tuning_ok(const FusedTuning& t) {
     return (t.wpb == 1 || t.wpb == 2 || t.wpb == 4) && (t.vflux_epw == 2 || t.vflux_epw == 4) &&
-           (t.adfh_epw == 1 || t.adfh_epw == 2) && t.xcd >= 1;
+           (t.adfh_epw == 1 || t.adfh_epw == 2) && t.xcd >= 1 &&
+           (t.adfh_kc == 0 || t.adfh_kc == 4 || t.adfh_kc == 8) &&
+           (t.adfh_tpe == 1 || t.adfh_tpe == 2 || t.adfh_tpe == 4);
 }
 
 static void base_args(FusedArgs& fa, const DevMesh& m, const NodeTab& tab, const int2* ekr,

@@ -1,12 +1,16 @@
-"""Mesh bundles for the FCT path: the derivation of the 32 bundle arrays and an
-icosahedral-sphere generator that emits the same bundle.
+"""Mesh bundles for the FCT path: the derivation of the 32 bundle arrays from a
+triangulation in the rotated frame, and an icosahedral-sphere generator that emits the
+same bundle.
 
-``derive_bundle`` restates, vectorised, the derivation semantics of the reference's
-scripts/prepare_mesh.py:117-300 (orient clockwise, element-ascending CSR
-``nod_in_elem2D``, areas accumulated in that order, element centres with the
-min-anchored cyclic rule, ``gradient_sca``, edge geometry, the MFCT up/down triangle
-scan). It takes coordinates already in the rotated frame. It is checked against the
-reference's shipped pi bundle in tests/test_meshgen.py.
+``derive_bundle`` produces what the reference's mesh preparation produces
+(scripts/prepare_mesh.py:117-300 defines the bundle's semantics): clockwise triangles,
+node -> element lists in increasing element id, level tables, areas accumulated in that
+list order, element centroids with the cyclic rule, the linear-interpolant gradient
+operator, edge vectors, the smoothed resolution and the MFCT up/down triangles. It is
+checked against the shipped pi bundle in tests/test_meshgen.py (integers exactly, floats
+to 1e-15 of each array's largest value); the sector test of the up/down triangle search
+is kept in the same arithmetic as the reference's, since its decisions on a regular grid
+hang on exact ties.
 
 ``icosahedral_mesh`` builds a level-``n`` icosahedral triangulation of the sphere
 (``10*4**n + 2`` nodes), treats its coordinates as the rotated frame, carves land
@@ -30,208 +34,225 @@ from .mesh import Mesh, mesh_from_arrays
 PI_FULL = 3.14159265358979323846
 CYC = 2.0 * PI
 HALF_CYC = 0.5 * CYC
+# Euler angles (degrees) of the rotated frame about z, x, z
 ALPHA, BETA, GAMMA = 50.0, 15.0, -90.0
 
 
-def _trim(dx):
-    dx = np.where(dx > HALF_CYC, dx - CYC, dx)
-    return np.where(dx < -HALF_CYC, dx + CYC, dx)
+def _wrap(dlon):
+    """Longitude differences folded into [-half cycle, half cycle]."""
+    dlon = np.where(dlon > HALF_CYC, dlon - CYC, dlon)
+    return np.where(dlon < -HALF_CYC, dlon + CYC, dlon)
 
 
-def _rot_matrix():
-    al, be, ga = ALPHA * RAD, BETA * RAD, GAMMA * RAD
-    ca, sa, cb, sb, cg, sg = (np.cos(al), np.sin(al), np.cos(be),
-                              np.sin(be), np.cos(ga), np.sin(ga))
-    return np.array([
-        cg * ca - sg * cb * sa, cg * sa + sg * cb * ca, sg * sb,
-        -sg * ca - cg * cb * sa, -sg * sa + cg * cb * ca, cg * sb,
-        sb * sa, -sb * ca, cb])
+def _frame_rotation():
+    """3x3 matrix taking geographic unit vectors to the rotated frame: rotations by
+    ALPHA about z, BETA about the new x, GAMMA about the new z."""
+    def about_z(t):
+        c, s = math.cos(t), math.sin(t)
+        return np.array([[c, s, 0.0], [-s, c, 0.0], [0.0, 0.0, 1.0]])
+
+    def about_x(t):
+        c, s = math.cos(t), math.sin(t)
+        return np.array([[1.0, 0.0, 0.0], [0.0, c, s], [0.0, -s, c]])
+
+    return about_z(GAMMA * RAD) @ about_x(BETA * RAD) @ about_z(ALPHA * RAD)
 
 
 def r2g(rlon, rlat):
-    """Rotated -> geographic (lon, lat) radians (the transpose of the Euler rotation
-    with angles 50, 15, -90 degrees)."""
-    M = _rot_matrix()
-    xr, yr, zr = np.cos(rlat) * np.cos(rlon), np.cos(rlat) * np.sin(rlon), np.sin(rlat)
-    xg = M[0] * xr + M[3] * yr + M[6] * zr
-    yg = M[1] * xr + M[4] * yr + M[7] * zr
-    zg = M[2] * xr + M[5] * yr + M[8] * zr
-    glat = np.arcsin(np.clip(zg, -1.0, 1.0))
-    glon = np.where((yg == 0.0) & (xg == 0.0), 0.0, np.arctan2(yg, xg))
-    return glon, glat
+    """Rotated -> geographic (lon, lat) radians."""
+    rlon, rlat = np.asarray(rlon, np.float64), np.asarray(rlat, np.float64)
+    unit = np.stack([np.cos(rlat) * np.cos(rlon), np.cos(rlat) * np.sin(rlon), np.sin(rlat)])
+    x, y, z = _frame_rotation().T @ unit.reshape(3, -1)
+    lat = np.arcsin(np.clip(z, -1.0, 1.0))
+    lon = np.where((x == 0.0) & (y == 0.0), 0.0, np.arctan2(y, x))
+    return lon.reshape(rlon.shape), lat.reshape(rlat.shape)
 
 
-def _csr_slots(offsets, flat):
-    """Slot table of a CSR list: for slot j, the rows that have a j-th entry and that
-    entry's value (so a loop over slots visits each row's entries in stored order)."""
-    counts = np.diff(offsets)
-    out = []
-    for j in range(int(counts.max()) if counts.size else 0):
-        rows = np.nonzero(counts > j)[0]
-        out.append((rows, flat[offsets[rows] + j]))
-    return out
+class _Fans:
+    """Node -> element lists (increasing element id per node) of a triangulation, with
+    the list walked slot by slot: slot j holds the j-th element of every node that has
+    one, so a vectorised loop over slots visits each node's elements in list order."""
+
+    def __init__(self, tri: np.ndarray, N: int):
+        E = tri.shape[0]
+        node_of = tri.reshape(-1)
+        elem_of = np.repeat(np.arange(E, dtype=np.int64), 3)
+        order = np.lexsort((elem_of, node_of))
+        self.elems = elem_of[order].astype(np.int32)
+        self.offsets = np.concatenate([[0], np.cumsum(np.bincount(node_of, minlength=N))]).astype(np.int32)
+        sizes = np.diff(self.offsets)
+        self.slots = []
+        for j in range(int(sizes.max()) if N else 0):
+            nodes = np.nonzero(sizes > j)[0]
+            self.slots.append((nodes, self.elems[self.offsets[nodes] + j]))
+
+    def accumulate(self, out, value_of_elem):
+        """out[n] += value_of_elem(elements) over each node's list, in list order."""
+        for nodes, els in self.slots:
+            out[nodes] = out[nodes] + value_of_elem(els)
+        return out
 
 
-def _edge_x_between_bc(bx, by, cx, cy, xx, xy):
-    """Vectorised test whether direction x lies in the angular sector from c to b
+def _clockwise(lon, lat, tri_raw):
+    """Triangles with their last two vertices swapped where they run counter-clockwise
+    in the (lon, lat) plane."""
+    tri = np.array(tri_raw, dtype=np.int32, copy=True)
+    v0, v1, v2 = tri_raw[:, 0], tri_raw[:, 1], tri_raw[:, 2]
+    turn = (_wrap(lon[v1] - lon[v0]) * (lat[v2] - lat[v0])
+            - (lat[v1] - lat[v0]) * _wrap(lon[v2] - lon[v0]))
+    ccw = turn > 0.0
+    tri[ccw, 1], tri[ccw, 2] = tri_raw[ccw, 2], tri_raw[ccw, 1]
+    return tri
+
+
+def _triangle_geometry(lon, lat, tri):
+    """Per triangle: centroid (lon with the cyclic rule: vertices more than half a
+    cycle east of the westernmost one are moved a cycle west), the metric factor
+    cos(centroid lat), the area in m^2, and the six coefficients of the gradient of the
+    linear interpolant in metres (x along cos(lat) * lon, y along lat)."""
+    p = [lon[tri[:, i]] for i in range(3)]
+    q = [lat[tri[:, i]] for i in range(3)]
+    cy = (q[0] + q[1] + q[2]) / 3.0
+    cosc = np.cos(cy)
+    west = np.minimum(np.minimum(p[0], p[1]), p[2])
+    shifted = [np.where(pi - west >= HALF_CYC, pi - CYC, pi) for pi in p]
+    cx = (shifted[0] + shifted[1] + shifted[2]) / 3.0
+    # edge vectors from vertex 0 (radians, x scaled by the centroid's metric factor)
+    x1, x2 = _wrap(p[1] - p[0]) * cosc, _wrap(p[2] - p[0]) * cosc
+    y1, y2 = q[1] - q[0], q[2] - q[0]
+    area = 0.5 * np.abs(x1 * y2 - x2 * y1) * (R_EARTH ** 2)
+    # Cramer's rule for the gradient: the determinant of the clockwise metric triangle
+    # is -2 * area / R; w is R over it
+    w = -0.5 * R_EARTH / area
+    grad = np.stack([(y1 - y2) * w, y2 * w, -y1 * w,
+                     (x2 - x1) * w, -x2 * w, x1 * w], axis=1)
+    return cx, cy, cosc, area, grad
+
+
+def _edge_vectors(lon, lat, edges, edge_tri, cx, cy, cosc):
+    """edge_dxdy (radians, from the first node to the second) and edge_cross_dxdy:
+    metres from the edge midpoint to the centroid of each adjacent triangle (x scaled by
+    the triangle's metric factor), zero where the second triangle is missing."""
+    a, b = edges[:, 0], edges[:, 1]
+    dxdy = np.stack([_wrap(lon[b] - lon[a]), lat[b] - lat[a]], axis=1)
+    la, lb = lon[a], lon[b]
+    # midpoint longitude: the end that is more than half a cycle east moves a cycle west
+    la = np.where(la - lb > HALF_CYC, la - CYC, la)
+    lb = np.where(lon[a] - lb < -HALF_CYC, lb - CYC, lb)
+    mx, my = 0.5 * (la + lb), 0.5 * (lat[a] + lat[b])
+    cross = np.zeros((edges.shape[0], 4))
+    for side in (0, 1):
+        el = edge_tri[:, side]
+        has = el >= 0
+        e = np.where(has, el, 0)
+        cross[:, 2 * side] = np.where(has, _wrap(cx[e] - mx) * cosc[e] * R_EARTH, 0.0)
+        cross[:, 2 * side + 1] = np.where(has, (cy[e] - my) * R_EARTH, 0.0)
+    return dxdy, cross
+
+
+def _sector_holds(bx, by, cx, cy, dx, dy):
+    """Whether direction d lies in the angle from c to b (both rays included), tested
+    as the reference does: angles measured in c's frame, exact ties counted in
     (scripts/prepare_mesh.py:251-265)."""
-    cr = cx * cx + cy * cy
-    ok = cr != 0.0
-    crs = np.where(ok, cr, 1.0)
-    bxp = (bx * cx + by * cy) / crs
-    byp = (-bx * cy + by * cx) / crs
-    xxp = (xx * cx + xy * cy) / crs
-    xyp = (-xx * cy + xy * cx) / crs
-    ab = np.arctan2(byp, bxp)
-    axg = np.arctan2(xyp, xxp)
-    hit = ((ab > 0.0) & (axg > 0.0) & (axg < ab)) | ((ab < 0.0) & (axg < 0.0) & (axg > ab)) \
-        | (ab == axg) | (axg == 0.0)
-    return ok & hit
+    c2 = cx * cx + cy * cy
+    nz = c2 != 0.0
+    c2 = np.where(nz, c2, 1.0)
+    ang_b = np.arctan2((-bx * cy + by * cx) / c2, (bx * cx + by * cy) / c2)
+    ang_d = np.arctan2((-dx * cy + dy * cx) / c2, (dx * cx + dy * cy) / c2)
+    inside = np.where(ang_b > 0.0, (ang_d > 0.0) & (ang_d < ang_b),
+                      (ang_b < 0.0) & (ang_d < 0.0) & (ang_d > ang_b))
+    return nz & (inside | (ang_b == ang_d) | (ang_d == 0.0))
 
 
-def _up_dn_scan(node, xx, xy, en, rlon, rlat, nie, offsets):
-    """For each query (node, direction x) the first element of ``node``'s CSR fan
-    whose sector contains x, else -1 (scripts/prepare_mesh.py:268-291)."""
-    res = np.full(node.shape[0], -1, np.int32)
-    todo = np.ones(node.shape[0], bool)
-    deg = offsets[node + 1] - offsets[node]
-    for j in range(int(deg.max()) if deg.size else 0):
-        q = np.nonzero(todo & (deg > j))[0]
-        if q.size == 0:
-            break
-        nd = node[q]
-        el = nie[offsets[nd] + j]
-        tri = en[el]
-        js = np.where(tri[:, 0] == nd, 0, np.where(tri[:, 1] == nd, 1, 2))
-        jb = np.where(js == 0, 1, 0)
-        jc = np.where(js == 2, 1, 2)
-        nb = tri[np.arange(q.size), jb]
-        nc = tri[np.arange(q.size), jc]
-        bx = _trim(rlon[nb] - rlon[nd]); by = rlat[nb] - rlat[nd]
-        cx = _trim(rlon[nc] - rlon[nd]); cy = rlat[nc] - rlat[nd]
-        hit = _edge_x_between_bc(bx, by, cx, cy, xx[q], xy[q])
-        res[q[hit]] = el[hit]
-        todo[q[hit]] = False
-    return res
+def _upwind_triangles(lon, lat, tri, fans, edges):
+    """edge_up_dn_tri: for each end of each edge, the first triangle in the end node's
+    list whose angle at that node contains the direction pointing away from the other
+    end (the MFCT stencil's upwind triangle), -1 if none."""
+    a = edges[:, 0].astype(np.int64)
+    b = edges[:, 1].astype(np.int64)
+    dx, dy = _wrap(lon[b] - lon[a]), lat[b] - lat[a]
+    sizes = np.diff(fans.offsets)
+    found = []
+    for node, ux, uy in ((a, -dx, -dy), (b, dx, dy)):
+        res = np.full(node.shape[0], -1, np.int32)
+        open_q = np.arange(node.shape[0])
+        for j in range(int(sizes.max()) if sizes.size else 0):
+            open_q = open_q[sizes[node[open_q]] > j] if j else open_q
+            if open_q.size == 0:
+                break
+            nd = node[open_q]
+            el = fans.elems[fans.offsets[nd] + j]
+            t = tri[el]
+            at = np.argmax(t == nd[:, None], axis=1)
+            # the two other vertices in the order (next, then the one after)
+            nb = t[np.arange(t.shape[0]), np.where(at == 0, 1, 0)]
+            nc = t[np.arange(t.shape[0]), np.where(at == 2, 1, 2)]
+            hit = _sector_holds(_wrap(lon[nb] - lon[nd]), lat[nb] - lat[nd],
+                                _wrap(lon[nc] - lon[nd]), lat[nc] - lat[nd],
+                                ux[open_q], uy[open_q])
+            res[open_q[hit]] = el[hit]
+            open_q = open_q[~hit]
+        found.append(res)
+    return np.stack(found, axis=1).astype(np.int32)
 
 
 def derive_bundle(coord_rot, geo, coast, elem_nodes_raw, edges, edge_tri, nlevels,
                   nlevels_nod2D, zbar, depth):
-    """The 32 bundle arrays + meta from primitive mesh data (rotated coordinates),
-    following scripts/prepare_mesh.py:117-243."""
-    N = coord_rot.shape[0]
-    E = elem_nodes_raw.shape[0]
-    nl = zbar.shape[0]
-    rlon, rlat = coord_rot[:, 0], coord_rot[:, 1]
-
-    en = elem_nodes_raw.astype(np.int32).copy()
-    ax, ay = rlon[en[:, 0]], rlat[en[:, 0]]
-    bx = _trim(rlon[en[:, 1]] - ax); by = rlat[en[:, 1]] - ay
-    cx = _trim(rlon[en[:, 2]] - ax); cy = rlat[en[:, 2]] - ay
-    ccw = (bx * cy - by * cx) > 0.0
-    en[ccw, 1], en[ccw, 2] = elem_nodes_raw[ccw, 2], elem_nodes_raw[ccw, 1]
-    n0, n1, n2 = en[:, 0], en[:, 1], en[:, 2]
-
-    pair_node = en.reshape(-1)
-    pair_elem = np.repeat(np.arange(E, dtype=np.int32), 3)
-    order = np.argsort(pair_node, kind="stable")
-    nie = pair_elem[order].astype(np.int32)
-    counts = np.bincount(pair_node, minlength=N)
-    offsets = np.zeros(N + 1, np.int32)
-    offsets[1:] = np.cumsum(counts)
-    slots = _csr_slots(offsets.astype(np.int64), nie)
-
+    """The 32 bundle arrays and the meta scalars from a triangulation in the rotated
+    frame: node coordinates (radians), geographic coordinates, coast flags, triangles,
+    edges with their adjacent triangles, per-element and per-node bottom levels,
+    interface depths and node depths."""
+    N, E, nl = coord_rot.shape[0], elem_nodes_raw.shape[0], zbar.shape[0]
+    lon, lat = coord_rot[:, 0], coord_rot[:, 1]
+    tri = _clockwise(lon, lat, elem_nodes_raw)
+    fans = _Fans(tri, N)
     nlevels = nlevels.astype(np.int32)
     nlevels_nod2D = nlevels_nod2D.astype(np.int32)
-    ulevels = np.ones(E, np.int32)
-    ulevels_nod2D = np.ones(N, np.int32)
-    nmin = np.full(N, np.iinfo(np.int32).max, np.int64)
-    for rows, els in slots:
-        nmin[rows] = np.minimum(nmin[rows], nlevels[els])
-    nlevels_nod2D_min = np.minimum(nmin, nlevels_nod2D).astype(np.int32)
-    ul_max = np.ones(N, np.int32)
+    ones_e, ones_n = np.ones(E, np.int32), np.ones(N, np.int32)   # no cavities
 
-    lat0, lat1, lat2 = rlat[n0], rlat[n1], rlat[n2]
-    lon0, lon1, lon2 = rlon[n0], rlon[n1], rlon[n2]
-    ayc = np.cos((lat0 + lat1 + lat2) / 3.0)
-    aX = _trim(lon1 - lon0) * ayc; aly = lat1 - lat0
-    bX = _trim(lon2 - lon0) * ayc; bly = lat2 - lat0
-    elem_area = 0.5 * np.abs(aX * bly - bX * aly) * (R_EARTH ** 2)
+    # deepest-common bottom of each node's triangles (capped at the node's own bottom)
+    fan_min = np.minimum.reduceat(nlevels[fans.elems], fans.offsets[:-1]) if E else ones_n
+    nlevels_nod2D_min = np.minimum(fan_min, nlevels_nod2D).astype(np.int32)
 
-    third = elem_area / 3.0
+    cx, cy, cosc, elem_area, grad = _triangle_geometry(lon, lat, tri)
+
+    # node areas per level: a third of each wet triangle, summed in list order
     k = np.arange(nl)[None, :]
-    area = np.zeros((N, nl))
-    for rows, els in slots:
-        area[rows] = area[rows] + np.where(k < (nlevels[els].astype(np.int64) - 1)[:, None],
-                                           third[els][:, None], 0.0)
+    third = elem_area / 3.0
+    wet_third = lambda els: np.where(k < (nlevels[els].astype(np.int64) - 1)[:, None],
+                                     third[els][:, None], 0.0)
+    area = fans.accumulate(np.zeros((N, nl)), wet_third)
     areasvol = np.where(k < (nlevels_nod2D.astype(np.int64) - 1)[:, None], area, 0.0)
-    ocean_area = float(areasvol[ulevels_nod2D <= 1, 0].sum())
+    ocean_area = float(areasvol[ones_n <= 1, 0].sum())
 
-    axk = np.stack([lon0, lon1, lon2], axis=1)
-    amin = axk.min(axis=1, keepdims=True)
-    axk = np.where(axk - amin >= HALF_CYC, axk - CYC, axk)
-    elem_center_x = axk.sum(axis=1) / 3.0
-    elem_center_y = (lat0 + lat1 + lat2) / 3.0
-    elem_cos = np.cos(elem_center_y)
-    metric_factor = np.tan(elem_center_y) / R_EARTH
-    _, cglat = r2g(elem_center_x, elem_center_y)
-    coriolis = 2.0 * OMEGA * np.sin(cglat)
-    coriolis_node = 2.0 * OMEGA * np.sin(geo[:, 1])
+    _, glat_c = r2g(cx, cy)
+    dxdy, cross = _edge_vectors(lon, lat, edges, edge_tri, cx, cy, cosc)
 
-    dX31 = _trim(lon2 - lon0) * elem_cos; dX21 = _trim(lon1 - lon0) * elem_cos
-    dY31 = lat2 - lat0; dY21 = lat1 - lat0
-    df = -0.5 * R_EARTH / elem_area
-    gradient_sca = np.stack([(-dY31 + dY21) * df, (dY31) * df, (-dY21) * df,
-                             (dX31 - dX21) * df, (-dX31) * df, (dX21) * df], axis=1)
-
-    e0, e1 = edges[:, 0], edges[:, 1]
-    edge_dxdy = np.stack([_trim(rlon[e1] - rlon[e0]), rlat[e1] - rlat[e0]], axis=1)
-    eax, ebx = rlon[e0].copy(), rlon[e1].copy()
-    diff = rlon[e0] - rlon[e1]
-    eax = np.where(diff > HALF_CYC, eax - CYC, eax)
-    ebx = np.where(diff < -HALF_CYC, ebx - CYC, ebx)
-    emx = 0.5 * (eax + ebx); emy = 0.5 * (rlat[e0] + rlat[e1])
-    el1, el2 = edge_tri[:, 0], edge_tri[:, 1]
-    cross = np.zeros((edges.shape[0], 4))
-    bxe = _trim(elem_center_x[el1] - emx); bye = elem_center_y[el1] - emy
-    cross[:, 0] = bxe * elem_cos[el1] * R_EARTH; cross[:, 1] = bye * R_EARTH
-    has2 = el2 >= 0
-    el2s = np.where(has2, el2, 0)
-    bx2 = _trim(elem_center_x[el2s] - emx); by2 = elem_center_y[el2s] - emy
-    cross[:, 2] = np.where(has2, bx2 * elem_cos[el2s] * R_EARTH, 0.0)
-    cross[:, 3] = np.where(has2, by2 * R_EARTH, 0.0)
-
-    inv_pi = 1.0 / PI_FULL
-    mres = 2.0 * np.sqrt(areasvol[:, 0] * inv_pi)
+    # resolution: the diameter of a disc of the node's area, smoothed three times by
+    # the area-weighted mean over the node's triangles of their vertex means
+    res = 2.0 * np.sqrt(areasvol[:, 0] * (1.0 / PI_FULL))
     for _ in range(3):
-        mean3 = (mres[n0] + mres[n1] + mres[n2]) / 3.0
-        vals = mean3 * elem_area
-        acc = np.zeros(N); vol = np.zeros(N)
-        for rows, els in slots:
-            acc[rows] = acc[rows] + vals[els]
-            vol[rows] = vol[rows] + elem_area[els]
-        mres = np.where(vol > 0.0, acc / np.where(vol > 0.0, vol, 1.0), 0.0)
-
-    zbar_3d_n = np.where(k < nlevels_nod2D.astype(np.int64)[:, None], zbar[None, :], 0.0)
-
-    x0 = _trim(rlon[e1] - rlon[e0]); x1 = rlat[e1] - rlat[e0]
-    up = _up_dn_scan(e0.astype(np.int64), -x0, -x1, en, rlon, rlat, nie, offsets.astype(np.int64))
-    dn = _up_dn_scan(e1.astype(np.int64), x0, x1, en, rlon, rlat, nie, offsets.astype(np.int64))
-    edge_up_dn_tri = np.stack([up, dn], axis=1).astype(np.int32)
+        tri_mean = (res[tri[:, 0]] + res[tri[:, 1]] + res[tri[:, 2]]) / 3.0
+        weighted = tri_mean * elem_area
+        num = fans.accumulate(np.zeros(N), lambda els: weighted[els])
+        den = fans.accumulate(np.zeros(N), lambda els: elem_area[els])
+        res = np.where(den > 0.0, num / np.where(den > 0.0, den, 1.0), 0.0)
 
     arrays = {
         "coord_nod2D": coord_rot, "geo_coord_nod2D": geo, "coast_flag": coast,
         "nlevels_nod2D": nlevels_nod2D, "nlevels_nod2D_min": nlevels_nod2D_min,
-        "ulevels_nod2D": ulevels_nod2D, "ulevels_nod2D_max": ul_max, "depth": depth,
-        "mesh_resolution": mres, "coriolis_node": coriolis_node, "area": area,
-        "areasvol": areasvol, "zbar_3d_n": zbar_3d_n, "nod_in_elem2D_offsets": offsets,
-        "elem_nodes": en, "nlevels": nlevels, "ulevels": ulevels, "elem_area": elem_area,
-        "elem_cos": elem_cos, "metric_factor": metric_factor, "coriolis": coriolis,
-        "elem_center_x": elem_center_x, "elem_center_y": elem_center_y,
-        "gradient_sca": gradient_sca, "edges": edges.astype(np.int32),
-        "edge_tri": edge_tri.astype(np.int32), "edge_dxdy": edge_dxdy,
-        "edge_cross_dxdy": cross, "edge_up_dn_tri": edge_up_dn_tri,
-        "zbar": zbar, "Z": 0.5 * (zbar[:-1] + zbar[1:]), "nod_in_elem2D": nie,
+        "ulevels_nod2D": ones_n, "ulevels_nod2D_max": ones_n.copy(), "depth": depth,
+        "mesh_resolution": res, "coriolis_node": 2.0 * OMEGA * np.sin(geo[:, 1]),
+        "area": area, "areasvol": areasvol,
+        "zbar_3d_n": np.where(k < nlevels_nod2D.astype(np.int64)[:, None], zbar[None, :], 0.0),
+        "nod_in_elem2D_offsets": fans.offsets, "elem_nodes": tri, "nlevels": nlevels,
+        "ulevels": ones_e, "elem_area": elem_area, "elem_cos": cosc,
+        "metric_factor": np.tan(cy) / R_EARTH, "coriolis": 2.0 * OMEGA * np.sin(glat_c),
+        "elem_center_x": cx, "elem_center_y": cy, "gradient_sca": grad,
+        "edges": edges.astype(np.int32), "edge_tri": edge_tri.astype(np.int32),
+        "edge_dxdy": dxdy, "edge_cross_dxdy": cross,
+        "edge_up_dn_tri": _upwind_triangles(lon, lat, tri, fans, edges),
+        "zbar": zbar, "Z": 0.5 * (zbar[:-1] + zbar[1:]), "nod_in_elem2D": fans.elems,
     }
     meta = dict(nod2D=N, elem2D=E, edge2D=int(edges.shape[0]), nl=nl,
                 edge2D_in=int(np.sum((edge_tri[:, 0] >= 0) & (edge_tri[:, 1] >= 0))),
@@ -414,12 +435,7 @@ def icosahedral_mesh(level: int, nl: int = 70, *, seed: int = 0,
 
     # orient clockwise in the rotated plane (the derivation would swap anyway; doing
     # it here lets the edge orientation below rely on it)
-    en = faces.astype(np.int64).copy()
-    ax, ay = rlon[en[:, 0]], rlat[en[:, 0]]
-    bx = _trim(rlon[en[:, 1]] - ax); by = rlat[en[:, 1]] - ay
-    cx = _trim(rlon[en[:, 2]] - ax); cy = rlat[en[:, 2]] - ay
-    ccw = (bx * cy - by * cx) > 0.0
-    en[ccw, 1], en[ccw, 2] = faces[ccw, 2], faces[ccw, 1]
+    en = _clockwise(rlon, rlat, faces).astype(np.int64)
 
     # half-edges p->q of a clockwise triangle have the triangle on their right
     hp = np.concatenate([en[:, 0], en[:, 1], en[:, 2]])
@@ -484,3 +500,24 @@ def icosa_counts(level: int) -> tuple[int, int, int]:
     """(nodes, triangles, edges) of the full sphere at subdivision ``level``."""
     n = 10 * 4 ** level + 2
     return n, 20 * 4 ** level, 30 * 4 ** level
+
+
+def shuffled(mesh: Mesh, seed: int) -> Mesh:
+    """The same mesh with nodes, triangles and edges renumbered by seeded random
+    permutations (edge orientation and triangle vertex order kept), re-derived through
+    ``derive_bundle``. A caller's numbering carries no locality in general; this is the
+    test case for the plan's own locality order."""
+    rng = np.random.default_rng(seed)
+    N, E, D = mesh.nod2D, mesh.elem2D, mesh.edge2D
+    pn, pe, pd = rng.permutation(N), rng.permutation(E), rng.permutation(D)
+    new_node = np.empty(N, np.int64); new_node[pn] = np.arange(N)
+    new_elem = np.empty(E, np.int64); new_elem[pe] = np.arange(E)
+    tri = new_node[mesh.elem_nodes[pe]].astype(np.int32)
+    edges = new_node[mesh.edges[pd]].astype(np.int32)
+    et = mesh.edge_tri[pd]
+    edge_tri = np.where(et >= 0, new_elem[np.maximum(et, 0)], -1).astype(np.int32)
+    arrays, meta = derive_bundle(mesh.coord_nod2D[pn], mesh.geo_coord_nod2D[pn],
+                                 mesh.coast_flag[pn], tri, edges, edge_tri,
+                                 mesh.nlevels[pe], mesh.nlevels_nod2D[pn], np.asarray(mesh.zbar),
+                                 mesh.depth[pn])
+    return mesh_from_arrays(arrays, meta)

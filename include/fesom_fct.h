@@ -120,10 +120,16 @@ enum {
     FCT_TUNE_W_EDGE_FLUX = 6,       /* fct_compute_w: 1 = edge terms once per edge (the
                                        step's edge flux pass), 0 = per node from the
                                        triangles; same bits either way (default 1)        */
-    FCT_TUNE_PAIR_TRACERS = 7       /* nl > 64, two tracers, node degree <= 6: 1 = the
+    FCT_TUNE_PAIR_TRACERS = 7,      /* nl > 64, two tracers, node degree <= 6: 1 = the
                                        shallow columns' node passes take both tracers
                                        in one wavefront, 0 = one wavefront per tracer
                                        (default 1)                                         */
+    FCT_TUNE_ADFH_CHUNK = 8,        /* adf_h pass: 4 or 8 = edge-patch kernel (edge
+                                       patches with their node columns staged in LDS,
+                                       that many levels at a time), 0 = one wavefront
+                                       per edge column; same bits (default 8)             */
+    FCT_TUNE_ADFH_THREADS = 9       /* edge-patch adf_h kernel: threads per edge, 1, 2 or
+                                       4, splitting each chunk's levels (default 1)       */
 };
 
 #define FCT_BATCH_SLOTS 4

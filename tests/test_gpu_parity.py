@@ -273,7 +273,9 @@ def test_degree_eight_nl70_depth_split(torch_dev):
 @pytest.mark.parametrize("knobs", [{"XCD": 8}, {"WAVES_PER_GROUP": 2, "XCD": 3},
                                    {"WAVES_PER_GROUP": 1, "VFLUX_EDGES": 2, "ADFH_EDGES": 1},
                                    {"XCD": 8, "XCD_CHUNK": 16}, {"XCD": 3, "XCD_CHUNK": 1000},
-                                   {"PAIR_TRACERS": 0}])
+                                   {"PAIR_TRACERS": 0}, {"ADFH_CHUNK": 0},
+                                   {"ADFH_CHUNK": 4, "ADFH_THREADS": 2},
+                                   {"ADFH_CHUNK": 8, "ADFH_THREADS": 4}])
 def test_launch_shapes_do_not_change_results(knobs, torch_dev):
     """Other launch shapes give the default launch's bits."""
     from fesom_jax_amd import _lib
@@ -291,7 +293,8 @@ def test_launch_shapes_do_not_change_results(knobs, torch_dev):
         got = _gpu_step(m, case, torch_dev, path="fused")
     finally:
         for k, v in (("XCD", 1), ("XCD_CHUNK", 0), ("WAVES_PER_GROUP", 4), ("VFLUX_EDGES", 4),
-                     ("ADFH_EDGES", 2), ("PAIR_TRACERS", 1)):
+                     ("ADFH_EDGES", 2), ("PAIR_TRACERS", 1), ("ADFH_CHUNK", 8),
+                     ("ADFH_THREADS", 1)):
             plan.lib.fct_plan_set_tuning(plan.ptr, getattr(_lib, "FCT_TUNE_" + k), v)
     for a, b in zip(got, ref):
         assert np.array_equal(a, b)
@@ -342,3 +345,17 @@ def test_small_icosa_matches_loop_oracle(torch_dev):
                                       case["T_old"][j], dt=case["dt"])
             assert np.array_equal(got[j], want), j
             assert rel_err(got[j], loops, m.node_layer_mask) <= 1e-11, j
+
+
+@pytest.mark.parametrize("nl", [48, 70])
+def test_shuffled_numbering(nl, torch_dev):
+    """A mesh numbered at random (nodes, triangles, edges): the plan's own locality order
+    drives the kernels, and the results are the oracle's bits on that numbering (every
+    per-node sum still runs in the numbering's (edge, side) order)."""
+    from fesom_jax_amd.inputs import make_case
+    from fesom_jax_amd.meshgen import icosahedral_mesh, shuffled
+    m = shuffled(icosahedral_mesh(4, nl, seed=31, degree8=6), seed=32)
+    case = sharp_case(m, make_case(m, seed=33, dt=2700.0), 2.0)
+    outs = _gpu_step(m, case, torch_dev, path="fused")
+    for j, g in enumerate(outs):
+        assert _check(f"shuffled nl={nl} tracer {j}", g, _oracle(m, case, j), m) == 0
