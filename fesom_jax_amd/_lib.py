@@ -49,7 +49,7 @@ EXPORTED_SYMBOLS = ("fct_abi_version", "fct_last_error", "fct_plan_create",
                     "fct_batch_create", "fct_batch_set_tuning", "fct_batch_bind",
                     "fct_batch_launch", "fct_batch_graph_launch", "fct_batch_destroy",
                     "fct_backward", "fct_forward_upwind", "fct_impl_vert_diff",
-                    "fct_compute_w", "fct_ice_fct_solve")
+                    "fct_compute_w", "fct_ice_fct_solve", "fct_impl_vert_diff_static")
 
 
 class FctMeshDesc(ctypes.Structure):
@@ -159,6 +159,8 @@ def load(path: Path | None = None, *, require_torch: bool = True):
     lib.fct_compute_w.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
     lib.fct_impl_vert_diff.restype = c_int32
     lib.fct_impl_vert_diff.argtypes = [c_void_p, POINTER(FctVdiffArgs), c_void_p]
+    lib.fct_impl_vert_diff_static.restype = c_int32
+    lib.fct_impl_vert_diff_static.argtypes = [c_void_p, POINTER(FctVdiffArgs), c_void_p]
     lib.fct_backward.restype = c_int32
     lib.fct_backward.argtypes = [c_void_p, POINTER(FctStepArgs), POINTER(FctAdjointArgs),
                                  c_void_p]

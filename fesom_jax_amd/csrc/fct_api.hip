@@ -876,6 +876,27 @@ int fct_impl_vert_diff(fct_plan* p, const fct_vdiff_args* a, void* stream) {
     return run_vdiff(p, a, p->vd_grid, p->vd_scratch, (hipStream_t)stream);
 }
 
+int fct_impl_vert_diff_static(fct_plan* p, const fct_vdiff_args* a, void* stream) {
+    int rc = check_vdiff(p, a);
+    if (rc) return rc;
+    // workgroups: as many as the fixed scratch has slots for, at most the resident count
+    const size_t slot = vdiff_scratch_doubles(p->m.L, 1);
+    const size_t slots = (size_t)FCT_VDIFF_STATIC_BYTES / sizeof(double) / slot;
+    if (slots < 1)
+        return set_error(FCT_E_INVALID,
+                         "nl = %d needs %zu bytes of scratch per workgroup, more than the "
+                         "%d-byte static scratch", p->m.L, slot * sizeof(double),
+                         (int)FCT_VDIFF_STATIC_BYTES);
+    int cus = 0;
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
+    int grid = 0;
+    if ((rc = vdiff_max_grid(cus, &grid))) return rc;
+    if ((size_t)grid > slots) grid = (int)slots;
+    double* scr = nullptr;
+    if ((rc = vdiff_static_scratch(p->device, &scr))) return rc;
+    return run_vdiff(p, a, grid, scr, (hipStream_t)stream);
+}
+
 int fct_forward_phase(fct_plan* p, const fct_step_args* a, int32_t j, int32_t phase,
                       const fct_phase_bufs* b, void* stream) {
     int rc = check_args(p, a);

@@ -256,5 +256,8 @@ int vdiff_max_grid(int cus, int* out);
 size_t vdiff_scratch_doubles(int L, int grid);
 int launch_vdiff(const DevMesh& m, const VdiffCall& c, int max_grid, double* scratch,
                  hipStream_t s);
+// Device address of the fixed scratch of fct_impl_vert_diff_static on `device` (the
+// current device), looked up once per device.
+int vdiff_static_scratch(int device, double** out);
 
 }  // namespace fct

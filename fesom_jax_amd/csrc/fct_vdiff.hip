@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <vector>
 
 #include "fct_common.h"
 
@@ -436,6 +438,30 @@ int launch_vdiff(const DevMesh& m, const VdiffCall& c, int max_grid, double* scr
     const int grid = std::min(a.nblk, max_grid);
     return c.K == 1 ? vd_dispatch<1>(a, c.Z3d != nullptr, sw, grid, s)
                     : vd_dispatch<2>(a, c.Z3d != nullptr, sw, grid, s);
+}
+
+// The scratch of fct_impl_vert_diff_static: a fixed device array, so that call allocates
+// nothing; every GPU the library runs on holds its own copy until the process ends. Host
+// code never passes the array's name to a kernel (on the host it is only a stand-in for
+// the symbol): it uses the address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) double scratch[FCT_VDIFF_STATIC_BYTES / sizeof(double)];
+
+int vdiff_static_scratch(int device, double** out) {
+    static std::mutex mu;
+    static std::vector<double*> addr;   // per device ordinal
+    std::lock_guard<std::mutex> lock(mu);
+    if (device < 0) return set_error(FCT_E_INVALID, "device %d", device);
+    if ((int)addr.size() <= device) addr.resize((size_t)device + 1, nullptr);
+    if (!addr[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return set_error(FCT_E_HIP, "hipGetSymbolAddress(scratch) on device %d failed: %s",
+                             device, hipGetErrorString(e));
+        addr[device] = static_cast<double*>(ptr);
+    }
+    *out = addr[device];
+    return FCT_OK;
 }
 
 }  // namespace fct

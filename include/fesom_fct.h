@@ -251,6 +251,21 @@ typedef struct fct_vdiff_args {
 
 int fct_impl_vert_diff(fct_plan* plan, const fct_vdiff_args* args, void* stream);
 
+/* ---- implicit vertical diffusion that allocates nothing ----------------------------
+ * No counterpart in the reference. For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid it): the
+ * same arguments, checks and results as fct_impl_vert_diff, bit for bit, but the solver
+ * scratch is a fixed device array of FCT_VDIFF_STATIC_BYTES per GPU (reserved when the
+ * library is loaded, held until the process ends) instead of plan workspace, and the
+ * call makes no allocation. The workgroup count is capped to the array's slots
+ * (3 * nl * 64 doubles each), so at large nl the call may run below
+ * fct_impl_vert_diff's speed; an nl whose single slot exceeds the array is refused
+ * (FCT_E_INVALID). Nothing in the array needs to start at zero. Every call on a GPU
+ * shares that GPU's array and leaves its contents behind: calls that overlap on one GPU
+ * (other streams, other threads) race. */
+#define FCT_VDIFF_STATIC_BYTES (64 << 20)
+int fct_impl_vert_diff_static(fct_plan* plan, const fct_vdiff_args* args, void* stream);
+
 /* ---- reverse mode ------------------------------------------------------------------
  * Cotangents of one FCT step: for the cotangents g_T_new[j] of T_new[j], returns
  * g_T[j], g_T_old[j] and (summed over the tracers) g_uv and g_w_e, with the conventions
