@@ -20,10 +20,11 @@ from .tracer_adv import _pick_device, _ptr_array, _to_device, _torch, advect_fct
 
 
 def advect_fct_vjp(mesh, uv, w_e, helem, hnode, hnode_new, Ts, T_olds, g_T_news, *,
-                   dt=DT_DEFAULT, need_uv=True, need_w=True):
+                   dt=DT_DEFAULT, need_uv=True, need_w=True, plan=None):
     """Cotangents of one K-tracer FCT step. Returns ``(g_Ts, g_T_olds, g_uv, g_w_e)``
     (torch tensors on the device; ``g_uv`` / ``g_w_e`` summed over the tracers, or
-    None when not requested)."""
+    None when not requested). ``plan`` (an ``FctPlan`` of ``mesh`` on the tensors'
+    device) holds the adjoint workspace; by default the mesh's cached plan does."""
     torch = _torch()
     device = _pick_device(Ts[0], uv)
     N, E, L = int(mesh.nod2D), int(mesh.elem2D), int(mesh.nl)
@@ -43,7 +44,10 @@ def advect_fct_vjp(mesh, uv, w_e, helem, hnode, hnode_new, Ts, T_olds, g_T_news,
     gTo = [torch.empty(ns, **f64) for _ in T_t]
     g_uv = torch.empty((E, L, 2), **f64) if need_uv else None
     g_w = torch.empty(ns, **f64) if need_w else None
-    plan = get_plan(mesh, device)
+    if plan is None:
+        plan = get_plan(mesh, device)
+    elif plan.device != device or plan.nod2D != N or plan.nl != L:
+        raise ValueError("plan does not belong to this mesh and device")
     keep = [_ptr_array(T_t), _ptr_array(To_t), _ptr_array(g_t), _ptr_array(gT), _ptr_array(gTo)]
     P = lambda a: ctypes.cast(a, ctypes.POINTER(c_void_p))
     fwd = _lib.FctStepArgs(ntracers=len(T_t), dt=float(dt), uv=uv_t.data_ptr(),
@@ -53,7 +57,7 @@ def advect_fct_vjp(mesh, uv, w_e, helem, hnode, hnode_new, Ts, T_olds, g_T_news,
     adj = _lib.FctAdjointArgs(g_T_new=P(keep[2]), g_T=P(keep[3]), g_T_old=P(keep[4]),
                               g_uv=None if g_uv is None else g_uv.data_ptr(),
                               g_w_e=None if g_w is None else g_w.data_ptr())
-    with torch.cuda.device(device):
+    with torch.cuda.device(device), plan.on(torch.cuda.current_stream(device)):
         s = torch.cuda.current_stream(device).cuda_stream
         _lib.check(plan.lib.fct_backward(plan.ptr, ctypes.byref(fwd), ctypes.byref(adj),
                                          c_void_p(s)), "fct_backward")

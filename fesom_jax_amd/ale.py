@@ -27,7 +27,7 @@ def compute_w(mesh, uv, helem):
     he_t = _to_device(helem, "helem", device, (E, L))
     w = torch.empty((N, L), dtype=torch.float64, device=device)
     plan = get_plan(mesh, device)
-    with torch.cuda.device(device):
+    with torch.cuda.device(device), plan.on(torch.cuda.current_stream(device)):
         stream = torch.cuda.current_stream(device).cuda_stream
         _lib.check(plan.lib.fct_compute_w(plan.ptr, c_void_p(uv_t.data_ptr()),
                                           c_void_p(he_t.data_ptr()), c_void_p(w.data_ptr()),

@@ -49,8 +49,22 @@ class Instance:
         t = lambda a: (a if isinstance(a, torch.Tensor) else torch.from_numpy(a)).to(
             device=device, dtype=torch.float64).contiguous()
         K = len(case["T"])
+        N, E, L = int(mesh.nod2D), int(mesh.elem2D), int(mesh.nl)
+        want = {"uv": (E, L, 2), "w_e": (N, L), "helem": (E, L), "hnode": (N, L),
+                "hnode_new": (N, L)}
+        for k, shape in want.items():
+            if tuple(case[k].shape) != shape:
+                raise ValueError(f"{k} has shape {tuple(case[k].shape)}, expected {shape}")
+        if len(case["T_old"]) != K or K < 1:
+            raise ValueError("T and T_old must hold the same number (>= 1) of tracers")
+        for j, (T, To) in enumerate(zip(case["T"], case["T_old"])):
+            if tuple(T.shape) != (N, L) or tuple(To.shape) != (N, L):
+                raise ValueError(f"tracer {j} has shape {tuple(T.shape)} / {tuple(To.shape)}, "
+                                 f"expected {(N, L)}")
         if plan is None:
             plan = FctPlan(mesh, device, max_tracers=max_tracers or K)
+        elif plan.nod2D != N or plan.nl != L or plan.elem2D != E:
+            raise ValueError("plan was built for another mesh")
         bufs = []
         for T, To in zip(case["T"], case["T_old"]):
             Tt, Tot = t(T), t(To)

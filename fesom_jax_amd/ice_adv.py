@@ -49,7 +49,7 @@ def fct_solve(cfg, mesh, a_ice, m_ice, m_snow, u_ice, v_ice, exch=None):
                            v_ice=ins[4].data_ptr(), a_new=outs[0].data_ptr(),
                            m_new=outs[1].data_ptr(), ms_new=outs[2].data_ptr())
     plan = get_plan(mesh, device)
-    with torch.cuda.device(device):
+    with torch.cuda.device(device), plan.on(torch.cuda.current_stream(device)):
         stream = torch.cuda.current_stream(device).cuda_stream
         _lib.check(plan.lib.fct_ice_fct_solve(plan.ptr, ctypes.byref(args), c_void_p(stream)),
                    "fct_ice_fct_solve")

@@ -36,6 +36,9 @@ class RequestSpec:
     nsteps: int
     adjoint: bool
     seed: int
+    # test hook: "nan" puts a NaN into the request's first tracer, "fail" makes its
+    # setup fail (an argument the library refuses); None for ordinary requests
+    inject: str | None = None
 
 
 def make_stream(rate, duration, *, levels=(4, 7), nls=(47, 70), ks=(2, 3, 4), steps=(1, 24),
@@ -72,19 +75,52 @@ class MeshCache:
         return self.meshes[key]
 
 
-def estimate_bytes(mesh, K, nsteps, adjoint):
-    """Device bytes a request holds: plan topology and fused workspace, fields, and
-    (adjoint) the stored trajectory plus the adjoint workspace."""
+def estimate_plan_bytes(mesh, K, adjoint):
+    """Device bytes of a plan for ``mesh`` with ``K`` tracers: topology, the fused
+    workspace and, once it has run an adjoint, the phased and adjoint workspace (an
+    upper estimate; the server replaces it by the plan's own count once the plan
+    exists)."""
     N, E, D, L = mesh.nod2D, mesh.elem2D, mesh.edge2D, mesh.nl
     NL, EL, DL = N * L, E * L, D * L
     topo = 200 * N + 200 * E + 150 * D + 8 * 4 * NL     # + area, areasvol and their pairs
     work = 8 * (DL + K * (6 * NL + DL))
-    fields = 8 * (3 * EL + 3 * NL + 3 * K * NL)
-    total = topo + work + fields
+    total = topo + work
     if adjoint:
-        total += 8 * (nsteps + 1) * 2 * K * NL            # trajectory (T, T_old)
+        # phased tape, cotangent scratch, element -> edge lists
         total += 8 * (12 * NL + 2 * EL + DL) + 8 * (20 * NL + 10 * DL + 2 * EL)
+        total += 16 * D + 8 * E + 64
     return int(total)
+
+
+def estimate_field_bytes(mesh, K, nsteps, adjoint):
+    """Device bytes of a request's own fields: flow, three buffers per tracer and
+    (adjoint) the checkpointed states and one recomputed segment of them."""
+    N, E, L = mesh.nod2D, mesh.elem2D, mesh.nl
+    NL, EL = N * L, E * L
+    total = 8 * (3 * EL + 3 * NL + 3 * K * NL)
+    if adjoint:
+        # checkpoints and one recomputed segment of states (T, T_old)
+        total += 8 * (2 * checkpoint_stride(nsteps) + 1) * 2 * K * NL
+    return int(total)
+
+
+def checkpoint_stride(nsteps):
+    """Steps between stored states of an adjoint request: about sqrt(nsteps), so the
+    stored checkpoints and the one segment recomputed at a time both hold about
+    sqrt(nsteps) states."""
+    return max(1, int(np.ceil(np.sqrt(nsteps))))
+
+
+def estimate_base_bytes(mesh):
+    """Device bytes of a mesh's base inputs (flow, layer thicknesses and four tracers
+    with their previous values), built once per mesh and shared by its requests."""
+    NL, EL = mesh.nod2D * mesh.nl, mesh.elem2D * mesh.nl
+    return int(8 * (3 * EL + 3 * NL + 8 * NL))
+
+
+def estimate_bytes(mesh, K, nsteps, adjoint):
+    """Device bytes a request needs when it brings its own plan."""
+    return estimate_plan_bytes(mesh, K, adjoint) + estimate_field_bytes(mesh, K, nsteps, adjoint)
 
 
 @dataclass
@@ -97,10 +133,26 @@ class _Active:
     bytes: int
     t_admit: float
     setup_s: float
-    traj: list = field(default_factory=list)
+    # adjoint requests: states (T, T_old) at every `stride`-th step (checkpoints); the
+    # backward recomputes the states in between, one segment at a time
+    ckpt: dict = field(default_factory=dict)
+    stride: int = 1
+    done_steps: int = 0
 
 
 class Server:
+    """Continuous-batching loop; see module docstring.
+
+    Memory: the budget covers every request's own fields and every plan the server
+    holds, active or idle in the pool, at the plan's own byte count (its adjoint
+    workspace included once an adjoint has run on it). Idle plans are released when
+    an admission would not fit otherwise.
+
+    Failures stay with their request: a request whose setup raises, whose batched step
+    fails (found by stepping the batch's instances one at a time), or whose result or
+    gradients are not finite on the wet lanes is reported with an ``"error"`` entry and
+    leaves; the others carry on."""
+
     def __init__(self, device, *, budget_bytes=200 << 30, max_active=64, meshes=None,
                  dt=2700.0, chunk=4):
         self.device = device
@@ -110,39 +162,128 @@ class Server:
         self.meshes = meshes or MeshCache()
         self.dt = dt
         self.pool = collections.defaultdict(list)   # (level, nl, K) -> idle plans
+        self.plan_bytes = {}                        # id(plan) -> device bytes it holds
+        self.adjoint_plans = set()                  # ids of plans that ran an adjoint
+        self.base_bytes = {}                        # (level, nl) -> base input bytes
         self.results = {}
         self.latency = {}
         self.setup = {}
+        self.errors = {}
         self.active = []
+        self.peak_bytes = 0
+        self.peak_detail = None
+        self.timing = collections.Counter()         # seconds per activity (host clock)
+        self._finishing = []
+
+    # -- memory accounting ---------------------------------------------------------
+    def held_bytes(self, active):
+        """Bytes the server holds: the fields of the active and the finishing requests
+        (with the adjoint workspace an adjoint request will add to its plan), the
+        meshes' base inputs and every live plan."""
+        return (sum(a.bytes for a in active) + sum(f[0].bytes for f in self._finishing)
+                + sum(self.plan_bytes.values()) + sum(self.base_bytes.values()))
+
+    def _need(self, s):
+        """Bytes an admission of ``s`` adds: its fields, its mesh's base inputs the first
+        time, a plan unless one is idle, and the adjoint workspace if its plan has not
+        run an adjoint yet."""
+        m = self.meshes.get(s.level, s.nl)
+        key = (s.level, s.nl, s.K)
+        need = estimate_field_bytes(m, s.K, s.nsteps, s.adjoint)
+        if (s.level, s.nl) not in self.base_bytes:
+            need += estimate_base_bytes(m)
+        if not self.pool[key]:
+            return need + estimate_plan_bytes(m, s.K, s.adjoint)
+        if s.adjoint and id(self.pool[key][-1]) not in self.adjoint_plans:
+            need += estimate_plan_bytes(m, s.K, True) - estimate_plan_bytes(m, s.K, False)
+        return need
+
+    def _note_peak(self, active):
+        held = self.held_bytes(active)
+        if held > self.peak_bytes:
+            self.peak_bytes = held
+            self.peak_detail = {
+                "fields": sum(a.bytes for a in active), "active": len(active),
+                "plans": {k: sum(self.plan_bytes.get(id(p), 0) for p in ps)
+                          for k, ps in self.pool.items() if ps},
+                "active_plans": sum(self.plan_bytes.get(id(a.inst.plan), 0) for a in active),
+                "all_plans": sum(self.plan_bytes.values()), "base": dict(self.base_bytes)}
+
+    def _refresh(self, plan):
+        self.plan_bytes[id(plan)] = plan.device_bytes()
+
+    def _release_idle(self, need, active, busy=()):
+        """Close idle plans (largest first), then drop the base inputs of meshes no
+        active or finishing request uses (they are rebuilt when needed), until ``need``
+        more bytes fit; returns whether they do."""
+        idle = sorted(((self.plan_bytes[id(p)], k, p) for k, ps in self.pool.items() for p in ps),
+                      key=lambda x: -x[0])
+        for nbytes, key, plan in idle:
+            if self.held_bytes(active) + need <= self.budget:
+                return True
+            self.pool[key].remove(plan)
+            del self.plan_bytes[id(plan)]
+            self.adjoint_plans.discard(id(plan))
+            plan.close()
+        in_use = {(a.spec.level, a.spec.nl) for a in list(active) + list(busy)}
+        for key in sorted(self.base_bytes, key=lambda k: -self.base_bytes[k]):
+            if self.held_bytes(active) + need <= self.budget:
+                break
+            if key in in_use:
+                continue
+            _drop_base_case(self.meshes.get(*key), self.device, self.dt)
+            del self.base_bytes[key]
+        return self.held_bytes(active) + need <= self.budget
 
     # -- request life cycle --------------------------------------------------------
     def _plan(self, mesh, spec):
         from .tracer_adv import FctPlan
+        import torch
         key = (spec.level, spec.nl, spec.K)
         if self.pool[key]:
             return self.pool[key].pop()
-        return FctPlan(mesh, self.device, max_tracers=spec.K)
+        # memory torch caches but no tensor uses would otherwise stay out of the plan's
+        # reach
+        torch.cuda.empty_cache()
+        t = time.perf_counter()
+        plan = FctPlan(mesh, self.device, max_tracers=spec.K)
+        self._refresh(plan)
+        self.timing["plan_create"] += time.perf_counter() - t
+        return plan
 
     def _start(self, spec, now):
         from .batch import Instance
         t = time.perf_counter()
         mesh = self.meshes.get(spec.level, spec.nl)
+        self.base_bytes.setdefault((spec.level, spec.nl), estimate_base_bytes(mesh))
+        tc = time.perf_counter()
         case = request_case(mesh, spec, self.device, self.dt)
-        inst = Instance.from_case(mesh, case, self.device, plan=self._plan(mesh, spec))
+        self.timing["inputs"] += time.perf_counter() - tc
+        plan = self._plan(mesh, spec)
+        try:
+            inst = Instance.from_case(mesh, case, self.device, plan=plan)
+        except Exception:
+            self.pool[(spec.level, spec.nl, spec.K)].append(plan)
+            raise
         extra = {"uv": case["uv"], "target": case.get("target")}
         import torch
         torch.cuda.synchronize(self.device)
+        nbytes = estimate_field_bytes(mesh, spec.K, spec.nsteps, spec.adjoint)
+        if spec.adjoint and id(plan) not in self.adjoint_plans:
+            # the plan grows by its adjoint workspace when the request finishes
+            nbytes += estimate_plan_bytes(mesh, spec.K, True) - estimate_plan_bytes(mesh, spec.K, False)
         a = _Active(spec=spec, inst=inst, mesh=mesh, case_extra=extra, remaining=spec.nsteps,
-                    bytes=estimate_bytes(mesh, spec.K, spec.nsteps, spec.adjoint),
-                    t_admit=now, setup_s=time.perf_counter() - t)
+                    bytes=nbytes, t_admit=now, setup_s=time.perf_counter() - t)
         if spec.adjoint:
-            a.traj.append(([x.clone() for x in inst.T()], [x.clone() for x in inst.T_old()]))
+            a.stride = checkpoint_stride(spec.nsteps)
+            a.ckpt[0] = ([x.clone() for x in inst.T()], [x.clone() for x in inst.T_old()])
         return a
 
     def _finish_async(self, a, side):
-        """Queue the request's finishing work (adjoint, download into pinned host
-        memory) on the side stream after the steps queued so far; returns the event
-        that marks its completion and the host buffers."""
+        """Queue the request's finishing work (adjoint on the request's own plan,
+        finite check, download into pinned host memory) on the side stream after the
+        steps queued so far; returns the event that marks its completion, the host
+        buffers and the device tensors they come from."""
         import torch
         ev = torch.cuda.Event()
         ev.record(torch.cuda.current_stream(self.device))
@@ -151,6 +292,12 @@ class Server:
             out = {"T": [x for x in a.inst.T()]}
             if a.spec.adjoint:
                 out.update(adjoint_of_rollout(a, self.dt, to_host=False))
+            wet = torch.from_numpy(a.mesh.node_layer_mask).to(self.device)
+            bad = torch.zeros((), dtype=torch.bool, device=self.device)
+            for x in out["T"] + out.get("g_T0", []):
+                bad |= ~torch.isfinite(x[wet]).all()
+            if "g_uv" in out:
+                bad |= ~torch.isfinite(out["g_uv"]).all()
             host = {}
             for k, v in out.items():
                 vs = v if isinstance(v, list) else [v]
@@ -158,11 +305,37 @@ class Server:
                 for h, x in zip(hs, vs):
                     h.copy_(x, non_blocking=True)
                 host[k] = hs if isinstance(v, list) else hs[0]
+            host["_bad"] = torch.empty((), dtype=torch.bool, pin_memory=True)
+            host["_bad"].copy_(bad, non_blocking=True)
             done = torch.cuda.Event()
             done.record(side)
-        # the trajectory and the request's buffers stay referenced (through `a`) until
+        # the checkpoints and the request's buffers stay referenced (through `a`) until
         # `done` has completed, so the allocator cannot hand them out earlier
         return done, host, out
+
+    def _fail(self, spec, t0, msg, log=None):
+        self.errors[spec.rid] = msg
+        self.results[spec.rid] = {"error": msg}
+        self.latency[spec.rid] = time.perf_counter() - t0 - spec.arrival
+        if log:
+            log(f"req {spec.rid} failed: {msg}")
+
+    def _step_isolating(self, batch, active, graph, t0, log):
+        """One batched step; if it fails, step the instances one at a time so that only
+        the failing ones leave (their state is not advanced further)."""
+        try:
+            batch.step(graph=graph)
+            return list(active), []
+        except RuntimeError as exc:
+            first = str(exc)
+        ok, bad = [], []
+        for a in active:
+            try:
+                a.inst.step_alone()
+                ok.append(a)
+            except RuntimeError as exc:
+                bad.append((a, f"step failed: {exc} (batched launch: {first})"))
+        return ok, bad
 
     # -- the loop --------------------------------------------------------------------
     def run(self, specs, *, graph=False, log=None):
@@ -172,10 +345,11 @@ class Server:
         import torch
         from .batch import FctBatch
         pending = collections.deque(sorted(specs, key=lambda s: s.arrival))
-        active, used = [], 0
+        active = []
         self.active = active
         batch, changed, retired = None, False, []
-        finishing = []
+        finishing = self._finishing
+        finishing.clear()
         side = torch.cuda.Stream(self.device)      # default priority
         t0 = time.perf_counter()
         busy = 0.0
@@ -184,20 +358,26 @@ class Server:
             now = time.perf_counter() - t0
             while (pending and pending[0].arrival <= now and len(active) < self.max_active):
                 s = pending[0]
-                m = self.meshes.get(s.level, s.nl)
-                need = estimate_bytes(m, s.K, s.nsteps, s.adjoint)
-                if used + need > self.budget and active:
+                need = self._need(s)
+                if (not self._release_idle(need, active, [f[0] for f in finishing])
+                        and (active or finishing)):
                     break
                 pending.popleft()
-                a = self._start(s, now)
+                try:
+                    a = self._start(s, now)
+                except Exception as exc:          # setup failure stays with its request
+                    self._fail(s, t0, f"setup failed: {exc}", log)
+                    continue
                 self.setup[s.rid] = a.setup_s
-                used += a.bytes
                 active.append(a)
+                self._note_peak(active)
                 changed = True
             if not active:
                 if finishing:
                     # nothing to step: wait for the oldest finishing request
+                    tt = time.perf_counter()
                     finishing[0][1].synchronize()
+                    self.timing["wait_finishing"] += time.perf_counter() - tt
                 elif pending:
                     wait = pending[0].arrival - (time.perf_counter() - t0)
                     if wait > 0:
@@ -205,46 +385,77 @@ class Server:
             else:
                 tb = time.perf_counter()
                 if changed:
+                    tt = time.perf_counter()
                     if batch is not None:
                         retired.append(batch)     # closed once the GPU is idle
                     batch = FctBatch([a.inst for a in active])
                     batch.prepare()
                     changed = False
+                    self.timing["batch_build"] += time.perf_counter() - tt
                 # steps until the next completion; at most `chunk` while requests wait
                 nsteps = min(a.remaining for a in active)
                 if pending:
                     nsteps = min(nsteps, self.chunk)
+                tt = time.perf_counter()
                 for _ in range(nsteps):
-                    batch.step(graph=graph)
-                    for a in active:
+                    stepped, failed = self._step_isolating(batch, active, graph, t0, log)
+                    for a, msg in failed:
+                        active.remove(a)
+                        self._fail(a.spec, t0, msg, log)
+                        self.pool[(a.spec.level, a.spec.nl, a.spec.K)].append(a.inst.plan)
+                        changed = True
+                    if failed or len(stepped) != len(active):
+                        # the batch's instances moved one at a time: rebind it
+                        changed = True
+                    for a in stepped:
                         a.remaining -= 1
+                        a.done_steps += 1
                         updates += a.inst.updates
-                        if a.spec.adjoint:
-                            a.traj.append(([x.clone() for x in a.inst.T()],
-                                           [x.clone() for x in a.inst.T_old()]))
+                        if a.spec.adjoint and a.done_steps % a.stride == 0 and a.remaining:
+                            a.ckpt[a.done_steps] = ([x.clone() for x in a.inst.T()],
+                                                    [x.clone() for x in a.inst.T_old()])
+                    if changed or not active:
+                        break
+                self.timing["steps_queued"] += time.perf_counter() - tt
                 done = [a for a in active if a.remaining == 0]
                 for a in done:
                     # the instance leaves the batch; its finishing work runs on the side
                     # stream while the remaining requests keep stepping
-                    ev, host, keep = self._finish_async(a, side)
-                    finishing.append((a, ev, host, keep))
                     active.remove(a)
                     changed = True
+                    try:
+                        tt = time.perf_counter()
+                        ev, host, keep = self._finish_async(a, side)
+                        self.timing["finish_queued"] += time.perf_counter() - tt
+                    except (RuntimeError, MemoryError) as exc:
+                        side.synchronize()
+                        self._fail(a.spec, t0, f"finishing failed: {exc}", log)
+                        self.pool[(a.spec.level, a.spec.nl, a.spec.K)].append(a.inst.plan)
+                        self._refresh(a.inst.plan)
+                        continue
+                    finishing.append((a, ev, host, keep))
                 busy += time.perf_counter() - tb
             still = []
             for a, ev, host, keep in finishing:
                 if ev.query():
-                    self.results[a.spec.rid] = {
-                        k: ([h.numpy() for h in v] if isinstance(v, list) else v.numpy())
-                        for k, v in host.items()}
-                    self.latency[a.spec.rid] = time.perf_counter() - t0 - a.spec.arrival
-                    used -= a.bytes
+                    bad = bool(host.pop("_bad"))
+                    if bad:
+                        self._fail(a.spec, t0, "non-finite result", log)
+                    else:
+                        self.results[a.spec.rid] = {
+                            k: ([h.numpy() for h in v] if isinstance(v, list) else v.numpy())
+                            for k, v in host.items()}
+                        self.latency[a.spec.rid] = time.perf_counter() - t0 - a.spec.arrival
+                    self._refresh(a.inst.plan)     # the adjoint may have grown it
+                    if a.spec.adjoint:
+                        self.adjoint_plans.add(id(a.inst.plan))
                     self.pool[(a.spec.level, a.spec.nl, a.spec.K)].append(a.inst.plan)
-                    if log:
+                    if log and not bad:
                         log(f"req {a.spec.rid} done in {self.latency[a.spec.rid]:.3f}s")
                 else:
                     still.append((a, ev, host, keep))
-            finishing[:] = still
+            finishing[:] = still         # the same list object as self._finishing
+            self._note_peak(active)
             if len(retired) > 16 and not finishing and not active:
                 for old in retired:
                     old.close()
@@ -253,7 +464,21 @@ class Server:
         for old in retired + ([batch] if batch is not None else []):
             old.close()
         wall = time.perf_counter() - t0
-        return summarize(specs, self.latency, self.setup, wall, busy, self.meshes)
+        out = summarize([s for s in specs if s.rid not in self.errors], self.latency,
+                        self.setup, wall, busy, self.meshes)
+        out["failed"] = len(self.errors)
+        out["peak_bytes"] = int(self.peak_bytes)
+        out["timing_s"] = {k: round(v, 3) for k, v in self.timing.items()}
+        return out
+
+    def close(self):
+        """Release every pooled plan."""
+        for ps in self.pool.values():
+            for p in ps:
+                p.close()
+        self.pool.clear()
+        self.plan_bytes.clear()
+        self.adjoint_plans.clear()
 
 
 _BASE = {}
@@ -267,6 +492,10 @@ def _base_case(mesh, device, dt):
         _BASE[key] = make_case_device(mesh, device, seed=1234, dt=dt,
                                       tracers=("T", "S", "T", "S"))
     return _BASE[key]
+
+
+def _drop_base_case(mesh, device, dt):
+    _BASE.pop((id(mesh), str(device), dt), None)
 
 
 def request_case(mesh, spec, device, dt):
@@ -283,6 +512,11 @@ def request_case(mesh, spec, device, dt):
     fac = 0.8 + 0.4 * float(torch.rand((), generator=g, **f64))
     noise = lambda amp, t: torch.where(nlm, amp * torch.randn(t.shape, generator=g, **f64), zero)
     Ts = [base["T"][j] + noise(0.05, base["T"][j]) for j in range(spec.K)]
+    if spec.inject == "nan":
+        Ts[0][0, 0] = float("nan")
+    if spec.inject == "fail":
+        # a tracer of the wrong shape: the library refuses the instance
+        Ts[0] = Ts[0][:, :-1]
     case = dict(uv=base["uv"] * fac, w_e=base["w_e"] * fac, helem=base["helem"],
                 hnode=base["hnode"], hnode_new=base["hnode_new"], T=Ts,
                 T_old=[t.clone() for t in Ts], dt=dt)
@@ -293,12 +527,17 @@ def request_case(mesh, spec, device, dt):
 
 def adjoint_of_rollout(a, dt, to_host=True):
     """Gradients of sum_surface (T_final - T_target)^2 with respect to the initial
-    tracers (T and T_old of step 0) and uv, through the stored trajectory."""
+    tracers (T and T_old of step 0) and uv. The backward runs segment by segment from
+    the last checkpoint to the first: each segment's states are recomputed from its
+    checkpoint with the request's own plan (the same bits as the steps served), then
+    the step adjoints run back through them."""
     import torch
     from .adjoint import advect_fct_vjp
+    from .tracer_adv import advect_fct
     inst, mesh = a.inst, a.mesh
+    n = a.spec.nsteps
     surf = torch.from_numpy(mesh.node_layer_mask[:, 0]).to(inst.uv.device)
-    fin = a.traj[-1][0]
+    fin = inst.T()
     g = []
     for Tf, tg in zip(fin, a.case_extra["target"]):
         gj = torch.zeros_like(Tf)
@@ -306,23 +545,33 @@ def adjoint_of_rollout(a, dt, to_host=True):
         g.append(gj)
     gTo = [torch.zeros_like(x) for x in g]
     g_uv = torch.zeros_like(inst.uv)
-    for s in range(len(a.traj) - 2, -1, -1):
-        T, To = a.traj[s]
-        gT, gTo_s, guv, _ = advect_fct_vjp(mesh, inst.uv, inst.w_e, inst.helem, inst.hnode,
-                                           inst.hnode_new, T, To, g, dt=dt, need_w=False)
-        g_uv += guv
-        g = [x + y for x, y in zip(gT, gTo)]
-        gTo = gTo_s
+    flow = (inst.uv, inst.w_e, inst.helem, inst.hnode, inst.hnode_new)
+    for s0 in range((n - 1) // a.stride * a.stride, -1, -a.stride):
+        s1 = min(s0 + a.stride, n)
+        T, To = a.ckpt[s0]
+        states = [(T, To)]
+        for _ in range(s0, s1 - 1):
+            Tn = [o[0] for o in advect_fct(mesh, *flow, T, To, dt=dt, plan=inst.plan)]
+            To, T = T, Tn
+            states.append((T, To))
+        for T, To in reversed(states):
+            gT, gTo_s, guv, _ = advect_fct_vjp(mesh, *flow, T, To, g, dt=dt, need_w=False,
+                                               plan=inst.plan)
+            g_uv += guv
+            g = [x + y for x, y in zip(gT, gTo)]
+            gTo = gTo_s
     if not to_host:
         return {"g_T0": g, "g_T_old0": gTo, "g_uv": g_uv}
     return {"g_T0": [x.cpu().numpy() for x in g], "g_T_old0": [x.cpu().numpy() for x in gTo],
             "g_uv": g_uv.cpu().numpy()}
 
 
-def fifo_loop(specs, device, *, meshes=None, dt=2700.0):
+def fifo_loop(specs, device, *, meshes=None, dt=2700.0, budget_bytes=200 << 30):
     """The plain per-request loop: requests in arrival order, one at a time; each
     builds its plan and inputs, runs its steps with the plain entry point, downloads
-    its result and synchronises before the next one starts."""
+    its result and synchronises before the next one starts. Plans are kept per (mesh,
+    tracers) for later requests while their bytes stay within ``budget_bytes`` (the
+    least recently used ones are closed beyond it)."""
     import torch
     from .batch import Instance
     from .tracer_adv import FctPlan
@@ -340,7 +589,11 @@ def fifo_loop(specs, device, *, meshes=None, dt=2700.0):
         case = request_case(mesh, spec, device, dt)
         key = (spec.level, spec.nl, spec.K)
         if key not in plans:
+            need = estimate_plan_bytes(mesh, spec.K, spec.adjoint)
+            while plans and sum(p.device_bytes() for p in plans.values()) + need > budget_bytes:
+                plans.pop(next(iter(plans))).close()
             plans[key] = FctPlan(mesh, device, max_tracers=spec.K)
+        plans[key] = plans.pop(key)          # most recently used last
         inst = Instance.from_case(mesh, case, device, plan=plans[key])
         torch.cuda.synchronize(device)
         setup[spec.rid] = time.perf_counter() - tb
@@ -348,11 +601,12 @@ def fifo_loop(specs, device, *, meshes=None, dt=2700.0):
                     case_extra={"uv": case["uv"], "target": case.get("target")},
                     remaining=spec.nsteps, bytes=0, t_admit=0.0, setup_s=setup[spec.rid])
         if spec.adjoint:
-            a.traj.append(([x.clone() for x in inst.T()], [x.clone() for x in inst.T_old()]))
-        for _ in range(spec.nsteps):
+            a.stride = checkpoint_stride(spec.nsteps)
+            a.ckpt[0] = ([x.clone() for x in inst.T()], [x.clone() for x in inst.T_old()])
+        for i in range(spec.nsteps):
             inst.step_alone()
-            if spec.adjoint:
-                a.traj.append(([x.clone() for x in inst.T()], [x.clone() for x in inst.T_old()]))
+            if spec.adjoint and (i + 1) % a.stride == 0 and i + 1 < spec.nsteps:
+                a.ckpt[i + 1] = ([x.clone() for x in inst.T()], [x.clone() for x in inst.T_old()])
         res = {"T": [x.cpu().numpy() for x in inst.T()]}
         if spec.adjoint:
             res.update(adjoint_of_rollout(a, dt))

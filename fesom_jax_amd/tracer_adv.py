@@ -20,6 +20,7 @@ There is no CPU fallback: without a ROCm GPU or the built library the call raise
 
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import weakref
 from ctypes import c_void_p
@@ -68,6 +69,29 @@ class FctPlan:
         _lib.check(self.lib.fct_plan_get_info(ptr, ctypes.byref(info)), "fct_plan_get_info")
         self.workspace_bytes = int(info.workspace_bytes)
         self.fused_available = bool(info.paths & _lib.FCT_PATH_BIT_FUSED)
+        self._last = None          # stream of the last launch on this plan
+        self._event = None         # recorded after that launch
+
+    @contextlib.contextmanager
+    def on(self, stream):
+        """Order a launch on ``stream`` after the plan's previous launches: calls on one
+        plan share its workspace, so when the stream changes the new stream first waits
+        for the last one's work; the launch is then recorded for the next caller."""
+        torch = _torch()
+        if self._last is not None and self._last != stream.cuda_stream:
+            stream.wait_event(self._event)
+        yield
+        if self._event is None:
+            self._event = torch.cuda.Event()
+        self._event.record(stream)
+        self._last = stream.cuda_stream
+
+    def device_bytes(self) -> int:
+        """Device memory the plan holds now (tables and workspace, including what the
+        phased, adjoint, diffusion and sea-ice entry points allocated on first use)."""
+        info = _lib.FctPlanInfo()
+        _lib.check(self.lib.fct_plan_get_info(self.ptr, ctypes.byref(info)), "fct_plan_get_info")
+        return int(info.workspace_bytes)
 
     def close(self):
         self._finalizer()
@@ -157,7 +181,7 @@ def _ptr_array(ts):
 
 
 def _run(mesh, uv, w_e, helem, hnode, hnode_new, Ts, T_olds, dt, exch, Z3d, zbar3, w_i,
-         path=_lib.FCT_PATH_AUTO):
+         path=_lib.FCT_PATH_AUTO, plan=None):
     torch = _torch()
     device = _pick_device(T_olds[0], Ts[0], uv, w_e)
     N, E, L = int(mesh.nod2D), int(mesh.elem2D), int(mesh.nl)
@@ -176,7 +200,10 @@ def _run(mesh, uv, w_e, helem, hnode, hnode_new, Ts, T_olds, dt, exch, Z3d, zbar
     zb3_t = None if zbar3 is None else _to_device(zbar3, "zbar3", device, nshape)
     out = [torch.empty(nshape, dtype=torch.float64, device=device) for _ in Ts]
 
-    plan = get_plan(mesh, device)
+    if plan is None:
+        plan = get_plan(mesh, device)
+    elif plan.device != device or plan.nod2D != N or plan.nl != L:
+        raise ValueError("plan does not belong to this mesh and device")
     Tp, Top, Tnp = _ptr_array(T_t), _ptr_array(To_t), _ptr_array(out)
     args = _lib.FctStepArgs(
         ntracers=len(Ts), dt=float(dt), uv=uv_t.data_ptr(), w_e=w_t.data_ptr(),
@@ -187,7 +214,7 @@ def _run(mesh, uv, w_e, helem, hnode, hnode_new, Ts, T_olds, dt, exch, Z3d, zbar
         w_i=None if wi_t is None else wi_t.data_ptr(),
         Z3d=None if z3_t is None else z3_t.data_ptr(),
         zbar3=None if zb3_t is None else zb3_t.data_ptr())
-    with torch.cuda.device(device):
+    with torch.cuda.device(device), plan.on(torch.cuda.current_stream(device)):
         stream = torch.cuda.current_stream(device).cuda_stream
         if path == _UPWIND:
             _lib.check(plan.lib.fct_forward_upwind(plan.ptr, ctypes.byref(args), c_void_p(stream)),
@@ -269,10 +296,11 @@ def advect_one_fct(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, *,
 
 def advect_fct(mesh, uv, w_e, helem, hnode, hnode_new, Ts, T_olds, *,
                dt: float = DT_DEFAULT, exch=None, Z3d=None, zbar3=None, w_i=None,
-               path: str = "auto"):
+               path: str = "auto", plan=None):
     """K tracers sharing one flow in one call (the two calls of fesom_jax/step.py:409-416
     fused): returns ``[(T_new_k, T_k)]``. ``path`` selects the kernel path ("auto",
-    "fused", "phased"; all give the same bits). Extension with no reference
+    "fused", "phased"; all give the same bits); ``plan`` an ``FctPlan`` of the mesh to
+    run on instead of the mesh's cached one. Extension with no reference
     counterpart."""
     if len(Ts) != len(T_olds) or not Ts:
         raise ValueError("Ts and T_olds must be non-empty and of equal length")
@@ -281,5 +309,5 @@ def advect_fct(mesh, uv, w_e, helem, hnode, hnode_new, Ts, T_olds, *,
     if path not in codes:
         raise ValueError(f"path must be one of {sorted(codes)}")
     out = _run(mesh, uv, w_e, helem, hnode, hnode_new, list(Ts), list(T_olds), dt, exch, Z3d,
-               zbar3, w_i, codes[path])
+               zbar3, w_i, codes[path], plan=plan)
     return [(_from_device(o, _kind_of(T), T), T) for o, T in zip(out, Ts)]

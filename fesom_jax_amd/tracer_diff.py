@@ -79,7 +79,7 @@ def diffuse_tracers(mesh, Ts, Kv, hnode_new, *, dt: float = DT_DEFAULT, bc_surf=
         dt=float(dt), floor_value=float(floor_value), Kv=kv_t.data_ptr(),
         hnode_new=hnn_t.data_ptr(), Z3d=None if z3_t is None else z3_t.data_ptr(),
         T=P(Tp), bc_surf=P(bc_arr), sw_3d=P(sw_arr), T_new=P(Onp))
-    with torch.cuda.device(device):
+    with torch.cuda.device(device), plan.on(torch.cuda.current_stream(device)):
         stream = torch.cuda.current_stream(device).cuda_stream
         _lib.check(plan.lib.fct_impl_vert_diff(plan.ptr, ctypes.byref(args), c_void_p(stream)),
                    "fct_impl_vert_diff")

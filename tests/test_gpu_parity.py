@@ -359,3 +359,35 @@ def test_shuffled_numbering(nl, torch_dev):
     outs = _gpu_step(m, case, torch_dev, path="fused")
     for j, g in enumerate(outs):
         assert _check(f"shuffled nl={nl} tracer {j}", g, _oracle(m, case, j), m) == 0
+
+
+def test_two_streams_share_a_plan(icosa4_mesh, icosa4_case, torch_dev):
+    """Two torch streams stepping different states on one mesh share its cached plan
+    (and workspace); the plan orders them, so each result keeps its single-stream bits."""
+    import torch
+    from fesom_jax_amd.tracer_adv import advect_fct
+    m, c = icosa4_mesh, icosa4_case
+    sharp = sharp_case(m, c, 2.0)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(torch_dev)
+    flow = [t(c[k]) for k in ("uv", "w_e", "helem", "hnode", "hnode_new")]
+    flow2 = [t(sharp[k]) for k in ("uv", "w_e")] + flow[2:]
+    A = ([t(x) for x in c["T"]], [t(x) for x in c["T_old"]])
+    B = ([t(sharp["T"][0])], [t(sharp["T_old"][0])])
+    refA = [o[0].clone() for o in advect_fct(m, *flow, *A, dt=c["dt"])]
+    refB = [o[0].clone() for o in advect_fct(m, *flow2, *B, dt=c["dt"])]
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(torch_dev), torch.cuda.Stream(torch_dev)
+    s1.wait_stream(torch.cuda.current_stream(torch_dev))
+    s2.wait_stream(torch.cuda.current_stream(torch_dev))
+    outs = []
+    for _ in range(3):
+        with torch.cuda.stream(s1):
+            oa = advect_fct(m, *flow, *A, dt=c["dt"])
+        with torch.cuda.stream(s2):
+            ob = advect_fct(m, *flow2, *B, dt=c["dt"])
+        outs.append((oa, ob))
+    torch.cuda.synchronize()
+    for oa, ob in outs:
+        for x, y in zip(oa, refA):
+            assert torch.equal(x[0], y)
+        assert torch.equal(ob[0][0], refB[0])

@@ -48,3 +48,53 @@ def test_admission_respects_budget(torch_dev):
     srv.run(specs)
     assert len(srv.latency) == len(specs)
     assert peak["n"] <= 1
+
+
+def test_failures_stay_with_their_request(torch_dev):
+    """A request with a NaN in its tracer and one whose setup the library refuses fail
+    alone; every other request of the mix equals its stand-alone run bit for bit."""
+    import dataclasses
+    from fesom_jax_amd.serve import MeshCache, Server, fifo_loop, make_stream
+    specs = make_stream(300.0, 0.05, levels=(3, 4), nls=(47, 70), steps=(2, 5),
+                        p_adjoint=0.3, seed=8)
+    assert len(specs) >= 8
+    specs[1] = dataclasses.replace(specs[1], inject="nan")
+    specs[4] = dataclasses.replace(specs[4], inject="fail")
+    meshes = MeshCache(seed=12)
+    srv = Server(torch_dev, max_active=6, meshes=meshes)
+    summary = srv.run(specs)
+    assert set(srv.errors) == {specs[1].rid, specs[4].rid}, srv.errors
+    assert "non-finite" in srv.errors[specs[1].rid]
+    assert "setup failed" in srv.errors[specs[4].rid]
+    assert summary["failed"] == 2 and summary["requests"] == len(specs) - 2
+    clean = [s for s in specs if s.inject is None]
+    alone = fifo_loop(clean, torch_dev, meshes=meshes)
+    for s in clean:
+        a, b = srv.results[s.rid], alone["results"][s.rid]
+        for key in b:
+            xs = a[key] if isinstance(a[key], list) else [a[key]]
+            ys = b[key] if isinstance(b[key], list) else [b[key]]
+            for x, y in zip(xs, ys):
+                assert np.array_equal(x, y), (s.rid, key)
+    srv.close()
+
+
+def test_budget_counts_every_live_plan(torch_dev):
+    """The held bytes are the pooled plans' own counts (fct_plan_get_info), adjoint
+    workspace included, and never pass the budget while every request fits it."""
+    from fesom_jax_amd.serve import MeshCache, Server, estimate_bytes, make_stream
+    meshes = MeshCache(seed=13)
+    specs = make_stream(200.0, 0.06, levels=(3, 4), nls=(47,), steps=(2, 4), p_adjoint=0.5,
+                        seed=9)
+    big = max(estimate_bytes(meshes.get(s.level, s.nl), s.K, s.nsteps, s.adjoint) for s in specs)
+    budget = int(2.5 * big)
+    srv = Server(torch_dev, budget_bytes=budget, max_active=8, meshes=meshes)
+    srv.run(specs)
+    assert not srv.errors and len(srv.latency) == len(specs)
+    pooled = [p for ps in srv.pool.values() for p in ps]
+    assert pooled
+    assert srv.held_bytes([]) == (sum(p.device_bytes() for p in pooled)
+                                  + sum(srv.base_bytes.values()))
+    assert any(s.adjoint for s in specs)
+    assert srv.peak_bytes <= budget
+    srv.close()
