@@ -49,7 +49,7 @@ EXPORTED_SYMBOLS = ("fct_abi_version", "fct_last_error", "fct_plan_create",
                     "fct_batch_create", "fct_batch_set_tuning", "fct_batch_bind",
                     "fct_batch_launch", "fct_batch_graph_launch", "fct_batch_destroy",
                     "fct_backward", "fct_forward_upwind", "fct_impl_vert_diff",
-                    "fct_compute_w", "fct_ice_fct_solve", "fct_impl_vert_diff_static")
+                    "fct_compute_w", "fct_ice_fct_solve", "fct_impl_vert_diff_static", "fct_stage", "fct_tracer_ab")
 
 
 class FctMeshDesc(ctypes.Structure):
@@ -72,6 +72,19 @@ class FctStepArgs(ctypes.Structure):
                 ("T", POINTER(c_void_p)), ("T_old", POINTER(c_void_p)),
                 ("T_new", POINTER(c_void_p)),
                 ("w_i", c_void_p), ("Z3d", c_void_p), ("zbar3", c_void_p)]
+
+
+FCT_STAGE = {name: i + 1 for i, name in enumerate((
+    "TRACER_AB", "EDGE_VFLUX", "ADV_FLUX_HOR", "ADV_FLUX_VER", "FLUX2DTRACER",
+    "ALE_RECONSTRUCT", "ADV_FLUX_HOR_HO", "ADV_FLUX_VER_HO", "COMPUTE_FCT_LO",
+    "TRACER_GRADIENT_ELEMENTS", "FILL_UP_DN_GRAD", "ZALESAK_LIMIT", "ZALESAK_PM",
+    "ZALESAK_APPLY", "FLUX2DTRACER_FCT", "ADV_TRA_VERT_IMPL"))}
+FCT_STAGE_MAX_IN = 8
+
+
+class FctStageArgs(ctypes.Structure):
+    _fields_ = [("stage", c_int32), ("dt", c_double), ("in_", c_void_p * FCT_STAGE_MAX_IN),
+                ("out", c_void_p * 2)]
 
 
 class FctAdjointArgs(ctypes.Structure):
@@ -159,6 +172,10 @@ def load(path: Path | None = None, *, require_torch: bool = True):
     lib.fct_compute_w.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
     lib.fct_impl_vert_diff.restype = c_int32
     lib.fct_impl_vert_diff.argtypes = [c_void_p, POINTER(FctVdiffArgs), c_void_p]
+    lib.fct_tracer_ab.restype = c_int32
+    lib.fct_tracer_ab.argtypes = [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p]
+    lib.fct_stage.restype = c_int32
+    lib.fct_stage.argtypes = [c_void_p, POINTER(FctStageArgs), c_void_p]
     lib.fct_impl_vert_diff_static.restype = c_int32
     lib.fct_impl_vert_diff_static.argtypes = [c_void_p, POINTER(FctVdiffArgs), c_void_p]
     lib.fct_backward.restype = c_int32

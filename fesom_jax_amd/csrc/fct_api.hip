@@ -897,6 +897,43 @@ int fct_impl_vert_diff_static(fct_plan* p, const fct_vdiff_args* a, void* stream
     return run_vdiff(p, a, grid, scr, (hipStream_t)stream);
 }
 
+int fct_tracer_ab(const double* values, const double* valuesold, double* ttf_ab,
+                  int64_t count, void* stream) {
+    if (count < 0) return set_error(FCT_E_INVALID, "count %lld < 0", (long long)count);
+    if (count && (!values || !valuesold || !ttf_ab))
+        return set_error(FCT_E_INVALID, "NULL array");
+    return launch_tracer_ab(values, valuesold, ttf_ab, count, (hipStream_t)stream);
+}
+
+int fct_stage(fct_plan* p, const fct_stage_args* a, void* stream) {
+    if (!p || !a) return set_error(FCT_E_INVALID, "NULL plan or args");
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != p->device)
+        return set_error(FCT_E_INVALID, "current device %d is not the plan's device %d", cur,
+                         p->device);
+    // required inputs and outputs per stage (ADV_FLUX_VER_HO: in[2], in[3] optional)
+    static const int need_in[17] = {0, 2, 2, 3, 2, 2, 4, 4, 2, 5, 1, 1, 5, 5, 7, 6, 3};
+    static const int need_out[17] = {0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 1, 1};
+    const int st = a->stage;
+    if (st < 1 || st > 16) return set_error(FCT_E_INVALID, "unknown stage %d", st);
+    for (int i = 0; i < need_in[st]; ++i)
+        if (!a->in[i]) return set_error(FCT_E_INVALID, "stage %d: input %d is NULL", st, i);
+    for (int i = 0; i < need_out[st]; ++i)
+        if (!a->out[i]) return set_error(FCT_E_INVALID, "stage %d: output %d is NULL", st, i);
+    if (st == FCT_STAGE_ADV_FLUX_VER_HO && ((a->in[2] == nullptr) != (a->in[3] == nullptr)))
+        return set_error(FCT_E_INVALID, "Z3d and zbar3 must be given together");
+    StageScratch w;
+    w.vflux = p->vflux;
+    if (st == FCT_STAGE_FILL_UP_DN_GRAD || st == FCT_STAGE_ZALESAK_LIMIT ||
+        st == FCT_STAGE_ZALESAK_APPLY || st == FCT_STAGE_ADV_TRA_VERT_IMPL) {
+        int rc = ensure_phased(p);
+        if (rc) return rc;
+        w.gnode = p->gnode; w.tvmax = p->tvmax; w.tvmin = p->tvmin; w.fplus = p->fplus;
+        w.fminus = p->fminus; w.pf = p->pf; w.mf = p->mf;
+    }
+    return launch_stage(p->m, w, st, a->in, a->out, a->dt, (hipStream_t)stream);
+}
+
 int fct_forward_phase(fct_plan* p, const fct_step_args* a, int32_t j, int32_t phase,
                       const fct_phase_bufs* b, void* stream) {
     int rc = check_args(p, a);

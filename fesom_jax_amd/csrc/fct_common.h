@@ -131,6 +131,16 @@ int launch_final(const DevMesh& m, const fct_step_args* a, int j, const double* 
                  const double* tvmin, const double* fplus, const double* fminus, double* pf,
                  double* mf, hipStream_t s);
 
+// Scratch of the single-stage entry point (plan workspace).
+struct StageScratch {
+    double *vflux = nullptr, *gnode = nullptr, *tvmax = nullptr, *tvmin = nullptr;
+    double *fplus = nullptr, *fminus = nullptr, *pf = nullptr, *mf = nullptr;
+};
+int launch_stage(const DevMesh& m, const StageScratch& w, int stage, const double* const* in,
+                 double* const* out, double dt, hipStream_t s);
+int launch_tracer_ab(const double* T, const double* Told, double* out, int64_t count,
+                     hipStream_t s);
+
 int launch_fac(const DevMesh& m, const fct_step_args* a, const double* LO, const double* tvmax,
                const double* tvmin, const double* fplus, const double* fminus, double* pf,
                double* mf, hipStream_t s);

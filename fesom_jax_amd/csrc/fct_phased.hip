@@ -416,6 +416,243 @@ k_final(DevMesh m, const double* __restrict__ adfh, const double* __restrict__ a
 }
 
 // ---------------------------------------------------------------------------
+// single stages with the reference's inputs and outputs (fct_stage): each kernel
+// restates one public function of fesom_jax/tracer_adv.py on every lane of its output,
+// in the expression order of the stage kernels above
+// ---------------------------------------------------------------------------
+// tracer_ab, tracer_adv.py:59-66 (ttfAB only; T_old_new is the input T)
+__global__ void __launch_bounds__(256)
+k_st_ab(int64_t count, const double* __restrict__ T, const double* __restrict__ Told,
+        double* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) out[t] = AB_OLD * Told[t] + AB_NEW * T[t];
+}
+
+// adv_flux_hor, tracer_adv.py:96-105: upwind flux of ttf through each edge-level
+__global__ void __launch_bounds__(256)
+k_st_hlo(DevMesh m, const double* __restrict__ vflux, const double* __restrict__ ttf,
+         double* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)m.D * m.L) return;
+    const int e = (int)(t / m.L), k = (int)(t - (int64_t)e * m.L);
+    const int2 nd = m.edges[e];
+    const double vf = vflux[t];
+    const double av = fabs(vf);
+    const double T1 = ttf[(size_t)nd.x * m.L + k], T2 = ttf[(size_t)nd.y * m.L + k];
+    out[t] = -0.5 * (T1 * (vf + av) + T2 * (vf - av));
+}
+
+// adv_flux_ver, tracer_adv.py:108-120
+__global__ void __launch_bounds__(256)
+k_st_vlo(DevMesh m, const double* __restrict__ w_e, const double* __restrict__ ttf,
+         double* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)m.N * m.L) return;
+    const int n = (int)(t / m.L), k = (int)(t - (int64_t)n * m.L);
+    out[t] = flux_v_lo(m, ttf, w_e[t], n, k, m.nlev[n]);
+}
+
+// compute_fct_lo, tracer_adv.py:298-313, from given edge and vertical fluxes
+__global__ void __launch_bounds__(256)
+k_st_lo(DevMesh m, const double* __restrict__ fh, const double* __restrict__ fv,
+        const double* __restrict__ T, const double* __restrict__ hnode,
+        const double* __restrict__ hnn, double dt, double* __restrict__ LO) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)m.N * m.L) return;
+    const int n = (int)(t / m.L), k = (int)(t - (int64_t)n * m.L);
+    const int L = m.L;
+    double acc = 0.0;
+    for (int p = m.ecsr_off[n], pe = m.ecsr_off[n + 1]; p < pe; ++p) {
+        const int ent = m.ecsr[p];
+        const double f = fh[(size_t)(ent >> 1) * L + k];
+        acc = acc + ((ent & 1) ? -f : f);
+    }
+    const size_t i = (size_t)t;
+    const double div_v = fv[i] - ((k + 1 < L) ? fv[i + 1] : 0.0);
+    const double a = m.areasvol[i], hn = hnn[i];
+    const double safe_a = a > 0.0 ? a : 1.0, safe_hn = hn > 0.0 ? hn : 1.0;
+    const double numer = T[i] * hnode[i] + (acc + div_v) * dt / safe_a;
+    const double lo = numer / safe_hn;
+    LO[i] = (in_layer(m.nlev[n], k) && hn > 0.0 && a > 0.0) ? lo : 0.0;
+}
+
+// fill_up_dn_grad, tracer_adv.py:344-368: [D][L][4] gradient slots from tr_xy and the
+// node averages
+__global__ void __launch_bounds__(256)
+k_st_eud(DevMesh m, const double2* __restrict__ tr, const double2* __restrict__ gnode,
+         double4* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)m.D * m.L) return;
+    const int e = (int)(t / m.L), k = (int)(t - (int64_t)e * m.L);
+    const int L = m.L;
+    const int2 ur = m.utri_rng[e];
+    if (k >= ur.x && k < ur.y) {
+        const int2 ud = m.updn[e];
+        const double2 gu = tr[(size_t)ud.x * L + k], gd = tr[(size_t)ud.y * L + k];
+        out[t] = make_double4(gu.x, gd.x, gu.y, gd.y);
+    } else {
+        const int2 nd = m.edges[e];
+        const double2 ga = gnode[(size_t)nd.x * L + k], gb = gnode[(size_t)nd.y * L + k];
+        out[t] = make_double4(ga.x, gb.x, ga.y, gb.y);
+    }
+}
+
+// adv_flux_hor_ho, tracer_adv.py:196-225, from given gradient slots
+__global__ void __launch_bounds__(256)
+k_st_hho(DevMesh m, const double* __restrict__ vflux, const double* __restrict__ ttf,
+         const double4* __restrict__ eud, double* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)m.D * m.L) return;
+    const int e = (int)(t / m.L), k = (int)(t - (int64_t)e * m.L);
+    const int2 nd = m.edges[e];
+    const double T1 = ttf[(size_t)nd.x * m.L + k], T2 = ttf[(size_t)nd.y * m.L + k];
+    const double diff = T2 - T1;
+    const double4 g = eud[t];
+    const double2 cc = m.edge_c[e];
+    const double Tm1 = T1 + (2.0 * diff + cc.x * g.x + cc.y * g.z) / 6.0;
+    const double Tm2 = T2 - (2.0 * diff + cc.x * g.y + cc.y * g.w) / 6.0;
+    const double vf = vflux[t];
+    const double av = fabs(vf);
+    out[t] = -0.5 * ((vf + av) * Tm1 + (vf - av) * Tm2);
+}
+
+// adv_flux_ver_ho, tracer_adv.py:241-295 (static stencil, or Z3d / zbar3)
+__global__ void __launch_bounds__(256)
+k_st_vho(DevMesh m, const double* __restrict__ ttf, const double* __restrict__ w_e,
+         const double* __restrict__ Z3d, const double* __restrict__ zbar3,
+         double* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)m.N * m.L) return;
+    const int n = (int)(t / m.L), k = (int)(t - (int64_t)n * m.L);
+    const int L = m.L;
+    const int2 nr = m.nlev[n];
+    const size_t i = (size_t)t;
+    double hi = 0.0;
+    if (in_iface(nr, k)) {
+        const double W = w_e[i];
+        const size_t b0 = (size_t)n * L;
+        const int km1 = k > 0 ? k - 1 : 0, km2 = k > 1 ? k - 2 : 0;
+        const double ar = m.area[i];
+        const double A = ttf[i], Am1 = ttf[b0 + km1];
+        if (k == nr.x) {
+            hi = -A * W * ar;
+        } else if (k == nr.y) {
+            hi = 0.0;
+        } else if (k == nr.x + 1 || k == nr.y - 1) {
+            hi = -0.5 * (Am1 + A) * W * ar;
+        } else if (k >= nr.x + 2 && k <= nr.y - 2) {
+            const double Am2 = ttf[b0 + km2];
+            const double Ap1 = (k + 1 < L) ? ttf[i + 1] : 0.0;
+            double Znz, Zm1, Zm2, Zp1, zb;
+            if (Z3d) {
+                const int kp1 = k + 1 < L ? k + 1 : L - 1;
+                Znz = Z3d[i]; Zm1 = Z3d[b0 + km1]; Zm2 = Z3d[b0 + km2]; Zp1 = Z3d[b0 + kp1];
+                zb = zbar3[i];
+            } else {
+                Znz = m.Zs[k]; Zm1 = m.Zs[L + k]; Zm2 = m.Zs[2 * L + k]; Zp1 = m.Zs[3 * L + k];
+                zb = m.zbar[k];
+            }
+            const double qc = (Am1 - A) / nz_safe(Zm1 - Znz);
+            const double qu = (A - Ap1) / nz_safe(Znz - Zp1);
+            const double qd = (Am2 - Am1) / nz_safe(Zm2 - Zm1);
+            const double Tmean1 = A + (2.0 * qc + qu) * (zb - Znz) / 3.0;
+            const double Tmean2 = Am1 + (2.0 * qc + qd) * (zb - Zm1) / 3.0;
+            hi = -0.5 * (Tmean1 + Tmean2) * W * ar;
+        }
+    }
+    out[i] = hi;
+}
+
+// zalesak_limit b3, tracer_adv.py:453-466: limited edge and vertical fluxes
+__global__ void __launch_bounds__(256)
+k_st_hlim(DevMesh m, const double* __restrict__ adfh, const double* __restrict__ pf,
+          const double* __restrict__ mf, double* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)m.D * m.L) return;
+    const int e = (int)(t / m.L), k = (int)(t - (int64_t)e * m.L);
+    const int2 nd = m.edges[e];
+    const size_t i1 = (size_t)nd.x * m.L + k, i2 = (size_t)nd.y * m.L + k;
+    const double f = adfh[t];
+    const double ae = f >= 0.0 ? fmin(pf[i1], mf[i2]) : fmin(mf[i1], pf[i2]);
+    out[t] = ae * f;
+}
+
+__global__ void __launch_bounds__(256)
+k_st_vlim(DevMesh m, const double* __restrict__ adfv, const double* __restrict__ pf,
+          const double* __restrict__ mf, double* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)m.N * m.L) return;
+    const int n = (int)(t / m.L), k = (int)(t - (int64_t)n * m.L);
+    out[t] = adfv_lim(m, adfv, pf, mf, (size_t)n * m.L, k, m.nlev[n]);
+}
+
+// flux2dtracer_fct, tracer_adv.py:469-484 (from limited fluxes)
+__global__ void __launch_bounds__(256)
+k_st_f2d(DevMesh m, const double* __restrict__ T, const double* __restrict__ LO,
+         const double* __restrict__ adfh, const double* __restrict__ adfv,
+         const double* __restrict__ hnode, const double* __restrict__ hnn, double dt,
+         double* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)m.N * m.L) return;
+    const int n = (int)(t / m.L), k = (int)(t - (int64_t)n * m.L);
+    const int L = m.L;
+    double acc = 0.0;
+    for (int p = m.ecsr_off[n], pe = m.ecsr_off[n + 1]; p < pe; ++p) {
+        const int ent = m.ecsr[p];
+        const double f = adfh[(size_t)(ent >> 1) * L + k];
+        acc = acc + ((ent & 1) ? -f : f);
+    }
+    const size_t i = (size_t)t;
+    const double a = m.areasvol[i];
+    const double safe_a = a > 0.0 ? a : 1.0;
+    const bool nlm = in_layer(m.nlev[n], k);
+    const double lo_trans = -T[i] * hnode[i] + LO[i] * hnn[i];
+    const double vdiv = a > 0.0 ? (adfv[i] - ((k + 1 < L) ? adfv[i + 1] : 0.0)) * dt / safe_a : 0.0;
+    const double dttf_v = nlm ? lo_trans + vdiv : 0.0;
+    const double dttf_h = nlm ? acc * dt / safe_a : 0.0;
+    out[i] = dttf_h + dttf_v;
+}
+
+// flux2dtracer, tracer_adv.py:123-139 (the upwind driver's assembly)
+__global__ void __launch_bounds__(256)
+k_st_f2d_up(DevMesh m, const double* __restrict__ fh, const double* __restrict__ fv, double dt,
+            double* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)m.N * m.L) return;
+    const int n = (int)(t / m.L), k = (int)(t - (int64_t)n * m.L);
+    const int L = m.L;
+    double acc = 0.0;
+    for (int p = m.ecsr_off[n], pe = m.ecsr_off[n + 1]; p < pe; ++p) {
+        const int ent = m.ecsr[p];
+        const double f = fh[(size_t)(ent >> 1) * L + k];
+        acc = acc + ((ent & 1) ? -f : f);
+    }
+    const size_t i = (size_t)t;
+    const double a = m.areasvol[i];
+    const double safe = a > 0.0 ? a : 1.0;
+    const bool lm = in_layer(m.nlev[n], k);
+    const double dv = lm ? (fv[i] - ((k + 1 < L) ? fv[i + 1] : 0.0)) * dt / safe : 0.0;
+    const double dh = lm ? acc * dt / safe : 0.0;
+    out[i] = dh + dv;
+}
+
+// ale_reconstruct, tracer_adv.py:142-151
+__global__ void __launch_bounds__(256)
+k_st_ale(DevMesh m, const double* __restrict__ T, const double* __restrict__ del,
+         const double* __restrict__ hnode, const double* __restrict__ hnn,
+         double* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)m.N * m.L) return;
+    const int n = (int)(t / m.L), k = (int)(t - (int64_t)n * m.L);
+    const size_t i = (size_t)t;
+    const double Ti = T[i], hn = hnn[i];
+    const double d = del[i] + Ti * (hnode[i] - hn);
+    const double safe_hn = hn > 0.0 ? hn : 1.0;
+    const double dT = (in_layer(m.nlev[n], k) && hn > 0.0) ? d / safe_hn : 0.0;
+    out[i] = Ti + dT;
+}
+
+// ---------------------------------------------------------------------------
 // host-side launchers
 // ---------------------------------------------------------------------------
 static inline unsigned grid_for(int64_t count, int block) {
@@ -483,6 +720,91 @@ int launch_final(const DevMesh& m, const fct_step_args* a, int j, const double* 
     FCT_LAUNCH(k_final, nl, s, m, adfh, adfv, pf, mf, a->T[j], LO, a->hnode, a->hnode_new,
                a->dt, a->T_new[j]);
     return check_launch("phase 4 kernels");
+}
+
+// One reference stage (fct_stage): `in` and `out` as FCT_STAGE_* documents; scratch
+// holds vflux [D][L], then (for the stages that need them) gnode [N][L][2] and six
+// [N][L] arrays.
+int launch_stage(const DevMesh& m, const StageScratch& w, int stage, const double* const* in,
+                 double* const* out, double dt, hipStream_t s) {
+    const int64_t nl = (int64_t)m.N * m.L, dl = (int64_t)m.D * m.L, el = (int64_t)m.E * m.L;
+    switch (stage) {
+        case FCT_STAGE_TRACER_AB:
+            FCT_LAUNCH(k_st_ab, nl, s, nl, in[0], in[1], out[0]);
+            break;
+        case FCT_STAGE_EDGE_VFLUX:
+            FCT_LAUNCH(k_vflux, dl, s, m, in[0], in[1], out[0]);
+            break;
+        case FCT_STAGE_ADV_FLUX_HOR:
+            FCT_LAUNCH(k_vflux, dl, s, m, in[0], in[1], w.vflux);
+            FCT_LAUNCH(k_st_hlo, dl, s, m, w.vflux, in[2], out[0]);
+            break;
+        case FCT_STAGE_ADV_FLUX_VER:
+            FCT_LAUNCH(k_st_vlo, nl, s, m, in[0], in[1], out[0]);
+            break;
+        case FCT_STAGE_FLUX2DTRACER:
+            FCT_LAUNCH(k_st_f2d_up, nl, s, m, in[0], in[1], dt, out[0]);
+            break;
+        case FCT_STAGE_ALE_RECONSTRUCT:
+            FCT_LAUNCH(k_st_ale, nl, s, m, in[0], in[1], in[2], in[3], out[0]);
+            break;
+        case FCT_STAGE_COMPUTE_FCT_LO:
+            FCT_LAUNCH(k_st_lo, nl, s, m, in[0], in[1], in[2], in[3], in[4], dt, out[0]);
+            break;
+        case FCT_STAGE_TRACER_GRADIENT_ELEMENTS:
+            FCT_LAUNCH(k_trxy, el, s, m, in[0], (double2*)out[0]);
+            break;
+        case FCT_STAGE_FILL_UP_DN_GRAD:
+            FCT_LAUNCH(k_gnode, nl, s, m, (const double2*)in[0], (double2*)w.gnode);
+            FCT_LAUNCH(k_st_eud, dl, s, m, (const double2*)in[0], (const double2*)w.gnode,
+                       (double4*)out[0]);
+            break;
+        case FCT_STAGE_ADV_FLUX_HOR_HO:
+            FCT_LAUNCH(k_vflux, dl, s, m, in[0], in[1], w.vflux);
+            FCT_LAUNCH(k_st_hho, dl, s, m, w.vflux, in[2], (const double4*)in[3], out[0]);
+            break;
+        case FCT_STAGE_ADV_FLUX_VER_HO:
+            FCT_LAUNCH(k_st_vho, nl, s, m, in[1], in[0], in[2], in[3], out[0]);
+            break;
+        case FCT_STAGE_ZALESAK_PM:
+            FCT_LAUNCH(k_pm, nl, s, m, in[2], in[3], out[0], out[1]);
+            break;
+        case FCT_STAGE_ZALESAK_LIMIT:
+        case FCT_STAGE_ZALESAK_APPLY: {
+            const double* fp = w.fplus;
+            const double* fm = w.fminus;
+            if (stage == FCT_STAGE_ZALESAK_LIMIT) {
+                FCT_LAUNCH(k_pm, nl, s, m, in[2], in[3], w.fplus, w.fminus);
+            } else {
+                fp = in[5];
+                fm = in[6];
+            }
+            FCT_LAUNCH(k_bounds, nl, s, m, in[1], in[0], w.tvmax, w.tvmin);
+            FCT_LAUNCH(k_fac, nl, s, m, in[1], w.tvmax, w.tvmin, fp, fm, in[4], dt, w.pf, w.mf);
+            FCT_LAUNCH(k_st_hlim, dl, s, m, in[2], w.pf, w.mf, out[0]);
+            FCT_LAUNCH(k_st_vlim, nl, s, m, in[3], w.pf, w.mf, out[1]);
+            break;
+        }
+        case FCT_STAGE_FLUX2DTRACER_FCT:
+            FCT_LAUNCH(k_st_f2d, nl, s, m, in[0], in[1], in[2], in[3], in[4], in[5], dt, out[0]);
+            break;
+        case FCT_STAGE_ADV_TRA_VERT_IMPL: {
+            if (nl > 0 && hipMemcpyAsync(out[0], in[1], (size_t)nl * sizeof(double),
+                                         hipMemcpyDeviceToDevice, s) != hipSuccess)
+                return set_error(FCT_E_HIP, "adv_tra_vert_impl: copy of T failed");
+            FCT_LAUNCH(k_vert_impl, (int64_t)m.N, s, m, in[0], in[2], dt, out[0], w.pf, w.mf);
+            break;
+        }
+        default:
+            return set_error(FCT_E_INVALID, "unknown stage %d", stage);
+    }
+    return check_launch("stage kernels");
+}
+
+int launch_tracer_ab(const double* T, const double* Told, double* out, int64_t count,
+                     hipStream_t s) {
+    FCT_LAUNCH(k_st_ab, count, s, count, T, Told, out);
+    return check_launch("k_st_ab");
 }
 
 }  // namespace fct

@@ -30,7 +30,11 @@ import numpy as np
 from . import _lib
 from .config import DT_DEFAULT
 
-__all__ = ["advect_one_fct", "advect_fct", "FctPlan", "get_plan", "DT_DEFAULT"]
+__all__ = ["advect_one_fct", "advect_fct", "FctPlan", "get_plan", "DT_DEFAULT",
+           "tracer_ab", "edge_vflux", "adv_flux_hor", "adv_flux_ver", "flux2dtracer",
+           "ale_reconstruct", "adv_flux_hor_ho", "adv_flux_ver_ho", "compute_fct_lo",
+           "tracer_gradient_elements", "fill_up_dn_grad", "zalesak_limit",
+           "flux2dtracer_fct", "adv_tra_vert_impl"]
 
 
 def _torch():
@@ -311,3 +315,153 @@ def advect_fct(mesh, uv, w_e, helem, hnode, hnode_new, Ts, T_olds, *,
     out = _run(mesh, uv, w_e, helem, hnode, hnode_new, list(Ts), list(T_olds), dt, exch, Z3d,
                zbar3, w_i, codes[path], plan=plan)
     return [(_from_device(o, _kind_of(T), T), T) for o, T in zip(out, Ts)]
+
+
+# --------------------------------------------------------------------------
+# the reference's stage functions, one GPU stage each (fct_stage in the C ABI)
+# --------------------------------------------------------------------------
+def _stage(mesh, name, ins, out_shapes, *, dt=0.0):
+    """Run stage ``name`` on ``ins`` (a list of (array or None, name, shape)); returns
+    the outputs in the kind of the first input."""
+    torch = _torch()
+    given = [x for x, _, _ in ins if x is not None]
+    device = _pick_device(*given)
+    ts = [None if x is None else _to_device(x, nm, device, shape) for x, nm, shape in ins]
+    outs = [torch.empty(shape, dtype=torch.float64, device=device) for shape in out_shapes]
+    plan = get_plan(mesh, device)
+    a = _lib.FctStageArgs(stage=_lib.FCT_STAGE[name], dt=float(dt))
+    for i, t in enumerate(ts):
+        a.in_[i] = None if t is None else t.data_ptr()
+    for i, o in enumerate(outs):
+        a.out[i] = o.data_ptr()
+    with torch.cuda.device(device), plan.on(torch.cuda.current_stream(device)):
+        stream = torch.cuda.current_stream(device).cuda_stream
+        _lib.check(plan.lib.fct_stage(plan.ptr, ctypes.byref(a), c_void_p(stream)),
+                   f"fct_stage({name})")
+    kind, like = _kind_of(ins[0][0]), ins[0][0]
+    return [_from_device(o, kind, like) for o in outs]
+
+
+def _shapes(mesh):
+    N, E, D, L = int(mesh.nod2D), int(mesh.elem2D), int(mesh.edge2D), int(mesh.nl)
+    return (N, L), (E, L), (D, L), N, E, D, L
+
+
+def tracer_ab(values, valuesold):
+    """``(ttfAB, T_old_new)``: ``ttfAB = -(0.5+0.1)*valuesold + (1.5+0.1)*values`` on the
+    GPU, ``T_old_new`` the input ``values`` itself (fesom_jax/tracer_adv.py:59-66)."""
+    torch = _torch()
+    device = _pick_device(values, valuesold)
+    shape = tuple(np.shape(values)) if not isinstance(values, torch.Tensor) else tuple(values.shape)
+    v = _to_device(values, "values", device, shape)
+    vo = _to_device(valuesold, "valuesold", device, shape)
+    out = torch.empty(shape, dtype=torch.float64, device=device)
+    lib = _lib.load()
+    with torch.cuda.device(device):
+        stream = torch.cuda.current_stream(device).cuda_stream
+        _lib.check(lib.fct_tracer_ab(c_void_p(v.data_ptr()), c_void_p(vo.data_ptr()),
+                                     c_void_p(out.data_ptr()), int(out.numel()),
+                                     c_void_p(stream)), "fct_tracer_ab")
+    return _from_device(out, _kind_of(values), values), values
+
+
+def edge_vflux(mesh, uv, helem):
+    """Volume flux through each edge-level (fesom_jax/tracer_adv.py:69-93)."""
+    ns, es, ds, N, E, D, L = _shapes(mesh)
+    return _stage(mesh, "EDGE_VFLUX", [(uv, "uv", (E, L, 2)), (helem, "helem", es)], [ds])[0]
+
+
+def adv_flux_hor(mesh, uv, helem, ttf):
+    """Low-order upwind horizontal flux (fesom_jax/tracer_adv.py:96-105)."""
+    ns, es, ds, N, E, D, L = _shapes(mesh)
+    return _stage(mesh, "ADV_FLUX_HOR", [(uv, "uv", (E, L, 2)), (helem, "helem", es),
+                                         (ttf, "ttf", ns)], [ds])[0]
+
+
+def adv_flux_ver(mesh, w_e, ttf):
+    """Low-order upwind vertical flux at the interfaces (fesom_jax/tracer_adv.py:108-120)."""
+    ns = _shapes(mesh)[0]
+    return _stage(mesh, "ADV_FLUX_VER", [(w_e, "w_e", ns), (ttf, "ttf", ns)], [ns])[0]
+
+
+def flux2dtracer(mesh, flux_h, flux_v, *, dt: float = DT_DEFAULT):
+    """Tracer change of the upwind driver (fesom_jax/tracer_adv.py:123-139)."""
+    ns, es, ds = _shapes(mesh)[:3]
+    return _stage(mesh, "FLUX2DTRACER", [(flux_h, "flux_h", ds), (flux_v, "flux_v", ns)], [ns],
+                  dt=dt)[0]
+
+
+def ale_reconstruct(mesh, T, del_ttf, hnode, hnode_new):
+    """``T + del_ttf / hnode_new`` on the wet layers with the thickness change
+    (fesom_jax/tracer_adv.py:142-151)."""
+    ns = _shapes(mesh)[0]
+    return _stage(mesh, "ALE_RECONSTRUCT", [(T, "T", ns), (del_ttf, "del_ttf", ns),
+                                            (hnode, "hnode", ns), (hnode_new, "hnode_new", ns)],
+                  [ns])[0]
+
+
+def adv_flux_hor_ho(mesh, uv, helem, ttf, eud):
+    """MFCT high-order horizontal flux from the gradient slots ``eud``
+    (fesom_jax/tracer_adv.py:196-225)."""
+    ns, es, ds, N, E, D, L = _shapes(mesh)
+    return _stage(mesh, "ADV_FLUX_HOR_HO", [(uv, "uv", (E, L, 2)), (helem, "helem", es),
+                                            (ttf, "ttf", ns), (eud, "eud", (D, L, 4))], [ds])[0]
+
+
+def adv_flux_ver_ho(mesh, w_e, ttf, Z3d=None, zbar3=None):
+    """QR4C high-order vertical flux (fesom_jax/tracer_adv.py:241-295)."""
+    ns = _shapes(mesh)[0]
+    if (Z3d is None) != (zbar3 is None):
+        raise ValueError("Z3d and zbar3 must be given together")
+    return _stage(mesh, "ADV_FLUX_VER_HO", [(w_e, "w_e", ns), (ttf, "ttf", ns),
+                                            (Z3d, "Z3d", ns), (zbar3, "zbar3", ns)], [ns])[0]
+
+
+def compute_fct_lo(mesh, flux_h, flux_v, T, hnode, hnode_new, *, dt: float = DT_DEFAULT):
+    """Low-order solution (fesom_jax/tracer_adv.py:298-313)."""
+    ns, es, ds = _shapes(mesh)[:3]
+    return _stage(mesh, "COMPUTE_FCT_LO", [(flux_h, "flux_h", ds), (flux_v, "flux_v", ns),
+                                           (T, "T", ns), (hnode, "hnode", ns),
+                                           (hnode_new, "hnode_new", ns)], [ns], dt=dt)[0]
+
+
+def tracer_gradient_elements(mesh, ttf):
+    """Per element-level gradient ``[elem2D, nl, 2]`` (fesom_jax/tracer_adv.py:316-326)."""
+    ns, es, ds, N, E, D, L = _shapes(mesh)
+    return _stage(mesh, "TRACER_GRADIENT_ELEMENTS", [(ttf, "ttf", ns)], [(E, L, 2)])[0]
+
+
+def fill_up_dn_grad(mesh, tr_xy):
+    """The four MFCT gradient slots ``[edge2D, nl, 4]`` (fesom_jax/tracer_adv.py:344-368)."""
+    ns, es, ds, N, E, D, L = _shapes(mesh)
+    return _stage(mesh, "FILL_UP_DN_GRAD", [(tr_xy, "tr_xy", (E, L, 2))], [(D, L, 4)])[0]
+
+
+def zalesak_limit(mesh, T, LO, adf_h, adf_v, hnode_new, *, dt: float = DT_DEFAULT, exch=None):
+    """Zalesak limiter: ``(adf_h_lim, adf_v_lim)`` (fesom_jax/tracer_adv.py:371-466).
+    With ``exch`` the P+/P- sums go through it (``"nod"``, :438-439) before the factors."""
+    ns, es, ds = _shapes(mesh)[:3]
+    ins = [(T, "T", ns), (LO, "LO", ns), (adf_h, "adf_h", ds), (adf_v, "adf_v", ns),
+           (hnode_new, "hnode_new", ns)]
+    if exch is None:
+        return tuple(_stage(mesh, "ZALESAK_LIMIT", ins, [ds, ns], dt=dt))
+    fp, fm = _stage(mesh, "ZALESAK_PM", ins, [ns, ns], dt=dt)
+    fp, fm = exch(fp, "nod"), exch(fm, "nod")
+    return tuple(_stage(mesh, "ZALESAK_APPLY", ins + [(fp, "fct_plus", ns),
+                                                      (fm, "fct_minus", ns)], [ds, ns], dt=dt))
+
+
+def flux2dtracer_fct(mesh, T, LO, adf_h, adf_v, hnode, hnode_new, *, dt: float = DT_DEFAULT):
+    """``del_ttf`` of the FCT path from the limited fluxes (fesom_jax/tracer_adv.py:469-484)."""
+    ns, es, ds = _shapes(mesh)[:3]
+    return _stage(mesh, "FLUX2DTRACER_FCT", [(T, "T", ns), (LO, "LO", ns), (adf_h, "adf_h", ds),
+                                             (adf_v, "adf_v", ns), (hnode, "hnode", ns),
+                                             (hnode_new, "hnode_new", ns)], [ns], dt=dt)[0]
+
+
+def adv_tra_vert_impl(mesh, w_i, T, hnode_new, *, dt: float = DT_DEFAULT):
+    """Implicit vertical advection by the split velocity ``w_i``
+    (fesom_jax/tracer_adv.py:487-529)."""
+    ns = _shapes(mesh)[0]
+    return _stage(mesh, "ADV_TRA_VERT_IMPL", [(w_i, "w_i", ns), (T, "T", ns),
+                                              (hnode_new, "hnode_new", ns)], [ns], dt=dt)[0]

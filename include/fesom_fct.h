@@ -266,6 +266,70 @@ int fct_impl_vert_diff(fct_plan* plan, const fct_vdiff_args* args, void* stream)
 #define FCT_VDIFF_STATIC_BYTES (64 << 20)
 int fct_impl_vert_diff_static(fct_plan* plan, const fct_vdiff_args* args, void* stream);
 
+/* ---- single stages -----------------------------------------------------------------
+ * The reference's public stage functions one at a time, for unit parity
+ * (fesom_jax/tracer_adv.py:59-529); fesom_jax_amd.tracer_adv exposes them under the
+ * reference's names. All arrays are device pointers in the reference's layouts
+ * ([entity][nl], tr_xy [elem2D][nl][2], eud [edge2D][nl][4]); every lane of every output
+ * is written, as the reference's whole-array expressions do. Inputs in[] and outputs
+ * out[] by stage:
+ *   TRACER_AB               in: values, valuesold                    out: ttfAB
+ *                           (tracer_ab, :59-66)
+ *   EDGE_VFLUX              in: uv, helem                            out: vflux [edge2D][nl] (:69-93)
+ *   ADV_FLUX_HOR            in: uv, helem, ttf                       out: flux [edge2D][nl] (:96-105)
+ *   ADV_FLUX_VER            in: w_e, ttf                             out: flux [nod2D][nl] (:108-120)
+ *   FLUX2DTRACER            in: flux_h, flux_v; dt                   out: del_ttf (:123-139)
+ *   ALE_RECONSTRUCT         in: T, del_ttf, hnode, hnode_new         out: T_new (:142-151)
+ *   ADV_FLUX_HOR_HO         in: uv, helem, ttf, eud                  out: flux [edge2D][nl] (:196-225)
+ *   ADV_FLUX_VER_HO         in: w_e, ttf, Z3d or NULL, zbar3 or NULL out: flux [nod2D][nl] (:241-295)
+ *   COMPUTE_FCT_LO          in: flux_h, flux_v, T, hnode, hnode_new; dt   out: LO (:298-313)
+ *   TRACER_GRADIENT_ELEMENTS in: ttf                                 out: tr_xy (:316-326)
+ *   FILL_UP_DN_GRAD         in: tr_xy                                out: eud (:344-368)
+ *   ZALESAK_LIMIT           in: T, LO, adf_h, adf_v, hnode_new; dt   out: adf_h_lim, adf_v_lim
+ *                           (:371-466)
+ *   ZALESAK_PM              in: as ZALESAK_LIMIT                     out: fct_plus, fct_minus
+ *                           (the sums the reference hands to exch, :426-439)
+ *   ZALESAK_APPLY           in: as ZALESAK_LIMIT, then fct_plus, fct_minus; dt
+ *                           out: adf_h_lim, adf_v_lim (:441-466 on the exchanged sums)
+ *   FLUX2DTRACER_FCT        in: T, LO, adf_h, adf_v, hnode, hnode_new; dt  out: del_ttf (:469-484)
+ *   ADV_TRA_VERT_IMPL       in: w_i, T, hnode_new; dt                out: T_new (:487-529)
+ * The stages share the plan's workspace (edge flux, node-average gradients, limiter
+ * sums and factors; the last three groups allocated on first use): calls on one plan
+ * must not overlap. */
+typedef enum {
+    FCT_STAGE_TRACER_AB = 1,
+    FCT_STAGE_EDGE_VFLUX = 2,
+    FCT_STAGE_ADV_FLUX_HOR = 3,
+    FCT_STAGE_ADV_FLUX_VER = 4,
+    FCT_STAGE_FLUX2DTRACER = 5,
+    FCT_STAGE_ALE_RECONSTRUCT = 6,
+    FCT_STAGE_ADV_FLUX_HOR_HO = 7,
+    FCT_STAGE_ADV_FLUX_VER_HO = 8,
+    FCT_STAGE_COMPUTE_FCT_LO = 9,
+    FCT_STAGE_TRACER_GRADIENT_ELEMENTS = 10,
+    FCT_STAGE_FILL_UP_DN_GRAD = 11,
+    FCT_STAGE_ZALESAK_LIMIT = 12,
+    FCT_STAGE_ZALESAK_PM = 13,
+    FCT_STAGE_ZALESAK_APPLY = 14,
+    FCT_STAGE_FLUX2DTRACER_FCT = 15,
+    FCT_STAGE_ADV_TRA_VERT_IMPL = 16
+} fct_stage_id;
+
+#define FCT_STAGE_MAX_IN 8
+typedef struct fct_stage_args {
+    int32_t stage;                     /* fct_stage_id */
+    double dt;
+    const double* in[FCT_STAGE_MAX_IN];
+    double* out[2];
+} fct_stage_args;
+
+int fct_stage(fct_plan* plan, const fct_stage_args* args, void* stream);
+
+/* tracer_ab (fesom_jax/tracer_adv.py:59-66) without a plan: ttf_ab[i] =
+ * -(0.5+0.1)*valuesold[i] + (1.5+0.1)*values[i] for i < count, on the current device. */
+int fct_tracer_ab(const double* values, const double* valuesold, double* ttf_ab,
+                  int64_t count, void* stream);
+
 /* ---- reverse mode ------------------------------------------------------------------
  * Cotangents of one FCT step: for the cotangents g_T_new[j] of T_new[j], returns
  * g_T[j], g_T_old[j] and (summed over the tracers) g_uv and g_w_e, with the conventions
