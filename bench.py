@@ -42,36 +42,63 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(level, nl, ntracers, seed, dt, min_seconds=10.0):
-    """Time the C loop restatement of the FCT step (oracle/fct_loops.c: scalar loops with
-    the original C kernel's semantics, one thread) on a smaller mesh of the same recipe,
-    repeating whole steps until at least ``min_seconds`` of CPU work."""
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
+def cpu_baseline(level, nl, ntracers, seed, dt, reps=5, numpy_level=5):
+    """CPU baselines of the FCT step on this host (the reference's own CPU path is JAX,
+    which is not installed, and its sources hold no C to compile):
+
+    * the C loop restatement of the step (oracle/fct_loops.c: the original C kernel's
+      loops, gcc -O2) in its OpenMP build on every core this process may use, on a
+      smaller mesh of the bench's recipe: median of ``reps`` warm steps (all tracers);
+    * the numpy restatement (oracle/fct_numpy.py, the array-op graph the reference's
+      jnp code runs) on icosa-``numpy_level``: median of ``reps`` warm steps.
+
+    Both results equal the GPU's bit for bit on the same inputs (the tests check the
+    two restatements against each other and the GPU against the numpy one)."""
+    import ctypes
     from fesom_jax_amd.inputs import make_case
     from fesom_jax_amd.meshgen import icosahedral_mesh
     from fesom_jax_amd.perfmodel import updates_per_step
+    from oracle import fct_numpy as O
     from oracle import loops as LP
-    mesh = icosahedral_mesh(level, nl, seed=seed)
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    os.environ["OMP_NUM_THREADS"] = str(cores)
     kinds = ("T", "S", "T", "S")[:ntracers]
-    case = make_case(mesh, seed=seed + 1, dt=dt, tracers=kinds)
-    steps = 0
-    t0 = time.perf_counter()
-    while True:
-        for j in range(ntracers):
-            LP.advect_one_fct(mesh, case["uv"], case["w_e"], case["helem"], case["hnode"],
-                              case["hnode_new"], case["T"][j], case["T_old"][j], dt=dt)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el >= min_seconds:
-            break
-    u = updates_per_step(mesh, ntracers)
-    return {"value": steps * u / el, "unit": "node-level updates/s", "cores": 1, "kind": "port",
-            "sample": (f"C loop restatement of the reference FCT step (oracle/fct_loops.c, "
-                       f"gcc -O2), {steps} steps x {ntracers} tracers on icosa-{level} "
-                       f"({mesh.nod2D} nodes, nl={nl}, {u} updates per step), {el:.1f} s "
-                       f"single-threaded on {cpu_model()} ({os.cpu_count()} logical CPUs on "
-                       f"the host)")}
 
+    def median_rate(mesh, case, fn):
+        def step():
+            for j in range(ntracers):
+                fn(mesh, case["uv"], case["w_e"], case["helem"], case["hnode"],
+                   case["hnode_new"], case["T"][j], case["T_old"][j], dt=dt)
+        step()                                       # warm-up
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            step()
+            ts.append(time.perf_counter() - t)
+        med = float(np.median(ts))
+        return updates_per_step(mesh, ntracers) / med, med
+
+    mesh = icosahedral_mesh(level, nl, seed=seed)
+    case = make_case(mesh, seed=seed + 1, dt=dt, tracers=kinds)
+    omp = lambda *a, **k: LP.advect_one_fct(*a, openmp=True, **k)
+    rate, med = median_rate(mesh, case, omp)
+    try:
+        threads = int(LP._lib(True).omp_get_max_threads())
+    except (AttributeError, OSError):
+        threads = cores
+    nmesh = icosahedral_mesh(numpy_level, nl, seed=seed)
+    ncase = make_case(nmesh, seed=seed + 1, dt=dt, tracers=kinds)
+    nrate, nmed = median_rate(nmesh, ncase, lambda *a, **k: O.advect_one_fct(*a, **k)[0])
+    return {"value": rate, "unit": "node-level updates/s", "cores": threads, "kind": "port",
+            "sample": (f"C loop restatement of the reference FCT step (oracle/fct_loops.c, "
+                       f"gcc -O2 -fopenmp, {threads} threads), median of {reps} warm steps x "
+                       f"{ntracers} tracers on icosa-{level} ({mesh.nod2D} nodes, nl={nl}, "
+                       f"{updates_per_step(mesh, ntracers)} updates per step; {med:.3f} s per "
+                       f"step) on {cpu_model()}"),
+            "numpy": {"value": nrate, "unit": "node-level updates/s", "cores": 1,
+                      "sample": (f"numpy restatement (oracle/fct_numpy.py), median of {reps} "
+                                 f"warm steps on icosa-{numpy_level} ({nmesh.nod2D} nodes, "
+                                 f"nl={nl}; {nmed:.3f} s per step)")}}
 
 def measured_traffic(args, mesh, K):
     """HBM-side bytes per step of the default workload from the committed rocprofv3
@@ -101,6 +128,8 @@ def main():
     ap.add_argument("--dt", type=float, default=2700.0)
     ap.add_argument("--cpu-level", type=int, default=7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-limiter-line", action="store_true",
+                    help="plain: skip the second timing on a limiter-binding input")
     ap.add_argument("--workload",
                     choices=("plain", "batched", "serve", "adjoint", "vdiff", "compute_w", "ice"),
                     default="plain",
@@ -234,8 +263,15 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": measured_traffic(args, mesh, K),
+                "traffic_source": ("not measured by this run: FETCH_SIZE + WRITE_SIZE from "
+                                   "rocprofv3 --pmc runs of this workload, copied from "
+                                   "profiles/fct_step_icosa9_pmc.md"),
                 "kernel": "fct_step (all launches of one step)",
                 "bytes_per_step": B}
+    lim = None
+    if not args.no_limiter_line:
+        lim = limiter_binding(mesh, dict(uv=uv, w_e=w_e, helem=helem, hnode=hnode,
+                                         hnode_new=hnn), K, args, dev, B, U)
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
@@ -253,8 +289,74 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "limiter_binding": lim,
     }
     print(json.dumps(out), flush=True)
+
+
+def limiter_binding(mesh, flow, K, args, dev, B, U):
+    """The same step timed on an input where the Zalesak limiter binds: the reference's
+    limiter-active recipe (a sharp tracer 10 + 5 sin(0.7 n) cos(1.3 k), T_old = 10,
+    fesom_jax/tests/test_tracers.py:578-588) with the flow scaled by 1.5. Also reports
+    the share of wet edge-levels whose antidiffusive flux the limiter reduces and the
+    share of wet node interfaces whose vertical one it reduces (from the single-stage
+    entry points on the same input)."""
+    import torch
+    from fesom_jax_amd import _lib
+    from fesom_jax_amd import tracer_adv as S
+    from fesom_jax_amd.tracer_adv import get_plan
+    wet = torch.from_numpy(mesh.node_layer_mask).to(dev)
+    n = torch.arange(mesh.nod2D, device=dev, dtype=torch.float64)[:, None]
+    k = torch.arange(mesh.nl, device=dev, dtype=torch.float64)[None, :]
+    zero = torch.zeros((), dtype=torch.float64, device=dev)
+    uv, w_e = flow["uv"] * 1.5, flow["w_e"] * 1.5
+    Ts = [torch.where(wet, 10.0 + 5.0 * torch.sin(n * (0.7 + 0.1 * j)) * torch.cos(k * 1.3), zero)
+          for j in range(K)]
+    To = [torch.where(wet, 10.0, 0.0).to(torch.float64) for _ in range(K)]
+    Tn = [torch.empty_like(Ts[0]) for _ in range(K)]
+    plan = get_plan(mesh, dev)
+    arr = lambda ts: ctypes.cast((c_void_p * K)(*[t.data_ptr() for t in ts]),
+                                 ctypes.POINTER(c_void_p))
+    keep = [arr(Ts), arr(To), arr(Tn)]
+    a = _lib.FctStepArgs(ntracers=K, dt=args.dt, uv=uv.data_ptr(), w_e=w_e.data_ptr(),
+                         helem=flow["helem"].data_ptr(), hnode=flow["hnode"].data_ptr(),
+                         hnode_new=flow["hnode_new"].data_ptr(), T=keep[0], T_old=keep[1],
+                         T_new=keep[2], w_i=None, Z3d=None, zbar3=None)
+    stream = torch.cuda.current_stream(dev)
+    sp = c_void_p(stream.cuda_stream)
+    for _ in range(args.warmup):
+        _lib.check(plan.lib.fct_forward(plan.ptr, ctypes.byref(a), sp), "fct_forward")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        _lib.check(plan.lib.fct_forward(plan.ptr, ctypes.byref(a), sp), "fct_forward")
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    # binding shares from the stages of the first tracer's step
+    T, Tol, dt = Ts[0], To[0], args.dt
+    fh = S.adv_flux_hor(mesh, uv, flow["helem"], T)
+    fv = S.adv_flux_ver(mesh, w_e, T)
+    LO = S.compute_fct_lo(mesh, fh, fv, T, flow["hnode"], flow["hnode_new"], dt=dt)
+    ttf = S.tracer_ab(T, Tol)[0]
+    eud = S.fill_up_dn_grad(mesh, S.tracer_gradient_elements(mesh, T))
+    adf_h = S.adv_flux_hor_ho(mesh, uv, flow["helem"], ttf, eud) - fh
+    del eud
+    adf_v = S.adv_flux_ver_ho(mesh, w_e, ttf) - fv
+    lim_h, lim_v = S.zalesak_limit(mesh, T, LO, adf_h, adf_v, flow["hnode_new"], dt=dt)
+    ewet = adf_h != 0.0
+    vwet = torch.from_numpy(mesh.node_iface_mask).to(dev) & (adf_v != 0.0)
+    share_h = float((ewet & (lim_h.abs() < adf_h.abs())).sum()) / max(1, int(ewet.sum()))
+    share_v = float((vwet & (lim_v.abs() < adf_v.abs())).sum()) / max(1, int(vwet.sum()))
+    del fh, fv, LO, ttf, adf_h, adf_v, lim_h, lim_v
+    log(f"limiter-binding input: {ms:.3f} ms/step, limiter reduces {100 * share_h:.1f} % of "
+        f"edge fluxes and {100 * share_v:.1f} % of vertical fluxes")
+    achieved = B / (ms * 1e-3) / 1e9
+    return {"input": "sharp tracer 10 + 5 sin(0.7 n) cos(1.3 k), T_old = 10, flow x 1.5",
+            "ms_per_step": ms, "value": U / (ms * 1e-3),
+            "frac": round(achieved / 8000.0, 5),
+            "limited_share_edge_fluxes": share_h, "limited_share_vertical_fluxes": share_v}
 
 
 def run_batched(args, dev):

@@ -89,7 +89,8 @@ class FctStageArgs(ctypes.Structure):
 
 class FctAdjointArgs(ctypes.Structure):
     _fields_ = [("g_T_new", POINTER(c_void_p)), ("g_T", POINTER(c_void_p)),
-                ("g_T_old", POINTER(c_void_p)), ("g_uv", c_void_p), ("g_w_e", c_void_p)]
+                ("g_T_old", POINTER(c_void_p)), ("g_uv", c_void_p), ("g_w_e", c_void_p),
+                ("g_w_i", c_void_p)]
 
 
 class FctVdiffArgs(ctypes.Structure):
@@ -201,11 +202,27 @@ def mesh_desc_from(mesh, max_tracers: int = 4):
     (the reference's Mesh names, fesom_jax/mesh.py:62-129). Returns the descriptor and
     the arrays it points into; keep both alive until ``fct_plan_create`` returns."""
     import numpy as np
+    N, E, D, L = int(mesh.nod2D), int(mesh.elem2D), int(mesh.edge2D), int(mesh.nl)
     host = {}
     for k in MESH_INT_FIELDS:
         host[k] = np.ascontiguousarray(np.asarray(getattr(mesh, k)), dtype=np.int32)
     for k in MESH_F64_FIELDS:
         host[k] = np.ascontiguousarray(np.asarray(getattr(mesh, k)), dtype=np.float64)
+    # the library reads these element counts from the pointers: refuse any mismatch
+    offsets = host["nod_in_elem2D_offsets"]
+    want = {"edges": (D, 2), "edge_tri": (D, 2), "edge_up_dn_tri": (D, 2), "elem_nodes": (E, 3),
+            "nlevels": (E,), "ulevels": (E,), "nlevels_nod2D": (N,), "nlevels_nod2D_min": (N,),
+            "ulevels_nod2D": (N,), "ulevels_nod2D_max": (N,), "nod_in_elem2D_offsets": (N + 1,),
+            "edge_cross_dxdy": (D, 4), "edge_dxdy": (D, 2), "gradient_sca": (E, 6),
+            "elem_area": (E,), "elem_cos": (E,), "area": (N, L), "areasvol": (N, L),
+            "Z": (L - 1,), "zbar": (L,)}
+    for k, shape in want.items():
+        if host[k].shape != shape:
+            raise ValueError(f"mesh field {k} has shape {host[k].shape}, expected {shape} "
+                             f"(nod2D={N}, elem2D={E}, edge2D={D}, nl={L})")
+    if offsets[0] != 0 or np.any(np.diff(offsets) < 0) or host["nod_in_elem2D"].shape != (
+            int(offsets[-1]),):
+        raise ValueError("nod_in_elem2D_offsets must rise from 0 to len(nod_in_elem2D)")
     desc = FctMeshDesc(nod2D=int(mesh.nod2D), elem2D=int(mesh.elem2D),
                        edge2D=int(mesh.edge2D), nl=int(mesh.nl), max_tracers=int(max_tracers))
     for k, a in host.items():

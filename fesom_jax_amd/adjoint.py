@@ -20,17 +20,26 @@ from .tracer_adv import _pick_device, _ptr_array, _to_device, _torch, advect_fct
 
 
 def advect_fct_vjp(mesh, uv, w_e, helem, hnode, hnode_new, Ts, T_olds, g_T_news, *,
-                   dt=DT_DEFAULT, need_uv=True, need_w=True, plan=None):
+                   dt=DT_DEFAULT, need_uv=True, need_w=True, plan=None, Z3d=None,
+                   zbar3=None, w_i=None):
     """Cotangents of one K-tracer FCT step. Returns ``(g_Ts, g_T_olds, g_uv, g_w_e)``
     (torch tensors on the device; ``g_uv`` / ``g_w_e`` summed over the tracers, or
-    None when not requested). ``plan`` (an ``FctPlan`` of ``mesh`` on the tensors'
-    device) holds the adjoint workspace; by default the mesh's cached plan does."""
+    None when not requested), and ``g_w_i`` as a fifth entry when the implicit split
+    ``w_i`` is given. ``Z3d`` / ``zbar3`` select the moving-level QR4C stencil (they,
+    ``helem`` and the thicknesses are constants). ``plan`` (an ``FctPlan`` of ``mesh``
+    on the tensors' device) holds the adjoint workspace; by default the mesh's cached
+    plan does."""
     torch = _torch()
     device = _pick_device(Ts[0], uv)
     N, E, L = int(mesh.nod2D), int(mesh.elem2D), int(mesh.nl)
     ns, es = (N, L), (E, L)
+    if (Z3d is None) != (zbar3 is None):
+        raise ValueError("Z3d and zbar3 must be given together")
     uv_t = _to_device(uv, "uv", device, (E, L, 2))
     w_t = _to_device(w_e, "w_e", device, ns)
+    wi_t = None if w_i is None else _to_device(w_i, "w_i", device, ns)
+    z_t = None if Z3d is None else _to_device(Z3d, "Z3d", device, ns)
+    zb_t = None if zbar3 is None else _to_device(zbar3, "zbar3", device, ns)
     he_t = _to_device(helem, "helem", device, es)
     hn_t = _to_device(hnode, "hnode", device, ns)
     hnn_t = _to_device(hnode_new, "hnode_new", device, ns)
@@ -44,6 +53,7 @@ def advect_fct_vjp(mesh, uv, w_e, helem, hnode, hnode_new, Ts, T_olds, g_T_news,
     gTo = [torch.empty(ns, **f64) for _ in T_t]
     g_uv = torch.empty((E, L, 2), **f64) if need_uv else None
     g_w = torch.empty(ns, **f64) if need_w else None
+    g_wi = None if wi_t is None else torch.empty(ns, **f64)
     if plan is None:
         plan = get_plan(mesh, device)
     elif plan.device != device or plan.nod2D != N or plan.nl != L:
@@ -53,23 +63,28 @@ def advect_fct_vjp(mesh, uv, w_e, helem, hnode, hnode_new, Ts, T_olds, g_T_news,
     fwd = _lib.FctStepArgs(ntracers=len(T_t), dt=float(dt), uv=uv_t.data_ptr(),
                            w_e=w_t.data_ptr(), helem=he_t.data_ptr(), hnode=hn_t.data_ptr(),
                            hnode_new=hnn_t.data_ptr(), T=P(keep[0]), T_old=P(keep[1]),
-                           T_new=None, w_i=None, Z3d=None, zbar3=None)
+                           T_new=None, w_i=None if wi_t is None else wi_t.data_ptr(),
+                           Z3d=None if z_t is None else z_t.data_ptr(),
+                           zbar3=None if zb_t is None else zb_t.data_ptr())
     adj = _lib.FctAdjointArgs(g_T_new=P(keep[2]), g_T=P(keep[3]), g_T_old=P(keep[4]),
                               g_uv=None if g_uv is None else g_uv.data_ptr(),
-                              g_w_e=None if g_w is None else g_w.data_ptr())
+                              g_w_e=None if g_w is None else g_w.data_ptr(),
+                              g_w_i=None if g_wi is None else g_wi.data_ptr())
     with torch.cuda.device(device), plan.on(torch.cuda.current_stream(device)):
         s = torch.cuda.current_stream(device).cuda_stream
         _lib.check(plan.lib.fct_backward(plan.ptr, ctypes.byref(fwd), ctypes.byref(adj),
                                          c_void_p(s)), "fct_backward")
+    if g_wi is not None:
+        return gT, gTo, g_uv, g_w, g_wi
     return gT, gTo, g_uv, g_w
 
 
 def advect_one_fct_vjp(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, g_T_new, *,
-                       dt=DT_DEFAULT):
-    """Single-tracer form: ``(g_T, g_T_old, g_uv, g_w_e)``."""
-    gT, gTo, g_uv, g_w = advect_fct_vjp(mesh, uv, w_e, helem, hnode, hnode_new, [T], [T_old],
-                                        [g_T_new], dt=dt)
-    return gT[0], gTo[0], g_uv, g_w
+                       dt=DT_DEFAULT, Z3d=None, zbar3=None, w_i=None):
+    """Single-tracer form: ``(g_T, g_T_old, g_uv, g_w_e)`` (+ ``g_w_i`` with ``w_i``)."""
+    out = advect_fct_vjp(mesh, uv, w_e, helem, hnode, hnode_new, [T], [T_old], [g_T_new],
+                         dt=dt, Z3d=Z3d, zbar3=zbar3, w_i=w_i)
+    return (out[0][0], out[1][0]) + tuple(out[2:])
 
 
 def _fct_step_function():

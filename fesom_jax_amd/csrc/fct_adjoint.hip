@@ -1,4 +1,5 @@
-// Reverse mode of one FP64 FCT step (w_i = NULL, static Z) for gfx950.
+// Reverse mode of one FP64 FCT step for gfx950 (with or without the implicit split
+// w_i and the moving-level stencil Z3d/zbar3, which are treated as constants).
 //
 // The forward tape is the phased path's dense intermediates of the same step (vflux,
 // LO, tr_xy, gnode, adf_h, adf_v, the node bounds tvmax/tvmin, P+/P-, the limiter
@@ -289,15 +290,22 @@ struct QrTerms {
     double a0 = 0.0, am1 = 0.0, am2 = 0.0, ap1 = 0.0, w = 0.0;
 };
 
+// QR4C inputs that do not depend on the tracer: the vertical velocity of adf_v (w_e, or
+// w_e + w_i under the implicit split) and the moving-level stencil (Z3d, zbar3), or
+// NULL for the mesh's static one
+struct QrFlow {
+    const double *w, *w_i, *Z3d, *zbar3;
+};
+
 __device__ __forceinline__ QrTerms qr_vjp(const DevMesh& m, const double* T, const double* Told,
-                                          const double* w, const double* g_adfv, size_t b0,
+                                          const QrFlow& f, const double* g_adfv, size_t b0,
                                           int j, int2 nr) {
     QrTerms r;
     const int L = m.L;
     if (!in_iface(nr, j) || j == nr.y) return r;
     const size_t i = b0 + j, im = b0 + (j > 0 ? j - 1 : 0);
     const double gav = g_adfv[i];
-    const double W = w[i], ar = m.area[i];
+    const double W = f.w_i ? f.w[i] + f.w_i[i] : f.w[i], ar = m.area[i];
     const double A = ab2a(Told[i], T[i]);
     const double Am1 = ab2a(Told[im], T[im]);
     if (j == nr.x) {
@@ -314,8 +322,14 @@ __device__ __forceinline__ QrTerms qr_vjp(const DevMesh& m, const double* T, con
         const size_t ip = b0 + (j + 1 < L ? j + 1 : L - 1);
         const double Am2 = ab2a(Told[im2], T[im2]);
         const double Ap1 = (j + 1 < L) ? ab2a(Told[ip], T[ip]) : 0.0;
-        const double Znz = m.Zs[j], Zm1 = m.Zs[L + j], Zm2 = m.Zs[2 * L + j],
-                     Zp1 = m.Zs[3 * L + j], zb = m.zbar[j];
+        double Znz, Zm1, Zm2, Zp1, zb;
+        if (f.Z3d) {
+            Znz = f.Z3d[i]; Zm1 = f.Z3d[im]; Zm2 = f.Z3d[im2]; Zp1 = f.Z3d[ip];
+            zb = f.zbar3[i];
+        } else {
+            Znz = m.Zs[j]; Zm1 = m.Zs[L + j]; Zm2 = m.Zs[2 * L + j]; Zp1 = m.Zs[3 * L + j];
+            zb = m.zbar[j];
+        }
         const double dc = nz_safe(Zm1 - Znz), du = nz_safe(Znz - Zp1), dd = nz_safe(Zm2 - Zm1);
         const double qc = (Am1 - A) / dc, qu = (A - Ap1) / du, qd = (Am2 - Am1) / dd;
         const double Tmean1 = A + (2.0 * qc + qu) * (zb - Znz) / 3.0;
@@ -339,21 +353,23 @@ __device__ __forceinline__ QrTerms qr_vjp(const DevMesh& m, const double* T, con
 }
 
 // A[k] gathers the terms of interfaces k, k+1, k+2 and k-1; w[k] those of interface k
+// (to w_e and, under the split, to w_i as well)
 __global__ void __launch_bounds__(256)
-ka_qr(DevMesh m, const double* __restrict__ T, const double* __restrict__ Told,
-      const double* __restrict__ w, const double* __restrict__ g_adfv, double* __restrict__ gA,
-      double* __restrict__ gw) {
+ka_qr(DevMesh m, const double* __restrict__ T, const double* __restrict__ Told, QrFlow f,
+      const double* __restrict__ g_adfv, double* __restrict__ gA, double* __restrict__ gw,
+      double* __restrict__ gwi) {
     FLAT_NL(m);
     const int L = m.L;
     const int2 nr = m.nlev[n];
     const size_t b0 = (size_t)n * L;
-    const QrTerms t0 = qr_vjp(m, T, Told, w, g_adfv, b0, k, nr);
+    const QrTerms t0 = qr_vjp(m, T, Told, f, g_adfv, b0, k, nr);
     double ga = t0.a0;
-    if (k + 1 < L) ga += qr_vjp(m, T, Told, w, g_adfv, b0, k + 1, nr).am1;
-    if (k + 2 < L) ga += qr_vjp(m, T, Told, w, g_adfv, b0, k + 2, nr).am2;
-    if (k > 0) ga += qr_vjp(m, T, Told, w, g_adfv, b0, k - 1, nr).ap1;
+    if (k + 1 < L) ga += qr_vjp(m, T, Told, f, g_adfv, b0, k + 1, nr).am1;
+    if (k + 2 < L) ga += qr_vjp(m, T, Told, f, g_adfv, b0, k + 2, nr).am2;
+    if (k > 0) ga += qr_vjp(m, T, Told, f, g_adfv, b0, k - 1, nr).ap1;
     gA[i] = ga;
     gw[i] += t0.w;
+    if (gwi) gwi[i] += t0.w;
 }
 
 // element extremum of max/min(LO, T) over its vertices at level k (a2)
@@ -473,6 +489,136 @@ ka_fv(DevMesh m, const double* __restrict__ T, const double* __restrict__ w,
     gT[i] += t;
     const double gwp = gs * T[i], gwm = gs * T[im];
     gw[i] += gwp + gwm + gabs(gwp - gwm, W);
+}
+
+// The same under the implicit split, where the two fluxes differ: LO's uses w_e and
+// takes the LO-divergence cotangent, adf_v's uses w_e + w_i and takes -g_adf_v
+// (tracer_adv.py:551-552, 568-572).
+__device__ __forceinline__ double fv_cot_part(const DevMesh& m, const double* g, size_t b0, int k,
+                                              int2 nr, bool div) {
+    const size_t i = b0 + k;
+    const double gd = g[i];
+    const double v = div ? (k > 0 ? gd + (-g[i - 1]) : gd) : gd;
+    return -0.5 * ((in_layer(nr, k) ? v : 0.0) * m.area[i]);
+}
+
+__global__ void __launch_bounds__(256)
+ka_fv_split(DevMesh m, const double* __restrict__ T, const double* __restrict__ w_e,
+            const double* __restrict__ w_i, const double* __restrict__ g_fv,
+            const double* __restrict__ g_div, double* __restrict__ gT, double* __restrict__ gw,
+            double* __restrict__ gwi) {
+    FLAT_NL(m);
+    const int L = m.L;
+    const int2 nr = m.nlev[n];
+    const size_t b0 = (size_t)n * L, im = b0 + (k > 0 ? k - 1 : 0);
+    const double gl = fv_cot_part(m, g_div, b0, k, nr, true);
+    const double gh = fv_cot_part(m, g_fv, b0, k, nr, false);
+    const double We = w_e[i], Wf = w_e[i] + w_i[i];
+    const double ae = fabs(We), af = fabs(Wf);
+    double t = gl * (We + ae) + gh * (Wf + af);
+    if (k == 0) t += gl * (We - ae) + gh * (Wf - af);
+    if (k + 1 < L) {
+        const double We1 = w_e[i + 1], Wf1 = w_e[i + 1] + w_i[i + 1];
+        t += fv_cot_part(m, g_div, b0, k + 1, nr, true) * (We1 - fabs(We1)) +
+             fv_cot_part(m, g_fv, b0, k + 1, nr, false) * (Wf1 - fabs(Wf1));
+    }
+    gT[i] += t;
+    const double lp = gl * T[i], lm = gl * T[im];
+    const double hp = gh * T[i], hm = gh * T[im];
+    const double gf = hp + hm + gabs(hp - hm, Wf);
+    gw[i] += (lp + lm + gabs(lp - lm, We)) + gf;
+    gwi[i] += gf;
+}
+
+// ---- adv_tra_vert_impl on LO (tracer_adv.py:555-556, 487-529) ------------------------
+// One thread per column: the Thomas recurrence of ops.tdma (ops.py:121-168) is rerun
+// from the tape's LO before the solve (X), then reversed step by step as jax.grad
+// reverses its two scans. On entry gX holds the cotangent of the solved LO; on exit the
+// cotangent of X. The w_i cotangent is added to gwi. cp, dp, xs, gd: [N*L] scratch.
+__global__ void __launch_bounds__(256)
+ka_vimpl(DevMesh m, const double* __restrict__ w_i, const double* __restrict__ hnn, double dt,
+         const double* __restrict__ X, double* __restrict__ gX, double* __restrict__ gwi,
+         double* __restrict__ cp, double* __restrict__ dp, double* __restrict__ xs,
+         double* __restrict__ gd) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= m.N) return;
+    const int L = m.L;
+    const int2 nr = m.nlev[n];
+    const size_t b0 = (size_t)n * L;
+    auto rhs_at = [&](const VimplRow& r, int k) {
+        const size_t i = b0 + k;
+        const double Td = X[k > 0 ? i - 1 : i], Tc = X[i], Tu = k + 1 < L ? X[i + 1] : 0.0;
+        return r.valid ? -(r.a * Td + r.b_adv * Tc + r.c * Tu) : 0.0;
+    };
+    // forward sweep and back substitution (x kept in xs)
+    double cprev = 0.0, dprev = 0.0;
+    for (int k = 0; k < L; ++k) {
+        const VimplRow r = vimpl_row(m, w_i, hnn, dt, b0, k, nr);
+        const double mm = r.b - r.a * cprev;
+        cprev = r.c / mm;
+        dprev = (rhs_at(r, k) - r.a * dprev) / mm;
+        cp[b0 + k] = cprev;
+        dp[b0 + k] = dprev;
+    }
+    double xn = 0.0;
+    for (int k = L - 1; k >= 0; --k) {
+        xn = dp[b0 + k] - cp[b0 + k] * xn;
+        xs[b0 + k] = xn;
+    }
+    // back substitution reversed: x[k] = dp[k] - cp[k] x[k+1]; gd <- cot of dp,
+    // xs <- cot of cp (x[k+1] is read before level k+1 is overwritten)
+    double carry = 0.0;
+    for (int k = 0; k < L; ++k) {
+        const size_t i = b0 + k;
+        const double G = (in_layer(nr, k) ? gX[i] : 0.0) + carry;
+        gd[i] = G;
+        const double x1 = k + 1 < L ? xs[i + 1] : 0.0;
+        xs[i] = -G * x1;
+        carry = -G * cp[i];
+    }
+    // forward sweep reversed; gX[k] keeps the identity term of X + dT and gathers the
+    // right-hand side's transpose; gwi[k] gathers the upper face (w_i[k]) of row k and
+    // the lower face of row k-1
+    double gcc = 0.0, gdc = 0.0;
+    for (int k = L - 1; k >= 0; --k) {
+        const size_t i = b0 + k;
+        const VimplRow r = vimpl_row(m, w_i, hnn, dt, b0, k, nr);
+        const double cpv = k > 0 ? cp[i - 1] : 0.0, dpv = k > 0 ? dp[i - 1] : 0.0;
+        const double mm = r.b - r.a * cpv;
+        const double rhs = rhs_at(r, k);
+        const double num = rhs - r.a * dpv;
+        const double GC = xs[i] + gcc, GD = gd[i] + gdc;
+        const double g_num = GD / mm;
+        double g_m = (-GD * num) * (1.0 / (mm * mm));
+        double g_a = -g_num * dpv;
+        gdc = -g_num * r.a;
+        double g_c = GC / mm;
+        g_m = g_m + (-GC * r.c) * (1.0 / (mm * mm));
+        const double g_b = g_m;
+        g_a += -g_m * cpv;
+        gcc = -g_m * r.a;
+        // rhs = where(valid, -(a X[k-1] + b_adv X[k] + c X[k+1]), 0)
+        const double gs = -(r.valid ? g_num : 0.0);
+        const double Td = X[k > 0 ? i - 1 : i], Tc = X[i], Tu = k + 1 < L ? X[i + 1] : 0.0;
+        g_a += gs * Td;
+        double g_badv = gs * Tc;
+        g_c += gs * Tu;
+        if (k > 0) gX[i - 1] += gs * r.a; else gX[i] += gs * r.a;
+        gX[i] += gs * r.b_adv;
+        if (k + 1 < L) gX[i + 1] += gs * r.c;
+        g_badv += r.valid ? g_b : 0.0;
+        const double g_aadv = (r.valid ? g_a : 0.0) * r.v_top;
+        const double g_cadv = (r.valid ? g_c : 0.0) * r.v_bot;
+        double g_wu = r.surf ? 0.0 : gmin_x(g_aadv, r.wu, 0.0);
+        const double gt = g_badv * r.v_top;
+        g_wu += r.surf ? gt : gmax_x(gt, r.wu, 0.0);
+        const double gbb = g_badv * r.v_bot;
+        double g_wd = r.bot ? 0.0 : -gmin_x(gbb, r.wd, 0.0);
+        g_wd += r.bot ? 0.0 : -gmax_x(g_cadv, r.wd, 0.0);
+        // wd of row k is w_i[k+1] (0 on the last level)
+        if (k + 1 < L) gwi[i + 1] += g_wd;
+        gwi[i] += g_wu;
+    }
 }
 
 // ---- edges: adf_h = HO(A, eud) - fh(T) and fh's LO-divergence cotangent ------------
@@ -648,19 +794,36 @@ inline unsigned grid(int64_t count) { return (unsigned)((count + 255) / 256); }
 
 int launch_backward(const DevMesh& m, const AdjointWork& w, const fct_step_args* a,
                     const double* const* g_out, double* const* gT, double* const* gTold,
-                    double* g_uv, double* g_w, hipStream_t s) {
+                    double* g_uv, double* g_w, double* g_wi, hipStream_t s) {
     const int64_t NL = (int64_t)m.N * m.L, DL = (int64_t)m.D * m.L, EL = (int64_t)m.E * m.L;
     const dim3 b(256);
+    const bool split = a->w_i != nullptr;
     // zero fills by kernel (no memset nodes if a caller captures this in a graph)
     double* gw = g_w ? g_w : w.gw;
+    double* gwi = split ? (g_wi ? g_wi : w.gwi) : nullptr;
     hipLaunchKernelGGL(ka_zero, dim3(grid(DL)), b, 0, s, w.g_vf, DL);
     hipLaunchKernelGGL(ka_zero, dim3(grid(NL)), b, 0, s, gw, NL);
+    if (split) hipLaunchKernelGGL(ka_zero, dim3(grid(NL)), b, 0, s, gwi, NL);
     int rc = launch_vflux(m, a, w.vflux, s);
     if (rc) return rc;
+    fct_step_args explicit_part = *a;   // LO before the implicit solve
+    explicit_part.w_i = nullptr;
+    const QrFlow qf{a->w_e, a->w_i, a->Z3d, a->zbar3};
     for (int j = 0; j < a->ntracers; ++j) {
-        // forward tape of tracer j (phased kernels: dense intermediates)
-        if ((rc = launch_lo(m, a, j, w.vflux, w.LO, nullptr, nullptr, s)) ||
-            (rc = launch_grad(m, a, j, w.trxy, s)) ||
+        // forward tape of tracer j (phased kernels: dense intermediates); under the split
+        // LO before the solve is kept in LOx and solved into LO (sh_x, sh_n as scratch)
+        if (split) {
+            if ((rc = launch_lo(m, &explicit_part, j, w.vflux, w.LOx, nullptr, nullptr, s)))
+                return rc;
+            if (hipMemcpyAsync(w.LO, w.LOx, NL * sizeof(double), hipMemcpyDeviceToDevice, s) !=
+                hipSuccess)
+                return set_error(FCT_E_HIP, "adjoint tape copy failed");
+            if ((rc = launch_vert_impl(m, a->w_i, a->hnode_new, a->dt, w.LO, w.sh_x, w.sh_n, s)))
+                return rc;
+        } else if ((rc = launch_lo(m, a, j, w.vflux, w.LO, nullptr, nullptr, s))) {
+            return rc;
+        }
+        if ((rc = launch_grad(m, a, j, w.trxy, s)) ||
             (rc = launch_pm(m, a, j, w.vflux, w.LO, w.trxy, w.gnode, w.adfh, w.adfv, w.tvmax,
                             w.tvmin, w.fplus, w.fminus, s)))
             return rc;
@@ -681,16 +844,24 @@ int launch_backward(const DevMesh& m, const AdjointWork& w, const fct_step_args*
         hipLaunchKernelGGL(ka_adfv, dim3(grid(NL)), b, 0, s, m, w.adfv, w.g_fp, w.g_fm,
                            w.g_fmax, w.g_fmin, w.tvmax, w.tvmin, w.g_adfv, w.g_fv, w.g_tvx,
                            w.g_tvn);
-        hipLaunchKernelGGL(ka_qr, dim3(grid(NL)), b, 0, s, m, a->T[j], a->T_old[j], a->w_e,
-                           w.g_adfv, w.gA, gw);
+        hipLaunchKernelGGL(ka_qr, dim3(grid(NL)), b, 0, s, m, a->T[j], a->T_old[j], qf,
+                           w.g_adfv, w.gA, gw, gwi);
         hipLaunchKernelGGL(ka_share, dim3(grid(NL)), b, 0, s, m, w.LO, a->T[j], w.tvmax,
                            w.tvmin, w.g_tvx, w.g_tvn, w.sh_x, w.sh_n);
         hipLaunchKernelGGL(ka_vertex, dim3(grid(NL)), b, 0, s, m, w.LO, a->T[j], w.tvmax,
                            w.tvmin, w.sh_x, w.sh_n, w.gLO, gT[j]);
+        if (split)   // the solve's transpose (sh_x, sh_n, g_tvx, g_tvn are free here)
+            hipLaunchKernelGGL(ka_vimpl, dim3((unsigned)((m.N + 255) / 256)), b, 0, s, m, a->w_i,
+                               a->hnode_new, a->dt, w.LOx, w.gLO, gwi, w.sh_x, w.sh_n, w.g_tvx,
+                               w.g_tvn);
         hipLaunchKernelGGL(ka_lo, dim3(grid(NL)), b, 0, s, m, a->hnode, a->hnode_new, a->dt,
                            w.gLO, w.g_div, gT[j]);
-        hipLaunchKernelGGL(ka_fv, dim3(grid(NL)), b, 0, s, m, a->T[j], a->w_e, w.g_fv, w.g_div,
-                           gT[j], gw);
+        if (split)
+            hipLaunchKernelGGL(ka_fv_split, dim3(grid(NL)), b, 0, s, m, a->T[j], a->w_e, a->w_i,
+                               w.g_fv, w.g_div, gT[j], gw, gwi);
+        else
+            hipLaunchKernelGGL(ka_fv, dim3(grid(NL)), b, 0, s, m, a->T[j], a->w_e, w.g_fv,
+                               w.g_div, gT[j], gw);
         hipLaunchKernelGGL(ka_edge, dim3(grid(DL)), b, 0, s, m, a->T[j], a->T_old[j], w.vflux,
                            (const double2*)w.trxy, (const double2*)w.gnode, w.g_adfh, w.g_div,
                            w.g_vf, w.eA, w.eT, (double4*)w.eud);

@@ -145,14 +145,33 @@ int validate_mesh(const fct_mesh_desc* d) {
             if (v < 0 || v >= N)
                 return set_error(FCT_E_INVALID, "element %d has bad node %d", e, v);
         }
-        if (d->ulevels[e] < 1 || d->nlevels[e] > L)
-            return set_error(FCT_E_INVALID, "element %d has bad levels (%d, %d)", e,
-                             d->ulevels[e], d->nlevels[e]);
+        // every level bound in [1, nl] and top <= bottom (the fused tables pack them into
+        // bytes, so an out-of-range value must not get that far)
+        if (d->ulevels[e] < 1 || d->ulevels[e] > L || d->nlevels[e] < d->ulevels[e] ||
+            d->nlevels[e] > L)
+            return set_error(FCT_E_INVALID, "element %d has bad levels (%d, %d), nl = %d", e,
+                             d->ulevels[e], d->nlevels[e], L);
     }
     for (int n = 0; n < N; ++n) {
-        if (d->ulevels_nod2D[n] < 1 || d->nlevels_nod2D[n] > L)
-            return set_error(FCT_E_INVALID, "node %d has bad levels (%d, %d)", n,
-                             d->ulevels_nod2D[n], d->nlevels_nod2D[n]);
+        const int lv[4] = {d->ulevels_nod2D[n], d->ulevels_nod2D_max[n], d->nlevels_nod2D_min[n],
+                           d->nlevels_nod2D[n]};
+        for (int q = 0; q < 4; ++q)
+            if (lv[q] < 1 || lv[q] > L)
+                return set_error(FCT_E_INVALID, "node %d has a level bound %d outside [1, %d]", n,
+                                 lv[q], L);
+        if (d->nlevels_nod2D[n] < d->ulevels_nod2D[n] ||
+            d->ulevels_nod2D_max[n] < d->ulevels_nod2D[n] ||
+            d->nlevels_nod2D_min[n] > d->nlevels_nod2D[n])
+            return set_error(FCT_E_INVALID, "node %d has inconsistent levels (%d, %d, %d, %d)", n,
+                             lv[0], lv[1], lv[2], lv[3]);
+    }
+    if (d->nod_in_elem2D_offsets) {
+        const int32_t* off = d->nod_in_elem2D_offsets;
+        if (off[0] != 0)
+            return set_error(FCT_E_INVALID, "nod_in_elem2D_offsets[0] = %d, expected 0", off[0]);
+        for (int n = 0; n < N; ++n)
+            if (off[n + 1] < off[n])
+                return set_error(FCT_E_INVALID, "nod_in_elem2D_offsets decrease at node %d", n);
     }
     return FCT_OK;
 }
@@ -596,7 +615,7 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
     p->host_edge_tri.assign(d->edge_tri, d->edge_tri + 2 * (size_t)D);
     p->host_updn.assign(d->edge_up_dn_tri, d->edge_up_dn_tri + 2 * (size_t)D);
     p->tab.dm = dm;
-    // locality order and edge patches of the adf_h patch kernel
+    // edge patches of the adf_h patch kernel
     LocalityHost loc;
     if (dm) {
         std::vector<unsigned char> has_updn(D);
@@ -609,14 +628,15 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
         }
         p->tab.ep.np = (int)loc.ptab.size();
     }
-    // node passes for nl > 64: shallow columns, then deep ones, each in the locality
-    // order (patch by patch)
+    // node passes for nl > 64: shallow columns, then deep ones, each in node order (the
+    // patch order measured slower for these passes on meshes numbered with locality, and
+    // no faster on shuffled ones, where the scattered column addresses dominate)
     std::vector<int> h_nlist;
     if (dm && L > 64) {
-        for (int n : loc.node_order)
+        for (int n = 0; n < N; ++n)
             if (d->nlevels_nod2D[n] - 1 < 64) h_nlist.push_back(n);
         p->tab.nshallow = (int)h_nlist.size();
-        for (int n : loc.node_order)
+        for (int n = 0; n < N; ++n)
             if (d->nlevels_nod2D[n] - 1 >= 64) h_nlist.push_back(n);
     }
     if (dm && D && ((rc = dev_upload(p, &p->tab.ep.ptab, loc.ptab.data(), loc.ptab.size())) ||
@@ -1242,9 +1262,8 @@ int fct_backward(fct_plan* p, const fct_step_args* a, const fct_adjoint_args* g,
         return set_error(FCT_E_INVALID, "need ntracers > 0 and T, T_old arrays");
     if (!a->uv || !a->w_e || !a->helem || !a->hnode || !a->hnode_new)
         return set_error(FCT_E_INVALID, "NULL flow or thickness field");
-    if (a->w_i) return set_error(FCT_E_UNSUPPORTED, "the adjoint does not support w_i");
-    if (a->Z3d || a->zbar3)
-        return set_error(FCT_E_UNSUPPORTED, "the adjoint supports the static Z stencil only");
+    if ((a->Z3d == nullptr) != (a->zbar3 == nullptr))
+        return set_error(FCT_E_INVALID, "Z3d and zbar3 must be given together");
     if (!g->g_T_new || !g->g_T || !g->g_T_old)
         return set_error(FCT_E_INVALID, "g_T_new, g_T and g_T_old arrays must be given");
     for (int j = 0; j < a->ntracers; ++j)
@@ -1252,8 +1271,12 @@ int fct_backward(fct_plan* p, const fct_step_args* a, const fct_adjoint_args* g,
             return set_error(FCT_E_INVALID, "tracer %d has a NULL field", j);
     int rc = ensure_adjoint(p);
     if (rc) return rc;
+    const size_t NL = (size_t)p->m.N * p->m.L;
+    if (a->w_i && !p->aw.LOx &&
+        ((rc = dev_alloc(p, &p->aw.LOx, NL)) || (rc = dev_alloc(p, &p->aw.gwi, NL))))
+        return rc;
     return launch_backward(p->m, p->aw, a, g->g_T_new, g->g_T, g->g_T_old, g->g_uv, g->g_w_e,
-                           (hipStream_t)stream);
+                           g->g_w_i, (hipStream_t)stream);
 }
 
 }  // extern "C"

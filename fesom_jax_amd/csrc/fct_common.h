@@ -96,12 +96,11 @@ struct NodeTab {
     EdgePatches ep;           // edge patches of the adf_h patch kernel
 };
 
-// Host side of the locality order (fct_locality.hip): `node_order` lists the nodes
-// patch by patch; the edge-patch tables are as in EdgePatches. `udv` and `has_updn`
-// are the up/down triangle vertex table and whether the edge has both triangles, `utri`
-// the level range in which an edge uses its up/down triangles' gradients.
+// Host side of the edge patches (fct_locality.hip), tables as in EdgePatches. `udv` and
+// `has_updn` are the up/down triangle vertex table and whether the edge has both
+// triangles, `utri` the level range in which an edge uses its up/down triangles'
+// gradients.
 struct LocalityHost {
-    std::vector<int> node_order;
     std::vector<int4> ptab;
     std::vector<int> pnodes, elist;
     std::vector<int2> eslot;
@@ -114,6 +113,39 @@ __device__ __forceinline__ bool in_layer(int2 r, int k) { return k >= r.x && k <
 __device__ __forceinline__ bool in_iface(int2 r, int k) { return k >= r.x && k <= r.y; }
 __device__ __forceinline__ double nz_safe(double x) { return x == 0.0 ? 1.0 : x; }
 
+// One row of adv_tra_vert_impl's tridiagonal system (fesom_jax/tracer_adv.py:487-529)
+// at level k of the column starting at b0: sub/super-diagonal a, c, diagonal b, the
+// advective part b_adv of the diagonal, and the pieces the reverse mode needs.
+struct VimplRow {
+    double a, b, c, b_adv, v_top, v_bot, wu, wd;
+    bool surf, bot, valid;
+};
+__device__ __forceinline__ VimplRow vimpl_row(const DevMesh& m, const double* __restrict__ w_i,
+                                              const double* __restrict__ hnn, double dt,
+                                              size_t b0, int k, int2 nr) {
+    const int L = m.L;
+    const size_t i = b0 + k;
+    VimplRow r;
+    const double av = m.areasvol[i];
+    const double safe_av = av > 0.0 ? av : 1.0;
+    r.v_top = dt * m.area[i] / safe_av;
+    r.v_bot = dt * (k + 1 < L ? m.area[i + 1] : 0.0) / safe_av;
+    r.wu = w_i[i];
+    r.wd = k + 1 < L ? w_i[i + 1] : 0.0;
+    r.surf = k == nr.x;
+    r.bot = k == nr.y - 1;
+    r.valid = in_layer(nr, k);
+    const double a_adv = (r.surf ? 0.0 : fmin(0.0, r.wu)) * r.v_top;
+    const double b_top = (r.surf ? r.wu : fmax(0.0, r.wu)) * r.v_top;
+    const double b_bot = (r.bot ? 0.0 : -fmin(0.0, r.wd)) * r.v_bot;
+    const double c_adv = (r.bot ? 0.0 : -fmax(0.0, r.wd)) * r.v_bot;
+    r.b_adv = b_top + b_bot;
+    r.a = r.valid ? a_adv : 0.0;
+    r.c = r.valid ? c_adv : 0.0;
+    r.b = r.valid ? hnn[i] + r.b_adv : 1.0;
+    return r;
+}
+
 // Host helpers (defined in fct_api.hip).
 int set_error(int code, const char* fmt, ...);
 int check_launch(const char* what);
@@ -123,6 +155,9 @@ int launch_vflux(const DevMesh& m, const fct_step_args* a, double* vflux, hipStr
 int launch_lo(const DevMesh& m, const fct_step_args* a, int j, const double* vflux,
               double* LO, double* scr0, double* scr1, hipStream_t s);
 int launch_grad(const DevMesh& m, const fct_step_args* a, int j, double* tr, hipStream_t s);
+// adv_tra_vert_impl of X in place (cp, dp: [N*L] scratch)
+int launch_vert_impl(const DevMesh& m, const double* w_i, const double* hnn, double dt,
+                     double* X, double* cp, double* dp, hipStream_t s);
 int launch_pm(const DevMesh& m, const fct_step_args* a, int j, const double* vflux,
               const double* LO, const double* tr, double* gnode, double* adfh, double* adfv,
               double* tvmax, double* tvmin, double* fplus, double* fminus, hipStream_t s);
@@ -159,11 +194,14 @@ struct AdjointWork {
     double* g_tr;
     // element -> (edge, up/down role) and element -> (edge, side) lists
     const int *ud_off, *ud_ent, *et_off, *et_ent;
+    // implicit split (allocated on the first backward with w_i): LO before the
+    // implicit solve, and the w_i cotangent when the caller does not take it
+    double *LOx = nullptr, *gwi = nullptr;
 };
 
 int launch_backward(const DevMesh& m, const AdjointWork& w, const fct_step_args* a,
                     const double* const* g_out, double* const* gT, double* const* gTold,
-                    double* g_uv, double* g_w, hipStream_t s);
+                    double* g_uv, double* g_w, double* g_wi, hipStream_t s);
 
 // Launch-shape knobs of the fused path (fct_plan_set_tuning / fct_batch_set_tuning);
 // results do not depend on them.

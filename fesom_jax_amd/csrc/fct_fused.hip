@@ -223,11 +223,8 @@ __device__ __forceinline__ void vflux_group(const FusedArgs& a, int g) {
     int kmax = 0;
 #pragma unroll
     for (int q = 0; q < EPW; ++q) {
-        // edges in the plan's locality order where it has one (so that a triangle's
-        // three edges are computed close together in time, whatever the numbering)
         const int eq = g * EPW + q;
-        const int ec = eq < m.D ? eq : m.D - 1;
-        e[q] = a.tab.ep.elist ? ldc(a.tab.ep.elist, ec) : ec;
+        e[q] = eq < m.D ? eq : m.D - 1;
         et[q] = ldc(m.edge_tri, e[q]);
         kr[q] = eq < m.D ? ldc(a.ekr, e[q]) : make_int2(0, 0);
         c[q] = ldc(m.cross, e[q]);

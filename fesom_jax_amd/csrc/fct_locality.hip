@@ -1,14 +1,14 @@
-// Locality order of the fused path, built on the host with the plan.
+// Edge patches of the adf_h patch kernel, built on the host with the plan.
 //
 // The caller's node and edge numbering is kept for every array (results are written
 // where the caller expects them, and every per-node sum still runs over the node's
-// (edge, side) list in increasing 2*edge + side, the reference's scatter order), but
-// the order in which the kernels visit nodes and edges comes from the mesh graph: nodes
-// are grouped into compact patches grown breadth-first, and each patch takes the edges
-// whose first node it holds. Patches are grouped the same way twice more (up to 16 per
-// group, then up to 16 groups per super-group), and the order follows that hierarchy,
-// so that the nodes and edges the GPU works on at the same time are close on the mesh
-// at every scale whatever the numbering.
+// (edge, side) list in increasing 2*edge + side, the reference's scatter order). Only
+// the adf_h patch kernel visits edges in another order: nodes are grouped into compact
+// balls grown breadth-first, each ball takes the edges whose first node it holds, and
+// the balls are grouped the same way twice more (up to 16 per group, then up to 16
+// groups per super-group), so that the edges one workgroup stages together share their
+// node columns whatever the numbering. The node passes and the vertical-flux kernel keep
+// the id order: on the meshes measured here the patch order made them slower.
 
 #include <algorithm>
 #include <vector>
@@ -83,8 +83,8 @@ int build_locality(int N, int D, const int2* edges, const int* eoff, const int* 
 
     // 1. node patches: breadth-first balls of nodes owning at most FCT_EP edges; then
     //    two coarser levels, each grouping up to 16 adjacent groups of the level below,
-    //    so that the order keeps neighbours close at every scale (a ball's neighbours in
-    //    its group, a group's in its super-group)
+    //    so that the patch order keeps neighbours close at every scale (a ball's
+    //    neighbours in its group, a group's in its super-group)
     std::vector<int> noff(N + 1), nadj(eoff[N]), nweight(N);
     for (int n = 0; n <= N; ++n) noff[n] = eoff[n];
     for (int n = 0; n < N; ++n) {
@@ -130,11 +130,6 @@ int build_locality(int N, int D, const int2* edges, const int* eoff, const int* 
     }
     std::vector<int> balls;   // balls in the final order
     for (const auto& g : members) balls.insert(balls.end(), g.begin(), g.end());
-    out.node_order.clear();
-    out.node_order.reserve(N);
-    for (int b : balls)
-        out.node_order.insert(out.node_order.end(), order.begin() + ball_start[b],
-                              order.begin() + ball_start[b + 1]);
 
     // 2. edge patches: the owned edges of each node patch, split where the node list
     //    (edge ends and up/down triangle vertices) would pass FCT_PU

@@ -100,30 +100,14 @@ k_vert_impl(DevMesh m, const double* __restrict__ w_i, const double* __restrict_
     double cp_prev = 0.0, dp_prev = 0.0;
     for (int k = 0; k < L; ++k) {
         const size_t i = b0 + k;
-        const double av = m.areasvol[i];
-        const double safe_av = av > 0.0 ? av : 1.0;
-        const double v_top = dt * m.area[i] / safe_av;
-        const double v_bot = dt * (k + 1 < L ? m.area[i + 1] : 0.0) / safe_av;
-        const double wu = w_i[i];
-        const double wd = k + 1 < L ? w_i[i + 1] : 0.0;
-        const bool surf = k == nr.x;
-        const bool bot = k == nr.y - 1;
-        const bool valid = in_layer(nr, k);
-        const double a_adv = (surf ? 0.0 : fmin(0.0, wu)) * v_top;
-        const double b_top = (surf ? wu : fmax(0.0, wu)) * v_top;
-        const double b_bot = (bot ? 0.0 : -fmin(0.0, wd)) * v_bot;
-        const double c_adv = (bot ? 0.0 : -fmax(0.0, wd)) * v_bot;
-        const double b_adv = b_top + b_bot;
-        const double a = valid ? a_adv : 0.0;
-        const double c = valid ? c_adv : 0.0;
-        const double b = valid ? hnn[i] + b_adv : 1.0;
+        const VimplRow r = vimpl_row(m, w_i, hnn, dt, b0, k, nr);
         const double Td = X[k > 0 ? i - 1 : i];
         const double Tc = X[i];
         const double Tu = k + 1 < L ? X[i + 1] : 0.0;
-        const double rhs = valid ? -(a * Td + b_adv * Tc + c * Tu) : 0.0;
-        const double mm = b - a * cp_prev;
-        const double cpi = c / mm;
-        const double dpi = (rhs - a * dp_prev) / mm;
+        const double rhs = r.valid ? -(r.a * Td + r.b_adv * Tc + r.c * Tu) : 0.0;
+        const double mm = r.b - r.a * cp_prev;
+        const double cpi = r.c / mm;
+        const double dpi = (rhs - r.a * dp_prev) / mm;
         cp[i] = cpi; dp[i] = dpi;
         cp_prev = cpi; dp_prev = dpi;
     }
@@ -682,6 +666,12 @@ int launch_lo(const DevMesh& m, const fct_step_args* a, int j, const double* vfl
         rc = check_launch("k_vert_impl");
     }
     return rc;
+}
+
+int launch_vert_impl(const DevMesh& m, const double* w_i, const double* hnn, double dt,
+                     double* X, double* cp, double* dp, hipStream_t s) {
+    FCT_LAUNCH(k_vert_impl, (int64_t)m.N, s, m, w_i, hnn, dt, X, cp, dp);
+    return check_launch("k_vert_impl");
 }
 
 int launch_grad(const DevMesh& m, const fct_step_args* a, int j, double* tr, hipStream_t s) {

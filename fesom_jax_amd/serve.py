@@ -212,12 +212,14 @@ class Server:
     def _refresh(self, plan):
         self.plan_bytes[id(plan)] = plan.device_bytes()
 
-    def _release_idle(self, need, active, busy=()):
-        """Close idle plans (largest first), then drop the base inputs of meshes no
-        active or finishing request uses (they are rebuilt when needed), until ``need``
-        more bytes fit; returns whether they do."""
-        idle = sorted(((self.plan_bytes[id(p)], k, p) for k, ps in self.pool.items() for p in ps),
-                      key=lambda x: -x[0])
+    def _release_idle(self, need, active, busy=(), keep=None, keep_base=None):
+        """Close idle plans (largest first, never ``keep``: the plan the admission that
+        asks will reuse, whose bytes ``need`` leaves out), then drop the base inputs of
+        meshes no active or finishing request uses, nor the asking admission
+        (``keep_base``; they are rebuilt when needed), until ``need`` more bytes fit;
+        returns whether they do."""
+        idle = sorted(((self.plan_bytes[id(p)], k, p) for k, ps in self.pool.items() for p in ps
+                       if p is not keep), key=lambda x: -x[0])
         for nbytes, key, plan in idle:
             if self.held_bytes(active) + need <= self.budget:
                 return True
@@ -226,6 +228,8 @@ class Server:
             self.adjoint_plans.discard(id(plan))
             plan.close()
         in_use = {(a.spec.level, a.spec.nl) for a in list(active) + list(busy)}
+        if keep_base is not None:
+            in_use.add(keep_base)
         for key in sorted(self.base_bytes, key=lambda k: -self.base_bytes[k]):
             if self.held_bytes(active) + need <= self.budget:
                 break
@@ -359,7 +363,10 @@ class Server:
             while (pending and pending[0].arrival <= now and len(active) < self.max_active):
                 s = pending[0]
                 need = self._need(s)
-                if (not self._release_idle(need, active, [f[0] for f in finishing])
+                ps = self.pool[(s.level, s.nl, s.K)]
+                if (not self._release_idle(need, active, [f[0] for f in finishing],
+                                           keep=ps[-1] if ps else None,
+                                           keep_base=(s.level, s.nl))
                         and (active or finishing)):
                     break
                 pending.popleft()

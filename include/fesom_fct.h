@@ -170,7 +170,9 @@ int fct_plan_set_tuning(fct_plan* plan, int32_t knob, int32_t value);
  * T_new[k] = advect_one_fct(mesh, uv, w_e, helem, hnode, hnode_new, T[k], T_old[k],
  *                           dt=dt, exch=None, Z3d, zbar3, w_i)[0]
  * (fesom_jax/tracer_adv.py:532-578). T_old_new is T itself and is not written.
- * No allocation; stream-ordered. */
+ * Stream-ordered. On the fused path it allocates nothing; the first call that takes
+ * the phased path (w_i given, nl > 128 or node degree > 8) allocates 10 * nod2D * nl +
+ * 2 * elem2D * nl + edge2D * nl doubles of workspace, later calls nothing. */
 int fct_forward(fct_plan* plan, const fct_step_args* args, void* stream);
 
 /* fct_forward with an explicit kernel path (FCT_PATH_*); both paths give the same
@@ -337,14 +339,19 @@ int fct_tracer_ab(const double* values, const double* valuesold, double* ttf_ab,
  * jax.grad of the forward, fesom_jax/docs/LIMITER_GRADIENTS.md:45-75: max/min send the
  * cotangent to the winner, half each on a tie; max/min reductions share it among tied
  * arguments; abs'(0) = +1). The forward intermediates are recomputed, not stored.
- * Supports w_i == NULL and the static QR4C stencil (Z3d == NULL). The first call
- * allocates the adjoint workspace; later calls allocate nothing. */
+ * With the implicit split (w_i given) the solve of fesom_jax/tracer_adv.py:555-556 is
+ * reversed through the Thomas recurrence (ops.py:121-168) and g_w_i is returned; the
+ * moving-level stencil (Z3d, zbar3) is supported. Z3d, zbar3, helem, hnode and
+ * hnode_new are constants (no cotangent). The first call allocates the adjoint
+ * workspace (and the first with w_i two more [nod2D][nl] arrays); later calls
+ * allocate nothing. */
 typedef struct fct_adjoint_args {
     const double* const* g_T_new;  /* ntracers x [nod2D][nl], device */
     double* const* g_T;            /* out: ntracers x [nod2D][nl] */
     double* const* g_T_old;        /* out: ntracers x [nod2D][nl] */
     double* g_uv;                  /* out: [elem2D][nl][2] or NULL (not computed) */
     double* g_w_e;                 /* out: [nod2D][nl] or NULL (not computed) */
+    double* g_w_i;                 /* out: [nod2D][nl] when fwd->w_i is given, or NULL */
 } fct_adjoint_args;
 
 /* fwd: the step's forward arguments (T_new is not read and may be NULL). */

@@ -106,10 +106,32 @@ int fct_loops_step(const loop_mesh* m, const double* uv, const double* w, const 
         !cmax || !cmin || !qmax || !qmin || !sp || !sm || !rp || !rm || !inc)
         return -1;
 
+    /* node -> (edge, side) lists in increasing 2*edge + side: the edge sums below run per
+     * node over this list, which adds each node's contributions in the order of a loop
+     * over the edges that adds to the first node and subtracts from the second, and
+     * leaves the nodes independent of each other (so they can run in parallel) */
+    int* noff = calloc((size_t)N + 1, sizeof(int));
+    int* nent = malloc(2 * (size_t)D * sizeof(int));
+    if (!noff || !nent) return -1;
+    for (int e = 0; e < D; ++e) { noff[m->edges[2 * e] + 1]++; noff[m->edges[2 * e + 1] + 1]++; }
+    for (int n = 0; n < N; ++n) noff[n + 1] += noff[n];
+    {
+        int* pos = malloc(((size_t)N + 1) * sizeof(int));
+        if (!pos) return -1;
+        memcpy(pos, noff, ((size_t)N + 1) * sizeof(int));
+        for (int e = 0; e < D; ++e) {
+            nent[pos[m->edges[2 * e]]++] = 2 * e;
+            nent[pos[m->edges[2 * e + 1]]++] = 2 * e + 1;
+        }
+        free(pos);
+    }
+
     /* AB2 field for the high-order fluxes (tracer_adv.py:43-46, 65) */
+#pragma omp parallel for schedule(static)
     for (size_t i = 0; i < NL; ++i) ab[i] = AB_OLD * Told[i] + AB_NEW * T[i];
 
     /* first-order upwind fluxes of T: horizontal per edge, vertical per node interface */
+#pragma omp parallel for schedule(static)
     for (int e = 0; e < D; ++e) {
         const int a = m->edges[2 * e], b = m->edges[2 * e + 1];
         for (int k = 0; k < nl; ++k) {
@@ -120,6 +142,7 @@ int fct_loops_step(const loop_mesh* m, const double* uv, const double* w, const 
                 -0.5 * (T[(size_t)a * nl + k] * (vf + av) + T[(size_t)b * nl + k] * (vf - av));
         }
     }
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < N; ++n) {
         const int top = m->ulevels_nod2D[n] - 1, bot = m->nlevels_nod2D[n] - 1;
         if (bot <= top) continue;
@@ -133,18 +156,22 @@ int fct_loops_step(const loop_mesh* m, const double* uv, const double* w, const 
 
     /* low-order solution: edge divergence (wet levels of either triangle), then the
      * vertical divergence and the thickness update on each wet layer */
-    for (int e = 0; e < D; ++e) {
-        const int a = m->edges[2 * e], b = m->edges[2 * e + 1];
-        const int t1 = m->edge_tri[2 * e], t2 = m->edge_tri[2 * e + 1];
-        int kmax = 0;
-        if (t1 >= 0 && m->nlevels[t1] - 1 > kmax) kmax = m->nlevels[t1] - 1;
-        if (t2 >= 0 && m->nlevels[t2] - 1 > kmax) kmax = m->nlevels[t2] - 1;
-        for (int k = 0; k < kmax; ++k) {
-            const double f = fh[(size_t)e * nl + k];
-            lo[(size_t)a * nl + k] += f;
-            lo[(size_t)b * nl + k] -= f;
+#pragma omp parallel for schedule(static)
+    for (int n = 0; n < N; ++n) {
+        for (int q = noff[n]; q < noff[n + 1]; ++q) {
+            const int e = nent[q] >> 1, side = nent[q] & 1;
+            const int t1 = m->edge_tri[2 * e], t2 = m->edge_tri[2 * e + 1];
+            int kmax = 0;
+            if (t1 >= 0 && m->nlevels[t1] - 1 > kmax) kmax = m->nlevels[t1] - 1;
+            if (t2 >= 0 && m->nlevels[t2] - 1 > kmax) kmax = m->nlevels[t2] - 1;
+            for (int k = 0; k < kmax; ++k) {
+                const double f = fh[(size_t)e * nl + k];
+                const size_t i = (size_t)n * nl + k;
+                lo[i] = side ? lo[i] - f : lo[i] + f;
+            }
         }
     }
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < N; ++n) {
         const size_t r = (size_t)n * nl;
         for (int k = m->ulevels_nod2D[n] - 1; k < m->nlevels_nod2D[n] - 1; ++k) {
@@ -159,6 +186,7 @@ int fct_loops_step(const loop_mesh* m, const double* uv, const double* w, const 
     }
 
     /* element gradients of T on wet element layers */
+#pragma omp parallel for schedule(static)
     for (int el = 0; el < E; ++el) {
         const int* v = m->elem_nodes + 3 * (size_t)el;
         const double* g = m->gradient_sca + 6 * (size_t)el;
@@ -171,7 +199,12 @@ int fct_loops_step(const loop_mesh* m, const double* uv, const double* w, const 
     }
 
     /* MFCT high-order horizontal flux of the AB2 field with the up/down gradients */
+#pragma omp parallel
+    {
+    double* egt = malloc(4 * (size_t)nl * sizeof(double));   /* this thread's slots */
+#pragma omp for schedule(static)
     for (int e = 0; e < D; ++e) {
+        double* eg = egt;
         const int a = m->edges[2 * e], b = m->edges[2 * e + 1];
         const int up = m->edge_up_dn_tri[2 * e], dn = m->edge_up_dn_tri[2 * e + 1];
         memset(eg, 0, 4 * (size_t)nl * sizeof(double));
@@ -209,9 +242,13 @@ int fct_loops_step(const loop_mesh* m, const double* uv, const double* w, const 
             hi_h[(size_t)e * nl + k] = -0.5 * ((vf + fabs(vf)) * tma + (vf - fabs(vf)) * tmb);
         }
     }
+    free(egt);
+    }
+#pragma omp parallel for schedule(static)
     for (size_t i = 0; i < DL; ++i) fh[i] = hi_h[i] - fh[i];   /* antidiffusive adf_h */
 
     /* QR4C high-order vertical flux of the AB2 field minus the upwind flux: adf_v */
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < N; ++n) {
         const int top = m->ulevels_nod2D[n] - 1, bot = m->nlevels_nod2D[n] - 1;
         if (bot <= top) continue;
@@ -240,11 +277,13 @@ int fct_loops_step(const loop_mesh* m, const double* uv, const double* w, const 
 
     /* Zalesak limiter: bounds from LO and T over the element cluster, sums of
      * incoming/outgoing antidiffusive fluxes, ratios, limited fluxes */
+#pragma omp parallel for schedule(static)
     for (size_t i = 0; i < NL; ++i) {
         bmax[i] = dmax(lo[i], T[i]);
         bmin[i] = dmin(lo[i], T[i]);
     }
     for (size_t i = 0; i < EL; ++i) { emax[i] = -BIG; emin[i] = BIG; }
+#pragma omp parallel for schedule(static)
     for (int el = 0; el < E; ++el) {
         const int* v = m->elem_nodes + 3 * (size_t)el;
         for (int k = m->ulevels[el] - 1; k < m->nlevels[el] - 1; ++k) {
@@ -255,6 +294,7 @@ int fct_loops_step(const loop_mesh* m, const double* uv, const double* w, const 
         }
     }
     for (size_t i = 0; i < NL; ++i) { cmax[i] = -BIG; cmin[i] = BIG; }
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < N; ++n) {
         const size_t r = (size_t)n * nl;
         for (int k = m->ulevels_nod2D[n] - 1; k < m->nlevels_nod2D[n] - 1; ++k)
@@ -264,6 +304,7 @@ int fct_loops_step(const loop_mesh* m, const double* uv, const double* w, const 
                 cmin[r + k] = dmin(cmin[r + k], emin[ie]);
             }
     }
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < N; ++n) {
         const int top = m->ulevels_nod2D[n] - 1, bot = m->nlevels_nod2D[n] - 1;
         const size_t r = (size_t)n * nl;
@@ -277,6 +318,7 @@ int fct_loops_step(const loop_mesh* m, const double* uv, const double* w, const 
             }
         }
     }
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < N; ++n) {
         const size_t r = (size_t)n * nl;
         for (int k = m->ulevels_nod2D[n] - 1; k < m->nlevels_nod2D[n] - 1; ++k) {
@@ -285,17 +327,21 @@ int fct_loops_step(const loop_mesh* m, const double* uv, const double* w, const 
             sm[r + k] += dmin(ft, 0.0) + dmin(-fb, 0.0);
         }
     }
-    for (int e = 0; e < D; ++e) {
-        const size_t ra = (size_t)m->edges[2 * e] * nl, rb = (size_t)m->edges[2 * e + 1] * nl;
-        for (int k = 0; k < nl; ++k) {
-            const double f = fh[(size_t)e * nl + k];
-            sp[ra + k] += dmax(f, 0.0);
-            sm[ra + k] += dmin(f, 0.0);
-            sp[rb + k] += dmax(-f, 0.0);
-            sm[rb + k] += dmin(-f, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int n = 0; n < N; ++n) {
+        const size_t r = (size_t)n * nl;
+        for (int q = noff[n]; q < noff[n + 1]; ++q) {
+            const int e = nent[q] >> 1, side = nent[q] & 1;
+            for (int k = 0; k < nl; ++k) {
+                const double f = fh[(size_t)e * nl + k];
+                const double g = side ? -f : f;
+                sp[r + k] += dmax(g, 0.0);
+                sm[r + k] += dmin(g, 0.0);
+            }
         }
     }
     for (size_t i = 0; i < NL; ++i) { rp[i] = 1.0; rm[i] = 1.0; }
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < N; ++n) {
         const size_t r = (size_t)n * nl;
         for (int k = m->ulevels_nod2D[n] - 1; k < m->nlevels_nod2D[n] - 1; ++k) {
@@ -305,6 +351,7 @@ int fct_loops_step(const loop_mesh* m, const double* uv, const double* w, const 
             rm[r + k] = dmin(1.0, qmin[r + k] / (sm[r + k] * dt / av / hn - FLUX_EPS));
         }
     }
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < N; ++n) {
         const int top = m->ulevels_nod2D[n] - 1, bot = m->nlevels_nod2D[n] - 1;
         if (bot <= top) continue;
@@ -317,6 +364,7 @@ int fct_loops_step(const loop_mesh* m, const double* uv, const double* w, const 
             fv[r + k] = lim * f;
         }
     }
+#pragma omp parallel for schedule(static)
     for (int e = 0; e < D; ++e) {
         const size_t ra = (size_t)m->edges[2 * e] * nl, rb = (size_t)m->edges[2 * e + 1] * nl;
         for (int k = 0; k < nl; ++k) {
@@ -327,6 +375,7 @@ int fct_loops_step(const loop_mesh* m, const double* uv, const double* w, const 
     }
 
     /* LO plus the limited antidiffusive divergence, then the ALE reconstruction */
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < N; ++n) {
         const size_t r = (size_t)n * nl;
         for (int k = m->ulevels_nod2D[n] - 1; k < m->nlevels_nod2D[n] - 1; ++k) {
@@ -335,15 +384,20 @@ int fct_loops_step(const loop_mesh* m, const double* uv, const double* w, const 
                 inc[r + k] += (fv[r + k] - fv[r + k + 1]) * dt / m->areasvol[r + k];
         }
     }
-    for (int e = 0; e < D; ++e) {
-        const size_t ra = (size_t)m->edges[2 * e] * nl, rb = (size_t)m->edges[2 * e + 1] * nl;
-        for (int k = 0; k < nl; ++k) {
-            const double f = fh[(size_t)e * nl + k];
-            if (m->areasvol[ra + k] > 0.0) inc[ra + k] += f * dt / m->areasvol[ra + k];
-            if (m->areasvol[rb + k] > 0.0) inc[rb + k] -= f * dt / m->areasvol[rb + k];
+#pragma omp parallel for schedule(static)
+    for (int n = 0; n < N; ++n) {
+        const size_t r = (size_t)n * nl;
+        for (int q = noff[n]; q < noff[n + 1]; ++q) {
+            const int e = nent[q] >> 1, side = nent[q] & 1;
+            for (int k = 0; k < nl; ++k) {
+                if (m->areasvol[r + k] <= 0.0) continue;
+                const double d = fh[(size_t)e * nl + k] * dt / m->areasvol[r + k];
+                inc[r + k] = side ? inc[r + k] - d : inc[r + k] + d;
+            }
         }
     }
     memcpy(Tnew, T, NL * sizeof(double));
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < N; ++n) {
         const size_t r = (size_t)n * nl;
         for (int k = m->ulevels_nod2D[n] - 1; k < m->nlevels_nod2D[n] - 1; ++k) {
@@ -355,6 +409,7 @@ int fct_loops_step(const loop_mesh* m, const double* uv, const double* w, const 
     free(ab); free(fh); free(fv); free(lo); free(grad); free(eg); free(hi_h);
     free(bmax); free(bmin); free(emax); free(emin); free(cmax); free(cmin);
     free(qmax); free(qmin); free(sp); free(sm); free(rp); free(rm); free(inc);
+    free(noff); free(nent);
     return 0;
 }
 

@@ -1,6 +1,8 @@
 """CPU oracle for the adjoint (test infrastructure only): a hand-written reverse-mode
-restatement of one FCT step, fesom_jax/tracer_adv.py:532-578 (w_i = None), giving the
-cotangents of ``T, T_old, uv, w_e`` for a cotangent of ``T_new``.
+restatement of one FCT step, fesom_jax/tracer_adv.py:532-578 (with or without the
+implicit split ``w_i`` and the moving-level stencil ``Z3d/zbar3``), giving the
+cotangents of ``T, T_old, uv, w_e`` (and ``w_i``) for a cotangent of ``T_new``.
+``Z3d``, ``zbar3``, ``helem``, ``hnode`` and ``hnode_new`` are treated as constants.
 
 It follows the differentiation conventions of ``jax.grad`` applied to the reference
 (fesom_jax/docs/LIMITER_GRADIENTS.md:45-75: plain subgradients, no custom VJP):
@@ -73,7 +75,109 @@ def _scatter_T(g_node, seg):
     return np.where((seg >= 0).reshape(seg.shape + (1,) * (out.ndim - seg.ndim)), out, 0.0)
 
 
-def forward_tape(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, dt):
+def _vimpl_rows(mesh, w_i, hnode_new, dt):
+    """Row coefficients of adv_tra_vert_impl (fesom_jax/tracer_adv.py:487-529) and the
+    pieces their cotangents need."""
+    m = mesh
+    L = m.nl
+    safe_av = np.where(m.areasvol > 0.0, m.areasvol, 1.0)
+    v_top = dt * m.area / safe_av
+    v_bot = dt * F._shift_up(m.area) / safe_av
+    wu, wd = w_i, F._shift_up(w_i)
+    k = np.arange(L)[None, :]
+    nzmin = (m.ulevels_nod2D.astype(np.int64) - 1)[:, None]
+    nzmax = (m.nlevels_nod2D.astype(np.int64) - 1)[:, None]
+    valid = m.node_layer_mask
+    surf = k == nzmin
+    bot = k == (nzmax - 1)
+    b_adv = (np.where(surf, wu, np.maximum(0.0, wu)) * v_top
+             + np.where(bot, 0.0, -np.minimum(0.0, wd)) * v_bot)
+    a = np.where(valid, np.where(surf, 0.0, np.minimum(0.0, wu)) * v_top, 0.0)
+    c = np.where(valid, np.where(bot, 0.0, -np.maximum(0.0, wd)) * v_bot, 0.0)
+    b = np.where(valid, hnode_new + b_adv, 1.0)
+    return dict(a=a, b=b, c=c, b_adv=b_adv, v_top=v_top, v_bot=v_bot, wu=wu, wd=wd,
+                surf=surf, bot=bot, valid=valid)
+
+
+def _vimpl_vjp(mesh, w_i, hnode_new, dt, X, g_out):
+    """Cotangents ``(g_X, g_w_i)`` of ``adv_tra_vert_impl(mesh, w_i, X, hnode_new)``
+    (fesom_jax/tracer_adv.py:487-529) for ``g_out``: reverse mode through the Thomas
+    recurrence of ops.tdma (fesom_jax/ops.py:121-168) step by step, as jax.grad
+    differentiates its two scans."""
+    r = _vimpl_rows(mesh, w_i, hnode_new, dt)
+    a, b, c, b_adv, valid = r["a"], r["b"], r["c"], r["b_adv"], r["valid"]
+    rhs = np.where(valid, -(a * F._shift_down(X) + b_adv * X + c * F._shift_up(X)), 0.0)
+    N, L = X.shape
+    cp = np.zeros((N, L))
+    dp = np.zeros((N, L))
+    mm = np.zeros((N, L))
+    cprev = np.zeros(N)
+    dprev = np.zeros(N)
+    for i in range(L):
+        mm[:, i] = b[:, i] - a[:, i] * cprev
+        cp[:, i] = c[:, i] / mm[:, i]
+        dp[:, i] = (rhs[:, i] - a[:, i] * dprev) / mm[:, i]
+        cprev, dprev = cp[:, i], dp[:, i]
+    x = np.zeros((N, L))
+    xn = np.zeros(N)
+    for i in range(L - 1, -1, -1):
+        x[:, i] = dp[:, i] - cp[:, i] * xn
+        xn = x[:, i]
+    # out = X + mask(x): the back substitution reversed (x[i] feeds x[i-1])
+    g_x = np.where(valid, g_out, 0.0)
+    g_cp = np.zeros((N, L))
+    g_dp = np.zeros((N, L))
+    carry = np.zeros(N)
+    for i in range(L):
+        G = g_x[:, i] + carry
+        g_dp[:, i] = G
+        g_cp[:, i] = -G * (x[:, i + 1] if i + 1 < L else 0.0)
+        carry = -G * cp[:, i]
+    # the forward sweep reversed
+    g_a = np.zeros((N, L))
+    g_b = np.zeros((N, L))
+    g_c = np.zeros((N, L))
+    g_rhs = np.zeros((N, L))
+    gcc = np.zeros(N)
+    gdc = np.zeros(N)
+    for i in range(L - 1, -1, -1):
+        GC = g_cp[:, i] + gcc
+        GD = g_dp[:, i] + gdc
+        cprev = cp[:, i - 1] if i > 0 else np.zeros(N)
+        dprev = dp[:, i - 1] if i > 0 else np.zeros(N)
+        num = rhs[:, i] - a[:, i] * dprev
+        g_num = GD / mm[:, i]
+        g_m = (-GD * num) * (1.0 / (mm[:, i] * mm[:, i]))
+        g_rhs[:, i] = g_num
+        g_a[:, i] += -g_num * dprev
+        gdc = -g_num * a[:, i]
+        g_c[:, i] += GC / mm[:, i]
+        g_m = g_m + (-GC * c[:, i]) * (1.0 / (mm[:, i] * mm[:, i]))
+        g_b[:, i] += g_m
+        g_a[:, i] += -g_m * cprev
+        gcc = -g_m * a[:, i]
+    # rhs = where(valid, -(a X[k-1] + b_adv X + c X[k+1]), 0)
+    gs = -np.where(valid, g_rhs, 0.0)
+    g_X = g_out.copy()
+    g_a += gs * F._shift_down(X)
+    g_badv = gs * X
+    g_c += gs * F._shift_up(X)
+    g_X += _sd_T(gs * a) + gs * b_adv + _su_T(gs * c)
+    g_badv += np.where(valid, g_b, 0.0)
+    g_aadv = np.where(valid, g_a, 0.0) * r["v_top"]
+    g_cadv = np.where(valid, g_c, 0.0) * r["v_bot"]
+    surf, bot, wu, wd = r["surf"], r["bot"], r["wu"], r["wd"]
+    g_wu = np.where(surf, 0.0, _min_vjp(g_aadv, wu, np.zeros_like(wu))[0])
+    gt = g_badv * r["v_top"]
+    g_wu += np.where(surf, gt, _max_vjp(gt, wu, np.zeros_like(wu))[0])
+    gb = g_badv * r["v_bot"]
+    g_wd = np.where(bot, 0.0, -_min_vjp(gb, wd, np.zeros_like(wd))[0])
+    g_wd += np.where(bot, 0.0, -_max_vjp(g_cadv, wd, np.zeros_like(wd))[0])
+    return g_X, g_wu + _su_T(g_wd)
+
+
+def forward_tape(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, dt, Z3d=None, zbar3=None,
+                 w_i=None):
     """The forward step with every intermediate the backward needs."""
     m = mesh
     t = dict(T=T, T_old=T_old)
@@ -88,6 +192,9 @@ def forward_tape(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, dt):
     fv = F.adv_flux_ver(m, w_e, T)
     t.update(fh=fh, fv=fv)
     LO = F.compute_fct_lo(m, fh, fv, T, hnode, hnode_new, dt=dt)
+    if w_i is not None:
+        t["LOx"] = LO
+        LO = F.adv_tra_vert_impl(m, w_i, LO, hnode_new, dt=dt)
     t["LO"] = LO
     tr = F.tracer_gradient_elements(m, T)
     gnode = F.node_avg_grad(m, tr)
@@ -95,8 +202,12 @@ def forward_tape(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, dt):
     t.update(tr=tr, gnode=gnode, eud=eud)
     ho = F.adv_flux_hor_ho(m, uv, helem, A, eud)
     adf_h = ho - fh
-    hov = F.adv_flux_ver_ho(m, w_e, A)
-    adf_v = hov - fv
+    if w_i is None:
+        adf_v = F.adv_flux_ver_ho(m, w_e, A, Z3d=Z3d, zbar3=zbar3) - fv
+    else:
+        w_full = w_e + w_i
+        adf_v = (F.adv_flux_ver_ho(m, w_full, A, Z3d=Z3d, zbar3=zbar3)
+                 - F.adv_flux_ver(m, w_full, T))
     t.update(adf_h=adf_h, adf_v=adf_v)
     adf_h_lim, adf_v_lim, fac = F.zalesak_limit(m, T, LO, adf_h, adf_v, hnode_new, dt=dt,
                                                 return_factors=True)
@@ -109,11 +220,13 @@ def forward_tape(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, dt):
 
 
 def advect_one_fct_vjp(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, g_Tnew, *,
-                       dt=F.DT_DEFAULT):
-    """Cotangents ``(g_T, g_T_old, g_uv, g_w_e)`` of one FCT step for ``g_Tnew``."""
+                       dt=F.DT_DEFAULT, Z3d=None, zbar3=None, w_i=None):
+    """Cotangents ``(g_T, g_T_old, g_uv, g_w_e)`` of one FCT step for ``g_Tnew``, and
+    ``g_w_i`` as a fifth entry when ``w_i`` is given."""
     m = mesh
     L = m.nl
-    tp = forward_tape(m, uv, w_e, helem, hnode, hnode_new, T, T_old, dt)
+    tp = forward_tape(m, uv, w_e, helem, hnode, hnode_new, T, T_old, dt, Z3d=Z3d,
+                      zbar3=zbar3, w_i=w_i)
     nlm = m.node_layer_mask
     nim = m.node_iface_mask
     a = m.areasvol
@@ -270,8 +383,10 @@ def advect_one_fct_vjp(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, g_Tnew,
     g_T += gt
     # ---- adf_v = adv_flux_ver_ho(A) - adv_flux_ver(T) (tracer_adv.py:241-295, 108-120)
     g_hov = g_adfv
-    g_fv = -g_adfv
-    W, area, A = w_e, m.area, tp["A"]
+    g_fv = -g_adfv      # the LO-form flux subtracted in adf_v (with w_e + w_i if split)
+    W = w_e if w_i is None else w_e + w_i
+    area, A = m.area, tp["A"]
+    g_wf = np.zeros((N, L))     # cotangent of the w in adf_v
     g_hov = np.where(nim, g_hov, 0.0)
     is_surf = k == nzmin
     is_bot = k == nzmax
@@ -282,19 +397,26 @@ def advect_one_fct_vjp(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, g_Tnew,
     gq_ = np.where(~is_surf & ~is_bot & ~is_cent & is_int, g_hov, 0.0)
     # surf = ((-A) * W) * area
     g_A += -(gs_ * area * W)
-    g_w += gs_ * area * (-A)
+    g_wf += gs_ * area * (-A)
     # cent = ((-0.5 * (Am1 + A)) * W) * area
     Am1 = F._shift_down(A)
     Am2 = F._shift_down(Am1)
     Ap1 = F._shift_up(A)
     gcm = gc_ * area
     g_sum = -0.5 * (gcm * W)
-    g_w += gcm * (-0.5 * (Am1 + A))
+    g_wf += gcm * (-0.5 * (Am1 + A))
     g_A += g_sum
     g_Am1 = g_sum.copy()
     # quad
-    Z_nz, Z_m1, Z_m2, Z_p1 = F._z_stencil(np.asarray(m.Z), L)
-    zb = np.asarray(m.zbar)
+    if Z3d is None:
+        Z_nz, Z_m1, Z_m2, Z_p1 = F._z_stencil(np.asarray(m.Z), L)
+        zb = np.asarray(m.zbar)
+    else:
+        Z_nz = Z3d
+        Z_m1 = F._shift_down(Z3d)
+        Z_m2 = F._shift_down(Z_m1)
+        Z_p1 = np.concatenate([Z3d[:, 1:], Z3d[:, -1:]], axis=1)
+        zb = zbar3
     sf = lambda x: np.where(x == 0.0, 1.0, x)
     d_c, d_u, d_d = sf(Z_m1 - Z_nz), sf(Z_nz - Z_p1), sf(Z_m2 - Z_m1)
     with np.errstate(divide="ignore", invalid="ignore"):
@@ -304,7 +426,7 @@ def advect_one_fct_vjp(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, g_Tnew,
         Tmean1 = A + (2.0 * qc + qu) * (zb - Z_nz) / 3.0
         Tmean2 = Am1 + (2.0 * qc + qd) * (zb - Z_m1) / 3.0
     gqm = gq_ * area
-    g_w += np.where(is_int, gqm * (-0.5 * (Tmean1 + Tmean2)), 0.0)
+    g_wf += np.where(is_int, gqm * (-0.5 * (Tmean1 + Tmean2)), 0.0)
     g_tm = -0.5 * (gqm * W)
     g_tm = np.where(is_int, g_tm, 0.0)
     g_A += g_tm
@@ -383,6 +505,10 @@ def advect_one_fct_vjp(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, g_Tnew,
     gs = m.gradient_sca
     g_tv3 = (g_tr[:, None, :, 0] * gs[:, 0:3, None] + g_tr[:, None, :, 1] * gs[:, 3:6, None])
     np.add.at(g_T, seg, g_tv3.reshape(3 * E, L))
+    # ---- adv_tra_vert_impl applied to LO (tracer_adv.py:555-556, 487-529)
+    g_wi = None
+    if w_i is not None:
+        g_LO, g_wi = _vimpl_vjp(m, w_i, hnode_new, dt, tp["LOx"], g_LO)
     # ---- LO (tracer_adv.py:298-313) and the low-order fluxes
     valid_lo = nlm & (hnode_new > 0.0) & (a > 0.0)
     g_num_lo = np.where(valid_lo, g_LO, 0.0) / safe_hn
@@ -390,7 +516,9 @@ def advect_one_fct_vjp(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, g_Tnew,
     g_div = g_num_lo / safe_a * dt
     g_fh_e = _scatter_T(g_div, m.edges)
     g_fh += g_fh_e[:, 0] - g_fh_e[:, 1]
-    g_fv += g_div + (-_su_T(g_div))
+    g_fvlo = g_div + (-_su_T(g_div))
+    if w_i is None:
+        g_fv = g_fv + g_fvlo
     # fh = -0.5 * (T1 (vf + av) + T2 (vf - av))
     gs1 = -0.5 * g_fh
     g_T1 = gs1 * (vf + av)
@@ -400,16 +528,28 @@ def advect_one_fct_vjp(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, g_Tnew,
     g_vf += g_vp + g_vm + _abs_vjp(g_vp - g_vm, vf)
     np.add.at(g_T, n1, g_T1)
     np.add.at(g_T, n2, g_T2)
-    # fv = mask(((-0.5 * (T (w + aw) + Tab (w - aw))) * area))
-    g_fv = np.where(nlm, g_fv, 0.0) * area
-    gs3 = -0.5 * g_fv
-    aw_ = np.abs(W)
-    Tab = F._shift_down(T)
-    g_T += gs3 * (W + aw_)
-    g_T += _sd_T(gs3 * (W - aw_))
-    g_wp = gs3 * T
-    g_wm = gs3 * Tab
-    g_w += g_wp + g_wm + _abs_vjp(g_wp - g_wm, W)
+    # fv = mask(((-0.5 * (T (w + aw) + Tab (w - aw))) * area)), with w_e in LO and the
+    # adf_v w (w_e + w_i when split) in adf_v
+    def fv_T(g, Wv):
+        g = np.where(nlm, g, 0.0) * area
+        gs3 = -0.5 * g
+        aw_ = np.abs(Wv)
+        gt = gs3 * (Wv + aw_) + _sd_T(gs3 * (Wv - aw_))
+        g_wp = gs3 * T
+        g_wm = gs3 * F._shift_down(T)
+        return gt, g_wp + g_wm + _abs_vjp(g_wp - g_wm, Wv)
+
+    if w_i is None:
+        gt, gw_ = fv_T(g_fv, w_e)
+        g_T += gt
+        g_w += gw_ + g_wf
+    else:
+        gt, gw_ = fv_T(g_fvlo, w_e)
+        gt2, gwf_ = fv_T(g_fv, W)
+        g_T += gt + gt2
+        g_wf += gwf_
+        g_w += gw_ + g_wf
+        g_wi = g_wi + g_wf
     # ---- tracer_ab (tracer_adv.py:59-66)
     g_T += F._AB_NEW * g_A
     g_Told = F._AB_OLD * g_A
@@ -426,6 +566,8 @@ def advect_one_fct_vjp(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, g_Tnew,
     np.add.at(g_uv[:, :, 1], el1s, -(gh1 * cross[:, 0:1]))
     np.add.at(g_uv[:, :, 1], el2s, gh2 * cross[:, 2:3])
     np.add.at(g_uv[:, :, 0], el2s, -(gh2 * cross[:, 3:4]))
+    if w_i is not None:
+        return g_T, g_Told, g_uv, g_w, g_wi
     return g_T, g_Told, g_uv, g_w
 
 

@@ -28,13 +28,13 @@ class _IceCfg(ctypes.Structure):
     _fields_ = [("dt", c_double), ("diff", c_double), ("gamma", c_double)]
 
 
-_LIB = None
+_LIBS = {}
 
 
-def _lib():
-    global _LIB
-    if _LIB is None:
-        _LIB = ctypes.CDLL(str(_build.ensure()))
+def _lib(openmp: bool = False):
+    if openmp not in _LIBS:
+        _LIBS[openmp] = lib = ctypes.CDLL(str(_build.ensure(openmp)))
+        _LIB = lib
         _LIB.fct_loops_step.restype = c_int
         _LIB.fct_loops_step.argtypes = [POINTER(_LoopMesh)] + [POINTER(c_double)] * 7 + [
             c_double, POINTER(c_double)]
@@ -43,7 +43,7 @@ def _lib():
             POINTER(c_double)] * 8
         _LIB.loops_compute_w.restype = c_int
         _LIB.loops_compute_w.argtypes = [POINTER(_LoopMesh)] + [POINTER(c_double)] * 3
-    return _LIB
+    return _LIBS[openmp]
 
 
 def _f64(a):
@@ -64,13 +64,14 @@ def _mesh_struct(mesh):
     return m, keep
 
 
-def advect_one_fct(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, *, dt):
-    """T_new of one FCT step (same arguments as the reference's advect_one_fct)."""
+def advect_one_fct(mesh, uv, w_e, helem, hnode, hnode_new, T, T_old, *, dt, openmp=False):
+    """T_new of one FCT step (same arguments as the reference's advect_one_fct);
+    ``openmp`` runs the OpenMP build (threads from OMP_NUM_THREADS, same bits)."""
     m, keep = _mesh_struct(mesh)
     args = [_f64(x) for x in (uv, w_e, helem, hnode, hnode_new, T, T_old)]
     out = np.empty((int(mesh.nod2D), int(mesh.nl)))
     P = lambda a: a.ctypes.data_as(POINTER(c_double))
-    rc = _lib().fct_loops_step(ctypes.byref(m), *[P(a) for a in args], float(dt), P(out))
+    rc = _lib(openmp).fct_loops_step(ctypes.byref(m), *[P(a) for a in args], float(dt), P(out))
     if rc != 0:
         raise MemoryError("fct_loops_step: allocation failed")
     return out

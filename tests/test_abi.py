@@ -3,6 +3,8 @@ only: no compute call is made without a GPU)."""
 
 import ctypes
 import re
+
+import numpy as np
 from pathlib import Path
 
 import pytest
@@ -64,3 +66,40 @@ def test_enum_constants_match_header():
     for name, value in pairs.items():
         assert hasattr(_lib, name), name
         assert getattr(_lib, name) == int(value), name
+
+
+def test_stage_ids_match_header():
+    text = HEADER.read_text()
+    ids = dict(re.findall(r"\bFCT_STAGE_(\w+)\s*=\s*(\d+)", text))
+    assert ids and {k: int(v) for k, v in ids.items()} == _lib.FCT_STAGE
+
+
+def test_mesh_desc_refuses_mismatched_shapes(pi_mesh):
+    """A field whose shape disagrees with the mesh counts is refused before the library
+    could read past its end."""
+    import dataclasses
+    bad = dataclasses.replace(pi_mesh, edge_up_dn_tri=pi_mesh.edge_up_dn_tri[:-3])
+    with pytest.raises(ValueError, match="edge_up_dn_tri"):
+        _lib.mesh_desc_from(bad)
+    bad = dataclasses.replace(pi_mesh, area=pi_mesh.area[:, :-1])
+    with pytest.raises(ValueError, match="area"):
+        _lib.mesh_desc_from(bad)
+
+
+@pytest.mark.parametrize("field, value, word", [
+    ("ulevels", 0, "element"), ("nlevels", 10_000, "element"),
+    ("ulevels_nod2D_max", 10_000, "node"), ("nlevels_nod2D_min", 0, "node")])
+def test_plan_create_checks_level_ranges_without_gpu(pi_mesh, field, value, word):
+    """Level bounds outside [1, nl] are refused by the plan's validation, which runs
+    before any device call."""
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("library not built")
+    lib = _lib.load(require_torch=False)
+    arr = np.array(getattr(pi_mesh, field), copy=True)
+    arr[5] = value
+    import dataclasses
+    desc, keep = _lib.mesh_desc_from(dataclasses.replace(pi_mesh, **{field: arr}))
+    out = ctypes.c_void_p()
+    rc = lib.fct_plan_create(ctypes.byref(desc), ctypes.byref(out))
+    assert rc == _lib.FCT_E_INVALID
+    assert word.encode() in lib.fct_last_error()

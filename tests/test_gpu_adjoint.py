@@ -20,34 +20,50 @@ def _rel(a, b):
     return float(np.abs(a - b).max()) / den
 
 
-def _oracle(mesh, c, Ts, Tos, gs):
+def _oracle(mesh, c, Ts, Tos, gs, **opts):
     out = [V.advect_one_fct_vjp(mesh, c["uv"], c["w_e"], c["helem"], c["hnode"],
-                                c["hnode_new"], T, To, g, dt=c["dt"])
+                                c["hnode_new"], T, To, g, dt=c["dt"], **opts)
            for T, To, g in zip(Ts, Tos, gs)]
-    return ([o[0] for o in out], [o[1] for o in out], sum(o[2] for o in out),
-            sum(o[3] for o in out))
+    res = ([o[0] for o in out], [o[1] for o in out], sum(o[2] for o in out),
+           sum(o[3] for o in out))
+    return res + ((sum(o[4] for o in out),) if opts.get("w_i") is not None else ())
 
 
-def _gpu(mesh, c, Ts, Tos, gs, dev):
+def _gpu(mesh, c, Ts, Tos, gs, dev, **opts):
     import torch
     from fesom_jax_amd.adjoint import advect_fct_vjp
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-    gT, gTo, guv, gw = advect_fct_vjp(mesh, t(c["uv"]), t(c["w_e"]), t(c["helem"]),
-                                      t(c["hnode"]), t(c["hnode_new"]), [t(x) for x in Ts],
-                                      [t(x) for x in Tos], [t(x) for x in gs], dt=c["dt"])
+    opts = {k: (None if v is None else t(v)) for k, v in opts.items()}
+    out = advect_fct_vjp(mesh, t(c["uv"]), t(c["w_e"]), t(c["helem"]), t(c["hnode"]),
+                         t(c["hnode_new"]), [t(x) for x in Ts], [t(x) for x in Tos],
+                         [t(x) for x in gs], dt=c["dt"], **opts)
     torch.cuda.synchronize()
-    return ([x.cpu().numpy() for x in gT], [x.cpu().numpy() for x in gTo], guv.cpu().numpy(),
-            gw.cpu().numpy())
+    gT, gTo = out[0], out[1]
+    return ([x.cpu().numpy() for x in gT], [x.cpu().numpy() for x in gTo]) + \
+        tuple(x.cpu().numpy() for x in out[2:])
 
 
 def _compare(name, got, want):
-    gT, gTo, guv, gw = got
-    wT, wTo, wuv, ww = want
+    gT, gTo, guv, gw = got[:4]
+    wT, wTo, wuv, ww = want[:4]
+    assert len(got) == len(want)
     for j in range(len(gT)):
         assert _rel(gT[j], wT[j]) <= TOL, (name, "g_T", j, _rel(gT[j], wT[j]))
         assert _rel(gTo[j], wTo[j]) <= TOL, (name, "g_T_old", j, _rel(gTo[j], wTo[j]))
     assert _rel(guv, wuv) <= TOL, (name, "g_uv", _rel(guv, wuv))
     assert _rel(gw, ww) <= TOL, (name, "g_w", _rel(gw, ww))
+    if len(got) > 4:
+        assert _rel(got[4], want[4]) <= TOL, (name, "g_w_i", _rel(got[4], want[4]))
+
+
+def _split(mesh, c, seed):
+    """An implicit split w_i of both signs and a moving-level stencil (Z3d, zbar3)."""
+    rng = np.random.default_rng(seed)
+    wm = mesh.node_iface_mask
+    w_i = np.where(wm, 0.3 * np.abs(c["w_e"]).max() * rng.standard_normal(wm.shape), 0.0)
+    zb3 = mesh.zbar_3d_n * (1.0 + 0.01 * rng.random(mesh.zbar_3d_n.shape))
+    Z3d = np.concatenate([0.5 * (zb3[:, :-1] + zb3[:, 1:]), zb3[:, -1:]], axis=1)
+    return w_i, Z3d, zb3
 
 
 def _cot(mesh, K, seed):
@@ -174,3 +190,34 @@ def test_backward_rejects_unsupported(pi_mesh, pi_case, torch_dev):
     with pytest.raises(ValueError):
         advect_fct_vjp(pi_mesh, t(c["uv"]), t(c["w_e"]), t(c["helem"]), t(c["hnode"]),
                        t(c["hnode_new"]), [t(c["T"][0])], [t(c["T_old"][0])], [], dt=c["dt"])
+
+
+@pytest.mark.parametrize("kind", ["w_i", "Z3d", "both"])
+def test_adjoint_split_and_zstar(kind, torch_dev):
+    """The implicit split (the Thomas solve reversed, w_e + w_i in adf_v, g_w_i) and the
+    moving-level QR4C stencil, smooth and limiter-active."""
+    from fesom_jax_amd.inputs import make_case
+    from fesom_jax_amd.meshgen import icosahedral_mesh
+    m = icosahedral_mesh(3, 48, seed=41)
+    c0 = make_case(m, seed=42, dt=2700.0)
+    for sharp in (False, True):
+        c = sharp_case(m, c0, 2.0) if sharp else c0
+        w_i, Z3d, zb3 = _split(m, c, 7)
+        opts = {"w_i": dict(w_i=w_i), "Z3d": dict(Z3d=Z3d, zbar3=zb3),
+                "both": dict(w_i=w_i, Z3d=Z3d, zbar3=zb3)}[kind]
+        Ts, Tos = c["T"][:2], [0.99 * T + 0.01 * T[::-1] for T in c["T"][:2]]
+        if sharp:
+            Tos = c["T_old"][:2]
+        gs = _cot(m, len(Ts), 3)
+        print(f"[adjoint] {kind} sharp={sharp}", flush=True)
+        _compare(f"{kind} sharp={sharp}", _gpu(m, c, Ts, Tos, gs, torch_dev, **opts),
+                 _oracle(m, c, Ts, Tos, gs, **opts))
+
+
+def test_adjoint_split_pi(pi_mesh, pi_case, torch_dev):
+    c = pi_case
+    w_i, Z3d, zb3 = _split(pi_mesh, c, 17)
+    Ts, Tos, gs = c["T"][:1], [0.995 * c["T"][0]], _cot(pi_mesh, 1, 18)
+    opts = dict(w_i=w_i, Z3d=Z3d, zbar3=zb3)
+    _compare("pi split", _gpu(pi_mesh, c, Ts, Tos, gs, torch_dev, **opts),
+             _oracle(pi_mesh, c, Ts, Tos, gs, **opts))

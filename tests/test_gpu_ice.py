@@ -41,8 +41,13 @@ def _shelf(mesh, frac=0.1, seed=5):
     for q in range(3):
         open_node[mesh.elem_nodes[ul <= 1, q]] = True
     uln = np.where(open_node, 1, 2).astype(np.asarray(mesh.ulevels_nod2D).dtype)
+    ulmax = np.zeros(mesh.nod2D, np.int64)
+    for q in range(3):
+        np.maximum.at(ulmax, mesh.elem_nodes[:, q], ul)
     object.__setattr__(m, "ulevels", ul)
     object.__setattr__(m, "ulevels_nod2D", uln)
+    object.__setattr__(m, "ulevels_nod2D_max",
+                       ulmax.astype(np.asarray(mesh.ulevels_nod2D_max).dtype))
     return m
 
 

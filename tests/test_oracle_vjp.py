@@ -4,6 +4,8 @@ points (fesom_jax/tests/test_tracers.py:609-626 does the same with jax.grad), an
 exact transpose identity of the linear compute_w map. Exact-tie conventions are
 restated from JAX's rules and are not pinned here (JAX is not installed)."""
 
+import sys
+
 import numpy as np
 import pytest
 
@@ -12,23 +14,33 @@ from oracle import fct_numpy as O
 from oracle import fct_vjp as V
 
 
-def _fwd(m, c, T, Told, uv=None, w=None):
+def _fwd(m, c, T, Told, uv=None, w=None, wi=None, opts=None):
+    """Forward step; ``opts`` may hold ``w_i`` (replaced by ``wi`` when given), ``Z3d``
+    and ``zbar3``."""
     uv = c["uv"] if uv is None else uv
     w = c["w_e"] if w is None else w
+    o = dict(opts or {})
+    if wi is not None:
+        o["w_i"] = wi
     return O.advect_one_fct(m, uv, w, c["helem"], c["hnode"], c["hnode_new"], T, Told,
-                            dt=c["dt"], return_parts=True)
+                            dt=c["dt"], return_parts=True, **o)
 
 
-def _signature(m, c, T, Told, uv=None, w=None):
+def _signature(m, c, T, Told, uv=None, w=None, wi=None, opts=None):
     """Every discrete choice the step makes: limiter branches, signs, which argument of
-    each min/max wins, which triangle and vertex realise each cluster extremum."""
-    _, _, p = _fwd(m, c, T, Told, uv, w)
+    each min/max wins, which triangle and vertex realise each cluster extremum, and the
+    upwind side of the split velocities."""
+    _, _, p = _fwd(m, c, T, Told, uv, w, wi, opts)
     LO, pf, mf = p["LO"], p["plus_fac"], p["minus_fac"]
     adf_h, adf_v = p["adf_h"], p["adf_v"]
     n1, n2 = m.edges[:, 0], m.edges[:, 1]
     parts = [pf < 1.0, mf < 1.0, adf_h >= 0, adf_v >= 0, p["flux_h_lo"] >= 0,
              np.maximum(LO, T) == LO, np.minimum(LO, T) == LO,
              p["fct_ttf_max"] == 0, p["fct_ttf_min"] == 0]
+    we = c["w_e"] if w is None else w
+    w_i = wi if wi is not None else (opts or {}).get("w_i")
+    if w_i is not None:
+        parts += [np.sign(w_i), np.sign(we + w_i)]
     pos = adf_h >= 0
     parts.append(np.where(pos, pf[n1], mf[n1]) <= np.where(pos, mf[n2], pf[n2]))
     parts.append(O._shift_down(mf) <= pf)
@@ -52,19 +64,22 @@ def _signature(m, c, T, Told, uv=None, w=None):
     return [np.asarray(x).copy() for x in parts]
 
 
-def _fd_check(m, c, T, Told, which, rng, tol=1e-6, ndir=16, need=4):
+def _fd_check(m, c, T, Told, which, rng, tol=1e-6, ndir=16, need=4, opts=None):
     """Compare the Richardson-extrapolated central difference of <g_out, F> with
-    <grad, d> along `ndir` directions localised on one random node column (T, T_old, w)
-    or one triangle column (uv). Each direction uses the largest step in 1e-4..1e-6
-    whose stencil crosses no kink; directions with none are skipped. At least `need`
-    directions must be usable and all must agree to max(tol |ad|, rounding floor)."""
+    <grad, d> along `ndir` directions localised on one random node column (T, T_old, w,
+    w_i) or one triangle column (uv). Each direction uses the largest step in
+    1e-4..1e-6 whose stencil crosses no kink; directions with none are skipped. At
+    least `need` directions must be usable and all must agree to max(tol |ad|, rounding
+    floor). ``opts``: ``w_i``, ``Z3d``, ``zbar3`` of the step."""
+    opts = dict(opts or {})
     g_out = rng.standard_normal((m.nod2D, m.nl)) * m.node_layer_mask
     grads = V.advect_one_fct_vjp(m, c["uv"], c["w_e"], c["helem"], c["hnode"],
-                                 c["hnode_new"], T, Told, g_out, dt=c["dt"])
-    gT, gTo, guv, gw = grads
-    base = {"T": T, "T_old": Told, "uv": c["uv"], "w": c["w_e"]}[which]
-    grad = {"T": gT, "T_old": gTo, "uv": guv, "w": gw}[which]
-    s0 = _signature(m, c, T, Told)
+                                 c["hnode_new"], T, Told, g_out, dt=c["dt"], **opts)
+    gT, gTo, guv, gw = grads[:4]
+    gwi = grads[4] if len(grads) > 4 else None
+    base = {"T": T, "T_old": Told, "uv": c["uv"], "w": c["w_e"], "w_i": opts.get("w_i")}[which]
+    grad = {"T": gT, "T_old": gTo, "uv": guv, "w": gw, "w_i": gwi}[which]
+    s0 = _signature(m, c, T, Told, opts=opts)
     good = 0
     for _ in range(ndir):
         d = np.zeros_like(base)
@@ -73,13 +88,15 @@ def _fd_check(m, c, T, Told, which, rng, tol=1e-6, ndir=16, need=4):
             d[e] = rng.standard_normal(d[e].shape) * m.elem_layer_mask[e][:, None]
         else:
             n = int(rng.integers(m.nod2D))
-            mask = m.node_iface_mask[n] if which == "w" else m.node_layer_mask[n]
+            mask = m.node_iface_mask[n] if which in ("w", "w_i") else m.node_layer_mask[n]
             d[n] = rng.standard_normal(m.nl) * mask
         d *= np.abs(base).max()
         args = lambda s: dict(T=T + s * d if which == "T" else T,
                               Told=Told + s * d if which == "T_old" else Told,
                               uv=c["uv"] + s * d if which == "uv" else None,
-                              w=c["w_e"] + s * d if which == "w" else None)
+                              w=c["w_e"] + s * d if which == "w" else None,
+                              wi=opts["w_i"] + s * d if which == "w_i" else None,
+                              opts=opts)
         for h in (1e-4, 1e-5, 1e-6):
             sp = _signature(m, c, **args(h))
             sm = _signature(m, c, **args(-h))
@@ -141,6 +158,101 @@ def test_vjp_icosa_nl70():
     c = make_case(m, seed=32, dt=2700.0)
     T = c["T"][0]
     _fd_check(m, c, T, 0.995 * T, "T", np.random.default_rng(3), ndir=8, need=3)
+
+
+def _split_opts(m, c, seed):
+    """An implicit split w_i of both signs (about a third of |w_e|) and a moving-level
+    stencil: zbar3 stretched per column, Z3d the layer mid-depths."""
+    rng = np.random.default_rng(seed)
+    wm = m.node_iface_mask
+    w_i = np.where(wm, 0.3 * np.abs(c["w_e"]).max() * rng.standard_normal(wm.shape), 0.0)
+    zb3 = m.zbar_3d_n * (1.0 + 0.01 * rng.random(m.zbar_3d_n.shape))
+    Z3d = np.concatenate([0.5 * (zb3[:, :-1] + zb3[:, 1:]), zb3[:, -1:]], axis=1)
+    return w_i, Z3d, zb3
+
+
+@pytest.mark.parametrize("which", ["T", "T_old", "uv", "w", "w_i"])
+@pytest.mark.parametrize("sharp", [False, True])
+def test_vjp_w_split_matches_fd(small, which, sharp):
+    """The implicit split: the Thomas recurrence reversed, w_e + w_i in adf_v, and the
+    cotangent of w_i."""
+    m, c0 = small
+    c = sharp_case(m, c0, 2.0) if sharp else c0
+    w_i, _, _ = _split_opts(m, c, 7)
+    T = c["T"][0]
+    Told = c["T_old"][0] if sharp else 0.99 * T + 0.01 * T[::-1]
+    _fd_check(m, c, T, Told, which, np.random.default_rng(8), ndir=10, need=3,
+              opts=dict(w_i=w_i))
+
+
+@pytest.mark.parametrize("which", ["T", "T_old", "w"])
+def test_vjp_zstar_matches_fd(small, which):
+    """The moving-level QR4C stencil (Z3d, zbar3)."""
+    m, c = small
+    _, Z3d, zb3 = _split_opts(m, c, 9)
+    T = c["T"][0]
+    _fd_check(m, c, T, 0.99 * T + 0.01 * T[::-1], which, np.random.default_rng(10),
+              ndir=10, need=3, opts=dict(Z3d=Z3d, zbar3=zb3))
+
+
+def test_vjp_split_and_zstar_together(small):
+    m, c = small
+    w_i, Z3d, zb3 = _split_opts(m, c, 11)
+    T = c["T"][0]
+    _fd_check(m, c, T, 0.995 * T, "w_i", np.random.default_rng(12), ndir=8, need=3,
+              opts=dict(w_i=w_i, Z3d=Z3d, zbar3=zb3))
+
+
+def test_vimpl_vjp_is_exact_transpose(small):
+    """adv_tra_vert_impl is linear in its tracer: <M(dX), g> = <dX, M^T g> to rounding,
+    and at w_i = 0 it is the identity (so is its transpose)."""
+    m, c = small
+    w_i, _, _ = _split_opts(m, c, 13)
+    rng = np.random.default_rng(14)
+    dX = rng.standard_normal((m.nod2D, m.nl)) * m.node_layer_mask
+    g = rng.standard_normal((m.nod2D, m.nl))
+    lhs = float(np.sum(O.adv_tra_vert_impl(m, w_i, dX, c["hnode_new"], dt=c["dt"]) * g))
+    gX, _ = V._vimpl_vjp(m, w_i, c["hnode_new"], c["dt"], dX, g)
+    rhs = float(np.sum(dX * gX))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    gX0, gw0 = V._vimpl_vjp(m, np.zeros_like(w_i), c["hnode_new"], c["dt"], dX, g)
+    assert np.array_equal(gX0, g)
+
+
+_VIMPL_MISTAKES = {
+    # the back substitution's carry x[i] -> x[i-1] dropped
+    "back substitution": ("carry = -G * cp[:, i]", "carry = 0.0 * G"),
+    # w_i[k+1] (the lower face of layer k) credited to w_i[k]
+    "lower-face shift": ("return g_X, g_wu + _su_T(g_wd)", "return g_X, g_wu + g_wd"),
+    # the off-diagonal terms of the right-hand side left out of the tracer's cotangent
+    "rhs transpose": ("g_X += _sd_T(gs * a) + gs * b_adv + _su_T(gs * c)",
+                      "g_X += gs * b_adv"),
+}
+
+
+@pytest.mark.parametrize("mistake", sorted(_VIMPL_MISTAKES))
+def test_planted_vimpl_mistake_fails_fd(small, monkeypatch, mistake):
+    """Each mistake planted in the oracle's reversed implicit solve fails the FD check
+    in w_i or in T."""
+    import types
+    m, c = small
+    w_i, _, _ = _split_opts(m, c, 7)
+    old, new = _VIMPL_MISTAKES[mistake]
+    src = open(V.__file__).read()
+    assert src.count(old) == 1
+    bad = types.ModuleType("oracle.fct_vjp_planted")
+    bad.__package__ = "oracle"
+    exec(compile(src.replace(old, new), V.__file__, "exec"), bad.__dict__)
+    monkeypatch.setattr(sys.modules[__name__], "V", bad)
+    T = c["T"][0]
+    caught = 0
+    for which in ("w_i", "T"):
+        try:
+            _fd_check(m, c, T, 0.99 * T + 0.01 * T[::-1], which, np.random.default_rng(8),
+                      ndir=10, need=3, opts=dict(w_i=w_i))
+        except AssertionError:
+            caught += 1
+    assert caught, mistake
 
 
 def test_compute_w_vjp_is_exact_transpose(pi_mesh, pi_case):

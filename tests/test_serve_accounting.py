@@ -1,0 +1,75 @@
+"""Host-side accounting of the server's HBM budget (no GPU): what an admission may
+release to make room."""
+
+import collections
+
+from fesom_jax_amd import serve
+
+
+class _FakePlan:
+    def __init__(self, nbytes):
+        self.nbytes = nbytes
+        self.closed = False
+
+    def close(self):
+        self.closed = True
+
+
+class _Spec:
+    def __init__(self, level, nl):
+        self.level, self.nl = level, nl
+
+
+class _Active:
+    def __init__(self, level, nl, nbytes):
+        self.spec = _Spec(level, nl)
+        self.bytes = nbytes
+
+
+def _server(budget):
+    srv = serve.Server("cpu", budget_bytes=budget)
+    return srv
+
+
+def test_release_keeps_the_plan_the_admission_reuses(monkeypatch):
+    """An admission that will reuse an idle plan counts no plan bytes in its need, so
+    that plan must survive the release: closing it would make the admission build a new
+    plan the budget never saw."""
+    srv = _server(1000)
+    reused, other = _FakePlan(400), _FakePlan(300)
+    srv.pool[(4, 47, 2)].append(reused)
+    srv.pool[(5, 47, 2)].append(other)
+    srv.plan_bytes = {id(reused): 400, id(other): 300}
+    ok = srv._release_idle(500, [], keep=reused)
+    assert other.closed and not reused.closed
+    assert ok            # 400 held + 500 needed fit 1000
+    srv2 = _server(800)
+    a, b = _FakePlan(400), _FakePlan(300)
+    srv2.pool[(4, 47, 2)].append(a)
+    srv2.pool[(5, 47, 2)].append(b)
+    srv2.plan_bytes = {id(a): 400, id(b): 300}
+    assert not srv2._release_idle(500, [], keep=a)    # only `a` could make room
+    assert not a.closed
+
+
+def test_release_keeps_the_admissions_base_inputs(monkeypatch):
+    dropped = []
+    monkeypatch.setattr(serve, "_drop_base_case", lambda mesh, dev, dt: dropped.append(mesh))
+    srv = _server(1000)
+    srv.meshes = type("M", (), {"get": staticmethod(lambda level, nl: (level, nl))})()
+    srv.base_bytes = {(9, 70): 600, (8, 70): 300}
+    ok = srv._release_idle(500, [_Active(8, 70, 0)], keep_base=(9, 70))
+    # (9, 70) is the asking admission's, (8, 70) an active request's: neither goes
+    assert dropped == [] and not ok
+    ok = srv._release_idle(500, [], keep_base=(8, 70))
+    assert dropped == [(9, 70)] and ok
+
+
+def test_held_bytes_sums_every_part():
+    srv = _server(1 << 40)
+    p = _FakePlan(7)
+    srv.plan_bytes = {id(p): 7}
+    srv.base_bytes = {(3, 47): 11}
+    srv._finishing = [(_Active(3, 47, 13), None, None, None)]
+    assert srv.held_bytes([_Active(3, 47, 17)]) == 7 + 11 + 13 + 17
+    assert isinstance(srv.timing, collections.Counter)
