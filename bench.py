@@ -142,6 +142,8 @@ def main():
     ap.add_argument("--shuffle", type=int, default=None, metavar="SEED",
                     help="plain: renumber nodes, triangles and edges at random (the plan's "
                          "locality order must make up for a numbering without locality)")
+    ap.add_argument("--fd-directions", type=int, default=20,
+                    help="adjoint workload: finite-difference directions checked (0 skips)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, run a 1 GiB read + 1 GiB write elementwise "
                          "kernel (the reference for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -511,6 +513,8 @@ def run_adjoint(args, dev):
     T0 = [t.clone().requires_grad_(True) for t in case["T"]]
     To0 = [t.clone() for t in case["T"]]
     uv = case["uv"].clone().requires_grad_(True)
+    # warm-up (untimed): the first backward allocates the plan's adjoint workspace
+    torch.autograd.grad(loss_of(T0, To0, uv), [*T0, uv])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     with torch.no_grad():
@@ -524,6 +528,24 @@ def run_adjoint(args, dev):
     torch.cuda.synchronize()
     t_fb = time.perf_counter() - t0
     log(f"forward {t_fwd:.3f}s, forward+adjoint {t_fb:.3f}s, loss {float(L0):.6e}")
+    # one step's backward alone (fct_backward, both tracers), HIP events on the stream
+    # the library launches on, against its compulsory bytes
+    from fesom_jax_amd.adjoint import advect_fct_vjp
+    from fesom_jax_amd.perfmodel import HBM_PEAK_GBS, adjoint_bytes
+    gnew = [torch.randn(t.shape, generator=g, **f64) for t in case["T"]]
+    bwd = lambda: advect_fct_vjp(mesh, case["uv"], *fixed, case["T"], To0, gnew, dt=args.dt)
+    bwd()
+    reps = 10
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        bwd()
+    e1.record()
+    torch.cuda.synchronize()
+    t_bwd = e0.elapsed_time(e1) / reps * 1e-3
+    B_adj = adjoint_bytes(mesh, K)
+    log(f"one step's backward {1e3 * t_bwd:.3f} ms, {B_adj / t_bwd / 1e9:.1f} GB/s compulsory")
     # central differences along 20 seeded directions, each on one random column (a wet
     # node's T or S column, or a triangle's uv column) near the surface signal, so that
     # few limiter switches lie on the segment; step h = 1e-6 * max|x| (and 1e-5, 1e-7:
@@ -533,7 +555,7 @@ def run_adjoint(args, dev):
     base = [x.detach() for x in T0] + [uv.detach()]
     wet_nodes = np.nonzero(mesh.node_layer_mask[:, 0])[0]
     with torch.no_grad():
-        for r in range(20):
+        for r in range(args.fd_directions):
             which = r % 3
             d = torch.zeros_like(base[which])
             if which < 2:
@@ -555,9 +577,9 @@ def run_adjoint(args, dev):
                 fd = (at(1.0) - at(-1.0)) / (2 * h)
                 best = min(best, abs(fd - ad) / max(abs(fd), abs(ad), 1e-300))
             rel.append(best)
-    rel = np.array(rel)
+    rel = np.array(rel if rel else [np.nan])
     log(f"FD check: median rel {np.median(rel):.2e}, max {rel.max():.2e}, "
-        f"{int((rel < 1e-5).sum())}/20 below 1e-5")
+        f"{int((rel < 1e-5).sum())}/{args.fd_directions} below 1e-5")
     U = updates_per_step(mesh, K) * nsteps
     out = {
         "metric": "FP64 FCT forward+adjoint rollout (12 steps) node-level updates/s",
@@ -568,10 +590,17 @@ def run_adjoint(args, dev):
         "config": {"workload": f"icosa-{level} x nl={nl}, {K} tracers, {nsteps} steps + adjoint",
                    "nodes": mesh.nod2D, "updates_forward": U},
         "forward_s": t_fwd, "forward_backward_s": t_fb, "backward_over_forward": (t_fb - t_fwd) / t_fwd,
-        "fd_check": {"directions": 20, "kind": "one random column each, h in 1e-6/1e-5/1e-7",
+        "fd_check": {"directions": args.fd_directions,
+                     "kind": "one random column each, h in 1e-6/1e-5/1e-7",
                      "median_rel": float(np.median(rel)),
                      "max_rel": float(rel.max()), "below_1e-5": int((rel < 1e-5).sum())},
-        "roofline": None, "cpu_baseline": None,
+        "backward_step_ms": 1e3 * t_bwd,
+        "roofline": {"bound": "hbm", "achieved": round(B_adj / t_bwd / 1e9, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(B_adj / t_bwd / 1e9 / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "fct_backward (all launches of one step's backward, 2 tracers)",
+                     "bytes_per_step": B_adj},
+        "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 

@@ -36,3 +36,18 @@ def algorithmic_bytes(mesh, ntracers: int) -> int:
     N, E, D, L = int(mesh.nod2D), int(mesh.elem2D), int(mesh.edge2D), int(mesh.nl)
     return (8 * (3 * K * w["sum_Ln"] + 3 * w["sum_Le"] + 2 * w["sum_In"] + 3 * w["sum_Ln"])
             + 72 * D + 80 * E + 16 * N + 4 * (3 * E + N + 1) + 16 * L)
+
+
+def adjoint_bytes(mesh, ntracers: int) -> int:
+    """Compulsory bytes of one step's backward (``fct_backward``): the step's inputs
+    read once (T, T_old, flow, thicknesses, mesh), the cotangent g_T_new read, and
+    g_T, g_T_old, g_uv, g_w_e written once, wet entries only:
+
+        8 * [5K*sum_Ln + 5*sum_Le + 3*sum_In + 3*sum_Ln] + topology (as above).
+
+    The recomputed forward intermediates are not compulsory and are not counted."""
+    w = wet_counts(mesh)
+    K = int(ntracers)
+    N, E, D, L = int(mesh.nod2D), int(mesh.elem2D), int(mesh.edge2D), int(mesh.nl)
+    return (8 * (5 * K * w["sum_Ln"] + 5 * w["sum_Le"] + 3 * w["sum_In"] + 3 * w["sum_Ln"])
+            + 72 * D + 80 * E + 16 * N + 4 * (3 * E + N + 1) + 16 * L)
