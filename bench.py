@@ -453,16 +453,23 @@ def run_serve(args, dev):
             meshes.get(lv, nl)
     log(f"{len(specs)} requests over {args.duration}s; meshes ready ({time.time() - t0:.1f}s)")
     # warm-up, untimed: one forward+adjoint request per (level, nl) through both paths,
-    # so that base inputs, plans and adjoint workspaces exist before either is timed
+    # so that meshes and base inputs exist and both code paths have run before either
+    # is timed (each path's plans are released before the next one runs)
     from fesom_jax_amd.serve import RequestSpec
     warm = [RequestSpec(rid=i, arrival=0.0, level=lv, nl=nl, K=2, nsteps=2, adjoint=True,
                         seed=i) for i, (lv, nl) in enumerate(
                             (lv, nl) for lv in range(lo, hi + 1) for nl in (47, 70))]
-    Server(dev, meshes=meshes).run(warm)
+    warm_srv = Server(dev, meshes=meshes)
+    warm_srv.run(warm)
+    warm_srv.close()
+    torch.cuda.empty_cache()
     fifo_loop(warm, dev, meshes=meshes)
+    torch.cuda.empty_cache()
     torch.cuda.synchronize()
+    log("warm-up done")
     srv = Server(dev, budget_bytes=200 << 30, max_active=64, meshes=meshes)
-    served = srv.run(specs)
+    served = srv.run(specs, log=log)
+    srv.close()
     log(f"served: {json.dumps(served)}")
     torch.cuda.empty_cache()
     fifo = fifo_loop(specs, dev, meshes=meshes)

@@ -18,6 +18,7 @@ with: one request at a time, each with its own plan, run and download, synchroni
 from __future__ import annotations
 
 import collections
+import itertools
 import time
 from dataclasses import dataclass, field
 
@@ -94,13 +95,15 @@ def estimate_plan_bytes(mesh, K, adjoint):
 
 def estimate_field_bytes(mesh, K, nsteps, adjoint):
     """Device bytes of a request's own fields: flow, three buffers per tracer and
-    (adjoint) the checkpointed states and one recomputed segment of them."""
+    (adjoint) the target, the checkpointed states, one recomputed segment of them and
+    the cotangents a step's backward holds (g_T, g_T_old in and out, g_uv twice)."""
     N, E, L = mesh.nod2D, mesh.elem2D, mesh.nl
     NL, EL = N * L, E * L
     total = 8 * (3 * EL + 3 * NL + 3 * K * NL)
     if adjoint:
         # checkpoints and one recomputed segment of states (T, T_old)
         total += 8 * (2 * checkpoint_stride(nsteps) + 1) * 2 * K * NL
+        total += 8 * (5 * K * NL + 4 * EL)
     return int(total)
 
 
@@ -143,27 +146,37 @@ class _Active:
 class Server:
     """Continuous-batching loop; see module docstring.
 
-    Memory: the budget covers every request's own fields and every plan the server
-    holds, active or idle in the pool, at the plan's own byte count (its adjoint
-    workspace included once an adjoint has run on it). Idle plans are released when
-    an admission would not fit otherwise.
+    Memory: the budget covers every request's own fields, the base inputs of the
+    meshes in use and every plan the server holds, active or idle in the pool, at the
+    plan's own byte count. The adjoint workspace lives in one adjoint plan per mesh
+    (the adjoints run one after another on the finishing stream), counted from the
+    first admission that asks for it. Idle plans and unused base inputs are released
+    when an admission would not fit otherwise.
+
+    Admission is in arrival order, except that while the oldest waiting request does
+    not fit (for at most ``backfill_s`` seconds) a later arrival that fits as things
+    stand may start first.
 
     Failures stay with their request: a request whose setup raises, whose batched step
     fails (found by stepping the batch's instances one at a time), or whose result or
-    gradients are not finite on the wet lanes is reported with an ``"error"`` entry and
+    gradients are not finite is reported with an ``"error"`` entry and
     leaves; the others carry on."""
 
     def __init__(self, device, *, budget_bytes=200 << 30, max_active=64, meshes=None,
-                 dt=2700.0, chunk=4):
+                 dt=2700.0, chunk=4, backfill_s=1.0, lookahead=32):
         self.device = device
         self.chunk = int(chunk)        # steps per iteration while requests are waiting
+        self.backfill_s = float(backfill_s)   # longest wait of the oldest request that
+        self.lookahead = int(lookahead)       # later arrivals (this many) may overtake
+        self.pinned_bytes = 1 << 30           # results up to this size return pinned
         self.budget = int(budget_bytes)
         self.max_active = int(max_active)
         self.meshes = meshes or MeshCache()
         self.dt = dt
         self.pool = collections.defaultdict(list)   # (level, nl, K) -> idle plans
         self.plan_bytes = {}                        # id(plan) -> device bytes it holds
-        self.adjoint_plans = set()                  # ids of plans that ran an adjoint
+        self.adj_plans = {}                         # (level, nl) -> the mesh's adjoint plan
+        self.adj_ready = set()                      # ids of adjoint plans with workspace
         self.base_bytes = {}                        # (level, nl) -> base input bytes
         self.results = {}
         self.latency = {}
@@ -172,7 +185,9 @@ class Server:
         self.active = []
         self.peak_bytes = 0
         self.peak_detail = None
+        self.device_peak = 0        # device memory in use (all users), highest seen
         self.timing = collections.Counter()         # seconds per activity (host clock)
+        self.counts = collections.Counter()         # plans built/closed, base inputs built/dropped
         self._finishing = []
 
     # -- memory accounting ---------------------------------------------------------
@@ -192,13 +207,16 @@ class Server:
         need = estimate_field_bytes(m, s.K, s.nsteps, s.adjoint)
         if (s.level, s.nl) not in self.base_bytes:
             need += estimate_base_bytes(m)
+        if s.adjoint and (s.level, s.nl) not in self.adj_plans:
+            need += estimate_plan_bytes(m, 1, True)
         if not self.pool[key]:
-            return need + estimate_plan_bytes(m, s.K, s.adjoint)
-        if s.adjoint and id(self.pool[key][-1]) not in self.adjoint_plans:
-            need += estimate_plan_bytes(m, s.K, True) - estimate_plan_bytes(m, s.K, False)
+            need += estimate_plan_bytes(m, s.K, False)
         return need
 
     def _note_peak(self, active):
+        import torch
+        free, total = torch.cuda.mem_get_info(self.device)
+        self.device_peak = max(self.device_peak, total - free)
         held = self.held_bytes(active)
         if held > self.peak_bytes:
             self.peak_bytes = held
@@ -218,15 +236,26 @@ class Server:
         meshes no active or finishing request uses, nor the asking admission
         (``keep_base``; they are rebuilt when needed), until ``need`` more bytes fit;
         returns whether they do."""
-        idle = sorted(((self.plan_bytes[id(p)], k, p) for k, ps in self.pool.items() for p in ps
-                       if p is not keep), key=lambda x: -x[0])
+        # an adjoint plan is in use while a request of its mesh that asks for the adjoint
+        # is active or finishing, and kept for the asking admission's mesh
+        adj_use = {(a.spec.level, a.spec.nl) for a in list(active) + list(busy) if a.spec.adjoint}
+        if keep_base is not None:
+            adj_use.add(keep_base)
+        idle = sorted([(self.plan_bytes[id(p)], k, p) for k, ps in self.pool.items() for p in ps
+                       if p is not keep] +
+                      [(self.plan_bytes[id(p)], k, p) for k, p in self.adj_plans.items()
+                       if k not in adj_use], key=lambda x: -x[0])
         for nbytes, key, plan in idle:
             if self.held_bytes(active) + need <= self.budget:
                 return True
-            self.pool[key].remove(plan)
+            if key in self.adj_plans and self.adj_plans[key] is plan:
+                del self.adj_plans[key]
+                self.adj_ready.discard(id(plan))
+            else:
+                self.pool[key].remove(plan)
             del self.plan_bytes[id(plan)]
-            self.adjoint_plans.discard(id(plan))
             plan.close()
+            self.counts["plans_closed"] += 1
         in_use = {(a.spec.level, a.spec.nl) for a in list(active) + list(busy)}
         if keep_base is not None:
             in_use.add(keep_base)
@@ -237,6 +266,7 @@ class Server:
                 continue
             _drop_base_case(self.meshes.get(*key), self.device, self.dt)
             del self.base_bytes[key]
+            self.counts["base_dropped"] += 1
         return self.held_bytes(active) + need <= self.budget
 
     # -- request life cycle --------------------------------------------------------
@@ -253,7 +283,25 @@ class Server:
         plan = FctPlan(mesh, self.device, max_tracers=spec.K)
         self._refresh(plan)
         self.timing["plan_create"] += time.perf_counter() - t
+        self.counts["plans_built"] += 1
         return plan
+
+    def _adjoint_plan(self, mesh, spec):
+        """The mesh's adjoint plan (one tracer of fused workspace: it only runs the
+        backward, one request after another on the finishing stream); its adjoint
+        workspace is counted from admission on."""
+        from .tracer_adv import FctPlan
+        import torch
+        key = (spec.level, spec.nl)
+        if key not in self.adj_plans:
+            torch.cuda.empty_cache()
+            t = time.perf_counter()
+            plan = FctPlan(mesh, self.device, max_tracers=1)
+            self.adj_plans[key] = plan
+            self.plan_bytes[id(plan)] = estimate_plan_bytes(mesh, 1, True)
+            self.timing["plan_create"] += time.perf_counter() - t
+            self.counts["adjoint_plans_built"] += 1
+        return self.adj_plans[key]
 
     def _start(self, spec, now):
         from .batch import Instance
@@ -261,6 +309,11 @@ class Server:
         mesh = self.meshes.get(spec.level, spec.nl)
         self.base_bytes.setdefault((spec.level, spec.nl), estimate_base_bytes(mesh))
         tc = time.perf_counter()
+        if not _has_base_case(mesh, self.device, self.dt):
+            _base_case(mesh, self.device, self.dt)
+            self.timing["base_inputs"] += time.perf_counter() - tc
+            self.counts["base_built"] += 1
+            tc = time.perf_counter()
         case = request_case(mesh, spec, self.device, self.dt)
         self.timing["inputs"] += time.perf_counter() - tc
         plan = self._plan(mesh, spec)
@@ -273,9 +326,8 @@ class Server:
         import torch
         torch.cuda.synchronize(self.device)
         nbytes = estimate_field_bytes(mesh, spec.K, spec.nsteps, spec.adjoint)
-        if spec.adjoint and id(plan) not in self.adjoint_plans:
-            # the plan grows by its adjoint workspace when the request finishes
-            nbytes += estimate_plan_bytes(mesh, spec.K, True) - estimate_plan_bytes(mesh, spec.K, False)
+        if spec.adjoint:
+            self._adjoint_plan(mesh, spec)
         a = _Active(spec=spec, inst=inst, mesh=mesh, case_extra=extra, remaining=spec.nsteps,
                     bytes=nbytes, t_admit=now, setup_s=time.perf_counter() - t)
         if spec.adjoint:
@@ -289,26 +341,41 @@ class Server:
         steps queued so far; returns the event that marks its completion, the host
         buffers and the device tensors they come from."""
         import torch
+        mkey = (a.spec.level, a.spec.nl)
+        adj_plan = self.adj_plans.get(mkey)
+        if a.spec.adjoint and adj_plan is not None and id(adj_plan) not in self.adj_ready:
+            # the plan's first adjoint allocates its workspace: hand back what torch
+            # caches but no tensor uses
+            torch.cuda.empty_cache()
         ev = torch.cuda.Event()
         ev.record(torch.cuda.current_stream(self.device))
         side.wait_event(ev)
         with torch.cuda.stream(side):
             out = {"T": [x for x in a.inst.T()]}
             if a.spec.adjoint:
-                out.update(adjoint_of_rollout(a, self.dt, to_host=False))
-            wet = torch.from_numpy(a.mesh.node_layer_mask).to(self.device)
+                out.update(adjoint_of_rollout(a, self.dt, to_host=False, adj_plan=adj_plan))
+                self.adj_ready.add(id(adj_plan))
+            # one reduction per field (a NaN or an infinity anywhere makes the sum
+            # non-finite), so the check allocates nothing the size of a field
             bad = torch.zeros((), dtype=torch.bool, device=self.device)
             for x in out["T"] + out.get("g_T0", []):
-                bad |= ~torch.isfinite(x[wet]).all()
+                bad |= ~torch.isfinite(x.sum())
             if "g_uv" in out:
-                bad |= ~torch.isfinite(out["g_uv"]).all()
+                bad |= ~torch.isfinite(out["g_uv"].sum())
             host = {}
-            for k, v in out.items():
-                vs = v if isinstance(v, list) else [v]
-                hs = [torch.empty(x.shape, dtype=x.dtype, pin_memory=True) for x in vs]
-                for h, x in zip(hs, vs):
-                    h.copy_(x, non_blocking=True)
-                host[k] = hs if isinstance(v, list) else hs[0]
+            nbytes = sum(x.numel() * x.element_size()
+                         for v in out.values() for x in (v if isinstance(v, list) else [v]))
+            if nbytes <= self.pinned_bytes:
+                # small results come back through pinned buffers on the finishing stream
+                for k, v in out.items():
+                    vs = v if isinstance(v, list) else [v]
+                    hs = [torch.empty(x.shape, dtype=x.dtype, pin_memory=True) for x in vs]
+                    for h, x in zip(hs, vs):
+                        h.copy_(x, non_blocking=True)
+                    host[k] = hs if isinstance(v, list) else hs[0]
+            else:
+                # large ones are copied out once finished (pinned host memory is scarce)
+                host["_deferred"] = True
             host["_bad"] = torch.empty((), dtype=torch.bool, pin_memory=True)
             host["_bad"].copy_(bad, non_blocking=True)
             done = torch.cuda.Event()
@@ -368,8 +435,21 @@ class Server:
                                            keep=ps[-1] if ps else None,
                                            keep_base=(s.level, s.nl))
                         and (active or finishing)):
-                    break
-                pending.popleft()
+                    # the oldest request waits for memory; until it has waited
+                    # `backfill_s`, a later arrival that fits as things stand goes ahead
+                    s = None
+                    if now - pending[0].arrival < self.backfill_s:
+                        held = self.held_bytes(active)
+                        for c in itertools.islice(pending, 1, 1 + self.lookahead):
+                            if c.arrival > now:
+                                break
+                            if held + self._need(c) <= self.budget:
+                                s = c
+                                break
+                    if s is None:
+                        break
+                    self.counts["backfilled"] += 1
+                pending.remove(s)
                 try:
                     a = self._start(s, now)
                 except Exception as exc:          # setup failure stays with its request
@@ -394,9 +474,24 @@ class Server:
                 if changed:
                     tt = time.perf_counter()
                     if batch is not None:
-                        retired.append(batch)     # closed once the GPU is idle
-                    batch = FctBatch([a.inst for a in active])
-                    batch.prepare()
+                        # closed once the launches queued so far have run
+                        ev = torch.cuda.Event()
+                        ev.record(torch.cuda.current_stream(self.device))
+                        retired.append((batch, ev))
+                    try:
+                        batch = FctBatch([a.inst for a in active])
+                        batch.prepare()
+                    except (RuntimeError, MemoryError):
+                        # out of device memory for the batch's tables: free what the
+                        # retired batches and torch's cache hold, then once more
+                        torch.cuda.synchronize(self.device)
+                        for old, _ in retired:
+                            old.close()
+                        retired.clear()
+                        torch.cuda.empty_cache()
+                        self.counts["batch_retry"] += 1
+                        batch = FctBatch([a.inst for a in active])
+                        batch.prepare()
                     changed = False
                     self.timing["batch_build"] += time.perf_counter() - tt
                 # steps until the next completion; at most `chunk` while requests wait
@@ -432,11 +527,27 @@ class Server:
                     changed = True
                     try:
                         tt = time.perf_counter()
-                        ev, host, keep = self._finish_async(a, side)
+                        try:
+                            ev, host, keep = self._finish_async(a, side)
+                        except (RuntimeError, MemoryError) as exc:
+                            if "out of memory" not in str(exc):
+                                raise
+                            # let the finishing work already queued run and free its
+                            # memory, then once more
+                            side.synchronize()
+                            torch.cuda.empty_cache()
+                            self.counts["finish_retry"] += 1
+                            ev, host, keep = self._finish_async(a, side)
                         self.timing["finish_queued"] += time.perf_counter() - tt
                     except (RuntimeError, MemoryError) as exc:
                         side.synchronize()
-                        self._fail(a.spec, t0, f"finishing failed: {exc}", log)
+                        free, total = torch.cuda.mem_get_info(self.device)
+                        self._fail(a.spec, t0, f"finishing failed: {exc} (icosa-{a.spec.level} "
+                                   f"x {a.spec.nl}, K {a.spec.K}, {a.spec.nsteps} steps, "
+                                   f"adjoint {a.spec.adjoint}; held "
+                                   f"{self.held_bytes(active) / 2**30:.1f} GiB, torch allocated "
+                                   f"{torch.cuda.memory_allocated(self.device) / 2**30:.1f} GiB, "
+                                   f"device free {free / 2**30:.1f} GiB)", log)
                         self.pool[(a.spec.level, a.spec.nl, a.spec.K)].append(a.inst.plan)
                         self._refresh(a.inst.plan)
                         continue
@@ -446,6 +557,9 @@ class Server:
             for a, ev, host, keep in finishing:
                 if ev.query():
                     bad = bool(host.pop("_bad"))
+                    if host.pop("_deferred", False):
+                        host = {k: ([x.cpu() for x in v] if isinstance(v, list) else v.cpu())
+                                for k, v in keep.items()}
                     if bad:
                         self._fail(a.spec, t0, "non-finite result", log)
                     else:
@@ -453,29 +567,36 @@ class Server:
                             k: ([h.numpy() for h in v] if isinstance(v, list) else v.numpy())
                             for k, v in host.items()}
                         self.latency[a.spec.rid] = time.perf_counter() - t0 - a.spec.arrival
-                    self._refresh(a.inst.plan)     # the adjoint may have grown it
-                    if a.spec.adjoint:
-                        self.adjoint_plans.add(id(a.inst.plan))
+                    mk = (a.spec.level, a.spec.nl)
+                    if a.spec.adjoint and mk in self.adj_plans:
+                        self._refresh(self.adj_plans[mk])   # its workspace now exists
                     self.pool[(a.spec.level, a.spec.nl, a.spec.K)].append(a.inst.plan)
                     if log and not bad:
-                        log(f"req {a.spec.rid} done in {self.latency[a.spec.rid]:.3f}s")
+                        free, total = torch.cuda.mem_get_info(self.device)
+                        log(f"req {a.spec.rid} (icosa-{a.spec.level} x {a.spec.nl}, K "
+                            f"{a.spec.K}, {a.spec.nsteps} steps{', adjoint' if a.spec.adjoint else ''})"
+                            f" done in {self.latency[a.spec.rid]:.3f}s; held "
+                            f"{self.held_bytes(active) / 2**30:.1f} GiB, device in use "
+                            f"{(total - free) / 2**30:.1f} GiB")
                 else:
                     still.append((a, ev, host, keep))
             finishing[:] = still         # the same list object as self._finishing
             self._note_peak(active)
-            if len(retired) > 16 and not finishing and not active:
-                for old in retired:
-                    old.close()
-                retired.clear()
+            done_r = [(b, e) for b, e in retired if e.query()]
+            for old, _ in done_r:
+                old.close()
+            retired[:] = [r for r in retired if r not in done_r]
         torch.cuda.synchronize(self.device)
-        for old in retired + ([batch] if batch is not None else []):
+        for old in [b for b, _ in retired] + ([batch] if batch is not None else []):
             old.close()
         wall = time.perf_counter() - t0
         out = summarize([s for s in specs if s.rid not in self.errors], self.latency,
                         self.setup, wall, busy, self.meshes)
         out["failed"] = len(self.errors)
         out["peak_bytes"] = int(self.peak_bytes)
+        out["device_peak_bytes"] = int(self.device_peak)
         out["timing_s"] = {k: round(v, 3) for k, v in self.timing.items()}
+        out["counts"] = dict(self.counts)
         return out
 
     def close(self):
@@ -483,9 +604,12 @@ class Server:
         for ps in self.pool.values():
             for p in ps:
                 p.close()
+        for p in self.adj_plans.values():
+            p.close()
         self.pool.clear()
+        self.adj_plans.clear()
+        self.adj_ready.clear()
         self.plan_bytes.clear()
-        self.adjoint_plans.clear()
 
 
 _BASE = {}
@@ -499,6 +623,10 @@ def _base_case(mesh, device, dt):
         _BASE[key] = make_case_device(mesh, device, seed=1234, dt=dt,
                                       tracers=("T", "S", "T", "S"))
     return _BASE[key]
+
+
+def _has_base_case(mesh, device, dt):
+    return (id(mesh), str(device), dt) in _BASE
 
 
 def _drop_base_case(mesh, device, dt):
@@ -532,12 +660,13 @@ def request_case(mesh, spec, device, dt):
     return case
 
 
-def adjoint_of_rollout(a, dt, to_host=True):
+def adjoint_of_rollout(a, dt, to_host=True, adj_plan=None):
     """Gradients of sum_surface (T_final - T_target)^2 with respect to the initial
     tracers (T and T_old of step 0) and uv. The backward runs segment by segment from
     the last checkpoint to the first: each segment's states are recomputed from its
     checkpoint with the request's own plan (the same bits as the steps served), then
-    the step adjoints run back through them."""
+    the step adjoints run back through them on ``adj_plan`` (a plan of the same mesh
+    holding the adjoint workspace; the request's own plan by default)."""
     import torch
     from .adjoint import advect_fct_vjp
     from .tracer_adv import advect_fct
@@ -563,7 +692,7 @@ def adjoint_of_rollout(a, dt, to_host=True):
             states.append((T, To))
         for T, To in reversed(states):
             gT, gTo_s, guv, _ = advect_fct_vjp(mesh, *flow, T, To, g, dt=dt, need_w=False,
-                                               plan=inst.plan)
+                                               plan=adj_plan or inst.plan)
             g_uv += guv
             g = [x + y for x, y in zip(gT, gTo)]
             gTo = gTo_s
@@ -577,14 +706,18 @@ def fifo_loop(specs, device, *, meshes=None, dt=2700.0, budget_bytes=200 << 30):
     """The plain per-request loop: requests in arrival order, one at a time; each
     builds its plan and inputs, runs its steps with the plain entry point, downloads
     its result and synchronises before the next one starts. Plans are kept per (mesh,
-    tracers) for later requests while their bytes stay within ``budget_bytes`` (the
-    least recently used ones are closed beyond it)."""
+    tracers) for later requests and base inputs per mesh while they, with the request's
+    own fields, stay within ``budget_bytes`` (the least recently used plans are closed
+    beyond it, then the base inputs of other meshes dropped)."""
     import torch
     from .batch import Instance
     from .tracer_adv import FctPlan
     meshes = meshes or MeshCache()
     lat, setup, results = {}, {}, {}
     plans = {}
+    adjoint_ready = set()          # ids of plans that hold the adjoint workspace
+    bases = {}                     # (level, nl) -> base input bytes, least recent first
+    counts, timing = collections.Counter(), collections.Counter()
     t0 = time.perf_counter()
     busy = 0.0
     for spec in sorted(specs, key=lambda s: s.arrival):
@@ -593,13 +726,41 @@ def fifo_loop(specs, device, *, meshes=None, dt=2700.0, budget_bytes=200 << 30):
             time.sleep(spec.arrival - now)
         tb = time.perf_counter()
         mesh = meshes.get(spec.level, spec.nl)
-        case = request_case(mesh, spec, device, dt)
-        key = (spec.level, spec.nl, spec.K)
+        key, mkey = (spec.level, spec.nl, spec.K), (spec.level, spec.nl)
+        need = estimate_field_bytes(mesh, spec.K, spec.nsteps, spec.adjoint)
+        if mkey not in bases:
+            need += estimate_base_bytes(mesh)
         if key not in plans:
-            need = estimate_plan_bytes(mesh, spec.K, spec.adjoint)
-            while plans and sum(p.device_bytes() for p in plans.values()) + need > budget_bytes:
-                plans.pop(next(iter(plans))).close()
+            need += estimate_plan_bytes(mesh, spec.K, spec.adjoint)
+        elif spec.adjoint and id(plans[key]) not in adjoint_ready:
+            need += (estimate_plan_bytes(mesh, spec.K, True)
+                     - estimate_plan_bytes(mesh, spec.K, False))
+        held = lambda: sum(p.device_bytes() for p in plans.values()) + sum(bases.values())
+        for k in [k for k in plans if k != key]:
+            if held() + need <= budget_bytes:
+                break
+            p = plans.pop(k)
+            adjoint_ready.discard(id(p))
+            p.close()
+            counts["plans_closed"] += 1
+        for k in [k for k in bases if k != mkey]:
+            if held() + need <= budget_bytes:
+                break
+            _drop_base_case(meshes.get(*k), device, dt)
+            del bases[k]
+            counts["base_dropped"] += 1
+        torch.cuda.empty_cache()
+        tc = time.perf_counter()
+        if not _has_base_case(mesh, device, dt):
+            counts["base_built"] += 1
+        bases[mkey] = bases.pop(mkey, estimate_base_bytes(mesh))     # most recent last
+        case = request_case(mesh, spec, device, dt)
+        timing["inputs"] += time.perf_counter() - tc
+        if key not in plans:
+            tc = time.perf_counter()
             plans[key] = FctPlan(mesh, device, max_tracers=spec.K)
+            timing["plan_create"] += time.perf_counter() - tc
+            counts["plans_built"] += 1
         plans[key] = plans.pop(key)          # most recently used last
         inst = Instance.from_case(mesh, case, device, plan=plans[key])
         torch.cuda.synchronize(device)
@@ -617,6 +778,7 @@ def fifo_loop(specs, device, *, meshes=None, dt=2700.0, budget_bytes=200 << 30):
         res = {"T": [x.cpu().numpy() for x in inst.T()]}
         if spec.adjoint:
             res.update(adjoint_of_rollout(a, dt))
+            adjoint_ready.add(id(plans[key]))
         torch.cuda.synchronize(device)
         results[spec.rid] = res
         lat[spec.rid] = time.perf_counter() - t0 - spec.arrival
@@ -626,6 +788,8 @@ def fifo_loop(specs, device, *, meshes=None, dt=2700.0, budget_bytes=200 << 30):
     for p in plans.values():
         p.close()
     out = summarize(specs, lat, setup, wall, busy, meshes)
+    out["timing_s"] = {k: round(v, 3) for k, v in timing.items()}
+    out["counts"] = dict(counts)
     out["results"] = results
     return out
 

@@ -91,7 +91,7 @@ def test_budget_counts_every_live_plan(torch_dev):
     srv = Server(torch_dev, budget_bytes=budget, max_active=8, meshes=meshes)
     srv.run(specs)
     assert not srv.errors and len(srv.latency) == len(specs)
-    pooled = [p for ps in srv.pool.values() for p in ps]
+    pooled = [p for ps in srv.pool.values() for p in ps] + list(srv.adj_plans.values())
     assert pooled
     assert srv.held_bytes([]) == (sum(p.device_bytes() for p in pooled)
                                   + sum(srv.base_bytes.values()))

@@ -16,8 +16,8 @@ class _FakePlan:
 
 
 class _Spec:
-    def __init__(self, level, nl):
-        self.level, self.nl = level, nl
+    def __init__(self, level, nl, adjoint=False):
+        self.level, self.nl, self.adjoint = level, nl, adjoint
 
 
 class _Active:
@@ -73,3 +73,18 @@ def test_held_bytes_sums_every_part():
     srv._finishing = [(_Active(3, 47, 13), None, None, None)]
     assert srv.held_bytes([_Active(3, 47, 17)]) == 7 + 11 + 13 + 17
     assert isinstance(srv.timing, collections.Counter)
+
+
+def test_adjoint_plan_stays_while_its_mesh_has_an_adjoint_in_flight():
+    """The per-mesh adjoint plan is idle (releasable) only when no active or finishing
+    request of that mesh asks for the adjoint."""
+    srv = _server(1000)
+    adj = _FakePlan(600)
+    srv.adj_plans[(5, 47)] = adj
+    srv.plan_bytes = {id(adj): 600}
+    a = _Active(5, 47, 100)
+    a.spec.adjoint = True
+    assert not srv._release_idle(500, [a])
+    assert not adj.closed
+    assert srv._release_idle(500, [_Active(5, 47, 100)])
+    assert adj.closed and (5, 47) not in srv.adj_plans
