@@ -13,7 +13,7 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libfesom_fct.so"
 
-FCT_ABI_VERSION = 1
+FCT_ABI_VERSION = 2
 FCT_OK = 0
 FCT_E_INVALID = -1
 FCT_E_HIP = -2
@@ -49,7 +49,8 @@ EXPORTED_SYMBOLS = ("fct_abi_version", "fct_last_error", "fct_plan_create",
                     "fct_batch_create", "fct_batch_set_tuning", "fct_batch_bind",
                     "fct_batch_launch", "fct_batch_graph_launch", "fct_batch_destroy",
                     "fct_backward", "fct_forward_upwind", "fct_impl_vert_diff",
-                    "fct_compute_w", "fct_ice_fct_solve", "fct_impl_vert_diff_static", "fct_stage", "fct_tracer_ab")
+                    "fct_compute_w", "fct_ice_fct_solve", "fct_impl_vert_diff_static", "fct_stage", "fct_tracer_ab",
+                    "fct_plan_clone")
 
 
 class FctMeshDesc(ctypes.Structure):
@@ -114,7 +115,8 @@ class FctPhaseBufs(ctypes.Structure):
 
 class FctPlanInfo(ctypes.Structure):
     _fields_ = [("nod2D", c_int32), ("elem2D", c_int32), ("edge2D", c_int32), ("nl", c_int32),
-                ("workspace_bytes", c_int64), ("device", c_int32), ("paths", c_int32)]
+                ("workspace_bytes", c_int64), ("device", c_int32), ("paths", c_int32),
+                ("table_bytes", c_int64), ("table_id", ctypes.c_uint64)]
 
 
 _LIB = None
@@ -144,6 +146,8 @@ def load(path: Path | None = None, *, require_torch: bool = True):
     lib.fct_plan_get_info.argtypes = [c_void_p, POINTER(FctPlanInfo)]
     lib.fct_plan_destroy.restype = c_int32
     lib.fct_plan_destroy.argtypes = [c_void_p]
+    lib.fct_plan_clone.restype = c_int32
+    lib.fct_plan_clone.argtypes = [c_void_p, c_int32, POINTER(c_void_p)]
     lib.fct_forward.restype = c_int32
     lib.fct_forward.argtypes = [c_void_p, POINTER(FctStepArgs), c_void_p]
     lib.fct_forward_ex.restype = c_int32

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "fct_common.h"
@@ -46,10 +47,21 @@ using namespace fct;
             return set_error(FCT_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
 
+// Device tables of one mesh (topology, geometry, stencil tables, edge patches): built
+// by fct_plan_create, shared by the plans fct_plan_clone makes from it, freed with the
+// last of them.
+struct TableSet {
+    int refs = 1;                 // guarded by table_mutex
+    std::vector<void*> allocs;
+    int64_t bytes = 0;
+};
+static std::mutex table_mutex;
+
 struct fct_plan {
     int device = -1;
     DevMesh m{};
-    std::vector<void*> allocs;
+    TableSet* tables = nullptr;
+    std::vector<void*> allocs;    // the plan's own workspace
     int64_t bytes = 0;
     // workspace
     double *vflux = nullptr, *LO = nullptr, *trxy = nullptr, *gnode = nullptr;
@@ -108,8 +120,38 @@ void free_plan(fct_plan* p) {
     if (p->device >= 0 && hipGetDevice(&cur) == hipSuccess && cur != p->device)
         (void)hipSetDevice(p->device);
     for (void* ptr : p->allocs) (void)hipFree(ptr);
+    if (p->tables) {
+        bool last;
+        {
+            std::lock_guard<std::mutex> g(table_mutex);
+            last = --p->tables->refs == 0;
+        }
+        if (last) {
+            for (void* ptr : p->tables->allocs) (void)hipFree(ptr);
+            delete p->tables;
+        }
+    }
     if (cur >= 0 && cur != p->device) (void)hipSetDevice(cur);
     delete p;
+}
+
+// The fused path's workspace of a plan (kw tracer slots); the edge passes write only
+// each edge's wet levels, so vflux and adf_h start at zero, the value the node passes
+// must read elsewhere.
+int alloc_fused_workspace(fct_plan* p) {
+    const size_t NL = (size_t)p->m.N * p->m.L, DL = (size_t)p->m.D * p->m.L;
+    int rc;
+    if ((rc = dev_alloc(p, &p->vflux, DL)) || (rc = dev_alloc(p, &p->fLO, NL * p->kw)) ||
+        (rc = dev_alloc(p, &p->fgnode, 2 * NL * p->kw)) ||
+        (rc = dev_alloc(p, &p->fadfh, DL * p->kw)) ||
+        (rc = dev_alloc(p, &p->fpf, 2 * NL * p->kw)) ||
+        (rc = dev_alloc(p, &p->fadfv, NL * p->kw)))
+        return rc;
+    if (hipMemset(p->vflux, 0, DL * sizeof(double)) != hipSuccess ||
+        hipMemset(p->fadfh, 0, DL * p->kw * sizeof(double)) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess)
+        return set_error(FCT_E_HIP, "workspace initialisation failed");
+    return FCT_OK;
 }
 
 int validate_mesh(const fct_mesh_desc* d) {
@@ -606,7 +648,6 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
         (rc = dev_upload(p, &m.zbar, d->zbar, (size_t)L)) ||
         (rc = dev_upload(p, &m.aav, h_aav.data(), NL)) ||
         (rc = dev_upload(p, &m.zzb, h_zzb.data(), (size_t)L)) ||
-        (rc = dev_alloc(p, &p->vflux, DL)) ||
         (rc = dev_upload(p, &p->ekr, h_ekr.data(), D))) {
         free_plan(p);
         return rc;
@@ -655,20 +696,50 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
         free_plan(p);
         return rc;
     }
-    if ((rc = dev_alloc(p, &p->fLO, NL * p->kw)) || (rc = dev_alloc(p, &p->fgnode, 2 * NL * p->kw)) ||
-        (rc = dev_alloc(p, &p->fadfh, DL * p->kw)) ||
-        (rc = dev_alloc(p, &p->fpf, 2 * NL * p->kw)) ||
-        (rc = dev_alloc(p, &p->fadfv, NL * p->kw))) {
+    // everything allocated so far is the mesh's tables; what follows is the plan's own
+    p->tables = new (std::nothrow) TableSet();
+    if (!p->tables) {
+        free_plan(p);
+        return set_error(FCT_E_NOMEM, "host allocation failed");
+    }
+    p->tables->allocs.swap(p->allocs);
+    p->tables->bytes = p->bytes;
+    p->bytes = 0;
+    if ((rc = alloc_fused_workspace(p))) {
         free_plan(p);
         return rc;
     }
-    // the fused edge passes write only each edge's wet levels: the rest stays zero,
-    // which is the value the node passes must read there
-    if (hipMemset(p->vflux, 0, DL * sizeof(double)) != hipSuccess ||
-        hipMemset(p->fadfh, 0, DL * p->kw * sizeof(double)) != hipSuccess ||
-        hipDeviceSynchronize() != hipSuccess) {
+    *out = p;
+    return FCT_OK;
+}
+
+int fct_plan_clone(const fct_plan* src, int32_t max_tracers, fct_plan** out) {
+    if (!out) return set_error(FCT_E_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!src || !src->tables) return set_error(FCT_E_INVALID, "NULL plan");
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != src->device)
+        return set_error(FCT_E_INVALID, "current device %d is not the plan's device %d", cur,
+                         src->device);
+    fct_plan* p = new (std::nothrow) fct_plan();
+    if (!p) return set_error(FCT_E_NOMEM, "host allocation failed");
+    p->device = src->device;
+    p->m = src->m;
+    p->ekr = src->ekr;
+    p->tab = src->tab;
+    p->tune = src->tune;
+    p->kw = std::max(1, std::min((int)FCT_KCHUNK, max_tracers > 0 ? max_tracers : 2));
+    p->host_edge_tri = src->host_edge_tri;
+    p->host_updn = src->host_updn;
+    {
+        std::lock_guard<std::mutex> g(table_mutex);
+        ++src->tables->refs;
+    }
+    p->tables = src->tables;
+    int rc = alloc_fused_workspace(p);
+    if (rc) {
         free_plan(p);
-        return set_error(FCT_E_HIP, "workspace initialisation failed");
+        return rc;
     }
     *out = p;
     return FCT_OK;
@@ -677,7 +748,9 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
 int fct_plan_get_info(const fct_plan* p, fct_plan_info* info) {
     if (!p || !info) return set_error(FCT_E_INVALID, "NULL argument");
     info->nod2D = p->m.N; info->elem2D = p->m.E; info->edge2D = p->m.D; info->nl = p->m.L;
-    info->workspace_bytes = p->bytes;
+    info->workspace_bytes = p->bytes + (p->tables ? p->tables->bytes : 0);
+    info->table_bytes = p->tables ? p->tables->bytes : 0;
+    info->table_id = (uint64_t)(uintptr_t)p->tables;
     info->device = p->device;
     info->paths = FCT_PATH_BIT_PHASED | (p->tab.dm > 0 ? FCT_PATH_BIT_FUSED : 0);
     return FCT_OK;

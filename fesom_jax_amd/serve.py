@@ -76,21 +76,31 @@ class MeshCache:
         return self.meshes[key]
 
 
-def estimate_plan_bytes(mesh, K, adjoint):
-    """Device bytes of a plan for ``mesh`` with ``K`` tracers: topology, the fused
-    workspace and, once it has run an adjoint, the phased and adjoint workspace (an
-    upper estimate; the server replaces it by the plan's own count once the plan
-    exists)."""
+def estimate_table_bytes(mesh):
+    """Device bytes of a mesh's tables (topology, geometry, stencil tables; shared by
+    every plan of the mesh the server clones from one another), an upper estimate."""
+    N, E, D, L = mesh.nod2D, mesh.elem2D, mesh.edge2D, mesh.nl
+    return int(300 * N + 200 * E + 160 * D + 8 * 4 * N * L)   # + area, areasvol, pairs
+
+
+def estimate_work_bytes(mesh, K, adjoint):
+    """Device bytes of a plan's own workspace: the fused path's for ``K`` tracers and,
+    once it has run an adjoint, the phased and adjoint workspace (an upper estimate;
+    the server replaces it by the plan's own count once the plan exists)."""
     N, E, D, L = mesh.nod2D, mesh.elem2D, mesh.edge2D, mesh.nl
     NL, EL, DL = N * L, E * L, D * L
-    topo = 200 * N + 200 * E + 150 * D + 8 * 4 * NL     # + area, areasvol and their pairs
-    work = 8 * (DL + K * (6 * NL + DL))
-    total = topo + work
+    total = 8 * (DL + K * (6 * NL + DL))
     if adjoint:
         # phased tape, cotangent scratch, element -> edge lists
         total += 8 * (12 * NL + 2 * EL + DL) + 8 * (20 * NL + 10 * DL + 2 * EL)
         total += 16 * D + 8 * E + 64
     return int(total)
+
+
+def estimate_plan_bytes(mesh, K, adjoint):
+    """Device bytes of a stand-alone plan for ``mesh`` with ``K`` tracers: its tables
+    and its workspace."""
+    return estimate_table_bytes(mesh) + estimate_work_bytes(mesh, K, adjoint)
 
 
 def estimate_field_bytes(mesh, K, nsteps, adjoint):
@@ -174,7 +184,9 @@ class Server:
         self.meshes = meshes or MeshCache()
         self.dt = dt
         self.pool = collections.defaultdict(list)   # (level, nl, K) -> idle plans
-        self.plan_bytes = {}                        # id(plan) -> device bytes it holds
+        self.plan_bytes = {}                        # id(plan) -> its own workspace bytes
+        self.table_sets = {}                        # table id -> [bytes, ids of live plans]
+        self.plan_table = {}                        # id(plan) -> its table id
         self.adj_plans = {}                         # (level, nl) -> the mesh's adjoint plan
         self.adj_ready = set()                      # ids of adjoint plans with workspace
         self.base_bytes = {}                        # (level, nl) -> base input bytes
@@ -192,11 +204,24 @@ class Server:
 
     # -- memory accounting ---------------------------------------------------------
     def held_bytes(self, active):
-        """Bytes the server holds: the fields of the active and the finishing requests
-        (with the adjoint workspace an adjoint request will add to its plan), the
-        meshes' base inputs and every live plan."""
+        """Bytes the server holds: the fields of the active and the finishing requests,
+        the meshes' base inputs, every live plan's workspace and each mesh's tables
+        once (its plans share them)."""
         return (sum(a.bytes for a in active) + sum(f[0].bytes for f in self._finishing)
-                + sum(self.plan_bytes.values()) + sum(self.base_bytes.values()))
+                + sum(self.plan_bytes.values()) + sum(b for b, _ in self.table_sets.values())
+                + sum(self.base_bytes.values()))
+
+    def _live_plan(self, level, nl):
+        """A live plan of mesh (level, nl) held by the server, or None."""
+        for (lv, n, _k), ps in self.pool.items():
+            if lv == level and n == nl and ps:
+                return ps[-1]
+        if (level, nl) in self.adj_plans:
+            return self.adj_plans[(level, nl)]
+        for a in list(self.active) + [f[0] for f in self._finishing]:
+            if a.spec.level == level and a.spec.nl == nl:
+                return a.inst.plan
+        return None
 
     def _need(self, s):
         """Bytes an admission of ``s`` adds: its fields, its mesh's base inputs the first
@@ -208,9 +233,11 @@ class Server:
         if (s.level, s.nl) not in self.base_bytes:
             need += estimate_base_bytes(m)
         if s.adjoint and (s.level, s.nl) not in self.adj_plans:
-            need += estimate_plan_bytes(m, 1, True)
+            need += estimate_work_bytes(m, 1, True)
         if not self.pool[key]:
-            need += estimate_plan_bytes(m, s.K, False)
+            need += estimate_work_bytes(m, s.K, False)
+        if self._live_plan(s.level, s.nl) is None:
+            need += estimate_table_bytes(m)
         return need
 
     def _note_peak(self, active):
@@ -228,7 +255,19 @@ class Server:
                 "all_plans": sum(self.plan_bytes.values()), "base": dict(self.base_bytes)}
 
     def _refresh(self, plan):
-        self.plan_bytes[id(plan)] = plan.device_bytes()
+        self.plan_bytes[id(plan)] = plan.own_bytes()
+        tid, tbytes = plan.tables()
+        self.plan_table[id(plan)] = tid
+        self.table_sets.setdefault(tid, [tbytes, set()])[1].add(id(plan))
+
+    def _forget(self, plan):
+        """Drop a closed plan from the accounting (its tables with their last plan)."""
+        del self.plan_bytes[id(plan)]
+        tid = self.plan_table.pop(id(plan))
+        users = self.table_sets[tid][1]
+        users.discard(id(plan))
+        if not users:
+            del self.table_sets[tid]
 
     def _release_idle(self, need, active, busy=(), keep=None, keep_base=None):
         """Close idle plans (largest first, never ``keep``: the plan the admission that
@@ -253,7 +292,7 @@ class Server:
                 self.adj_ready.discard(id(plan))
             else:
                 self.pool[key].remove(plan)
-            del self.plan_bytes[id(plan)]
+            self._forget(plan)
             plan.close()
             self.counts["plans_closed"] += 1
         in_use = {(a.spec.level, a.spec.nl) for a in list(active) + list(busy)}
@@ -280,10 +319,11 @@ class Server:
         # reach
         torch.cuda.empty_cache()
         t = time.perf_counter()
-        plan = FctPlan(mesh, self.device, max_tracers=spec.K)
+        share = self._live_plan(spec.level, spec.nl)
+        plan = FctPlan(mesh, self.device, max_tracers=spec.K, share=share)
         self._refresh(plan)
-        self.timing["plan_create"] += time.perf_counter() - t
-        self.counts["plans_built"] += 1
+        self.timing["plan_create" if share is None else "plan_clone"] += time.perf_counter() - t
+        self.counts["plans_built" if share is None else "plans_cloned"] += 1
         return plan
 
     def _adjoint_plan(self, mesh, spec):
@@ -296,10 +336,13 @@ class Server:
         if key not in self.adj_plans:
             torch.cuda.empty_cache()
             t = time.perf_counter()
-            plan = FctPlan(mesh, self.device, max_tracers=1)
+            plan = FctPlan(mesh, self.device, max_tracers=1,
+                           share=self._live_plan(spec.level, spec.nl))
             self.adj_plans[key] = plan
-            self.plan_bytes[id(plan)] = estimate_plan_bytes(mesh, 1, True)
-            self.timing["plan_create"] += time.perf_counter() - t
+            self._refresh(plan)
+            # the adjoint workspace is allocated by the first backward: count it now
+            self.plan_bytes[id(plan)] = estimate_work_bytes(mesh, 1, True)
+            self.timing["plan_clone"] += time.perf_counter() - t
             self.counts["adjoint_plans_built"] += 1
         return self.adj_plans[key]
 
@@ -610,6 +653,8 @@ class Server:
         self.adj_plans.clear()
         self.adj_ready.clear()
         self.plan_bytes.clear()
+        self.table_sets.clear()
+        self.plan_table.clear()
 
 
 _BASE = {}

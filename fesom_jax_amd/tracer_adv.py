@@ -52,7 +52,9 @@ def _torch():
 class FctPlan:
     """Device-resident topology + workspace for one mesh on one device."""
 
-    def __init__(self, mesh, device=None, max_tracers: int = 4):
+    def __init__(self, mesh, device=None, max_tracers: int = 4, *, share=None):
+        """``share``: a live plan of the same mesh and device whose device tables the
+        new plan shares (fct_plan_clone); the new plan then only allocates workspace."""
         torch = _torch()
         self.lib = _lib.load()
         dev = torch.device("cuda", torch.cuda.current_device() if device is None
@@ -62,11 +64,19 @@ class FctPlan:
         self.elem2D = int(mesh.elem2D)
         self.edge2D = int(mesh.edge2D)
         self.nl = int(mesh.nl)
-        desc, _host = _lib.mesh_desc_from(mesh, max_tracers)
         ptr = c_void_p()
-        with torch.cuda.device(dev):
-            _lib.check(self.lib.fct_plan_create(ctypes.byref(desc), ctypes.byref(ptr)),
-                       "fct_plan_create")
+        if share is not None:
+            if (share.device != dev or share.nod2D != self.nod2D or share.nl != self.nl
+                    or share.edge2D != self.edge2D or share.elem2D != self.elem2D):
+                raise ValueError("share must be a plan of the same mesh and device")
+            with torch.cuda.device(dev):
+                _lib.check(self.lib.fct_plan_clone(share.ptr, int(max_tracers),
+                                                   ctypes.byref(ptr)), "fct_plan_clone")
+        else:
+            desc, _host = _lib.mesh_desc_from(mesh, max_tracers)
+            with torch.cuda.device(dev):
+                _lib.check(self.lib.fct_plan_create(ctypes.byref(desc), ctypes.byref(ptr)),
+                           "fct_plan_create")
         self.ptr = ptr
         self._finalizer = weakref.finalize(self, self.lib.fct_plan_destroy, ptr)
         info = _lib.FctPlanInfo()
@@ -90,12 +100,26 @@ class FctPlan:
         self._event.record(stream)
         self._last = stream.cuda_stream
 
-    def device_bytes(self) -> int:
-        """Device memory the plan holds now (tables and workspace, including what the
-        phased, adjoint, diffusion and sea-ice entry points allocated on first use)."""
+    def _info(self):
         info = _lib.FctPlanInfo()
         _lib.check(self.lib.fct_plan_get_info(self.ptr, ctypes.byref(info)), "fct_plan_get_info")
-        return int(info.workspace_bytes)
+        return info
+
+    def device_bytes(self) -> int:
+        """Device memory the plan keeps alive now (its mesh tables, shared with the
+        plans cloned from it, and its own workspace, including what the phased,
+        adjoint, diffusion and sea-ice entry points allocated on first use)."""
+        return int(self._info().workspace_bytes)
+
+    def own_bytes(self) -> int:
+        """The plan's own workspace (device_bytes without the shared mesh tables)."""
+        info = self._info()
+        return int(info.workspace_bytes - info.table_bytes)
+
+    def tables(self):
+        """``(table_id, table_bytes)``: plans with the same id share their tables."""
+        info = self._info()
+        return int(info.table_id), int(info.table_bytes)
 
     def close(self):
         self._finalizer()

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define FCT_ABI_VERSION 1
+#define FCT_ABI_VERSION 2
 
 #define FCT_OK 0
 #define FCT_E_INVALID (-1)      /* bad argument: null pointer, size, range         */
@@ -136,9 +136,12 @@ enum {
 
 typedef struct fct_plan_info {
     int32_t nod2D, elem2D, edge2D, nl;
-    int64_t workspace_bytes;   /* device bytes owned by the plan */
+    int64_t workspace_bytes;   /* device bytes the plan keeps alive: its own workspace and
+                                  its mesh tables (shared with its clones) */
     int32_t device;            /* HIP device ordinal the plan lives on */
     int32_t paths;             /* FCT_PATH_BIT_* of the paths this mesh can take */
+    int64_t table_bytes;       /* the mesh tables' share of workspace_bytes */
+    uint64_t table_id;         /* equal for plans that share their mesh tables */
 } fct_plan_info;
 
 #define FCT_PATH_BIT_PHASED 1
@@ -159,6 +162,13 @@ const char* fct_last_error(void);
 int fct_plan_create(const fct_mesh_desc* mesh, fct_plan** out);
 
 int fct_plan_get_info(const fct_plan* plan, fct_plan_info* info);
+
+/* A plan for plan's mesh that shares its device tables (reference counted: they are
+ * freed with the last plan that uses them) and has its own workspace for max_tracers
+ * tracer slots: the cost of a new plan of a known mesh is its workspace only. Results
+ * are those of fct_plan_create for the same mesh. No reference counterpart (the
+ * reference rebuilds its device arrays per jitted function). */
+int fct_plan_clone(const fct_plan* plan, int32_t max_tracers, fct_plan** out);
 
 int fct_plan_destroy(fct_plan* plan);
 

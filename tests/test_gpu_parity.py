@@ -391,3 +391,36 @@ def test_two_streams_share_a_plan(icosa4_mesh, icosa4_case, torch_dev):
         for x, y in zip(oa, refA):
             assert torch.equal(x[0], y)
         assert torch.equal(ob[0][0], refB[0])
+
+
+def test_cloned_plan_shares_tables(icosa4_mesh, icosa4_case, torch_dev):
+    """fct_plan_clone: the clone shares the mesh tables (same table id, its own bytes
+    are workspace only), gives the original's bits, and keeps working after the
+    original is destroyed (the tables go with the last plan)."""
+    import torch
+    from fesom_jax_amd.tracer_adv import FctPlan, advect_fct
+    m, c = icosa4_mesh, icosa4_case
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(torch_dev)
+    flow = [t(c[k]) for k in ("uv", "w_e", "helem", "hnode", "hnode_new")]
+    Ts, Tos = [t(x) for x in c["T"]], [t(x) for x in c["T_old"]]
+    base = FctPlan(m, torch_dev, max_tracers=4)
+    clone = FctPlan(m, torch_dev, max_tracers=2, share=base)
+    assert clone.tables() == base.tables() and clone.tables()[1] > 0
+    assert clone.own_bytes() + clone.tables()[1] == clone.device_bytes()
+    assert clone.own_bytes() < base.own_bytes()      # 2 tracer slots against 4
+    ref = [o[0].clone() for o in advect_fct(m, *flow, Ts, Tos, dt=c["dt"], plan=base)]
+    got = [o[0].clone() for o in advect_fct(m, *flow, Ts, Tos, dt=c["dt"], plan=clone)]
+    torch.cuda.synchronize()
+    base.close()
+    again = [o[0] for o in advect_fct(m, *flow, Ts, Tos, dt=c["dt"], plan=clone)]
+    torch.cuda.synchronize()
+    for r, g, a in zip(ref, got, again):
+        assert torch.equal(r, g) and torch.equal(r, a)
+    with pytest.raises(ValueError):
+        FctPlan(icosa4_mesh, torch_dev, share=FctPlan(_pi_like(), torch_dev))
+    clone.close()
+
+
+def _pi_like():
+    from fesom_jax_amd.meshgen import icosahedral_mesh
+    return icosahedral_mesh(2, 24, seed=3)

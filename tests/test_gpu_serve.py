@@ -81,7 +81,8 @@ def test_failures_stay_with_their_request(torch_dev):
 
 def test_budget_counts_every_live_plan(torch_dev):
     """The held bytes are the pooled plans' own counts (fct_plan_get_info), adjoint
-    workspace included, and never pass the budget while every request fits it."""
+    workspace included, with each mesh's shared tables counted once, and never pass
+    the budget while every request fits it."""
     from fesom_jax_amd.serve import MeshCache, Server, estimate_bytes, make_stream
     meshes = MeshCache(seed=13)
     specs = make_stream(200.0, 0.06, levels=(3, 4), nls=(47,), steps=(2, 4), p_adjoint=0.5,
@@ -93,8 +94,15 @@ def test_budget_counts_every_live_plan(torch_dev):
     assert not srv.errors and len(srv.latency) == len(specs)
     pooled = [p for ps in srv.pool.values() for p in ps] + list(srv.adj_plans.values())
     assert pooled
-    assert srv.held_bytes([]) == (sum(p.device_bytes() for p in pooled)
+    tables = dict(p.tables() for p in pooled)      # each mesh's tables once
+    assert srv.held_bytes([]) == (sum(p.own_bytes() for p in pooled) + sum(tables.values())
                                   + sum(srv.base_bytes.values()))
+    # plans of one mesh share one table set
+    per_mesh = {}
+    for key, ps in srv.pool.items():
+        for p in ps:
+            per_mesh.setdefault(key[:2], set()).add(p.tables()[0])
+    assert all(len(ids) == 1 for ids in per_mesh.values())
     assert any(s.adjoint for s in specs)
     assert srv.peak_bytes <= budget
     srv.close()

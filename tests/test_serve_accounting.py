@@ -7,9 +7,16 @@ from fesom_jax_amd import serve
 
 
 class _FakePlan:
-    def __init__(self, nbytes):
+    def __init__(self, nbytes, table=None):
         self.nbytes = nbytes
+        self.table = table or (id(self), 0)     # (table id, table bytes)
         self.closed = False
+
+    def own_bytes(self):
+        return self.nbytes
+
+    def tables(self):
+        return self.table
 
     def close(self):
         self.closed = True
@@ -39,7 +46,8 @@ def test_release_keeps_the_plan_the_admission_reuses(monkeypatch):
     reused, other = _FakePlan(400), _FakePlan(300)
     srv.pool[(4, 47, 2)].append(reused)
     srv.pool[(5, 47, 2)].append(other)
-    srv.plan_bytes = {id(reused): 400, id(other): 300}
+    srv._refresh(reused)
+    srv._refresh(other)
     ok = srv._release_idle(500, [], keep=reused)
     assert other.closed and not reused.closed
     assert ok            # 400 held + 500 needed fit 1000
@@ -47,7 +55,8 @@ def test_release_keeps_the_plan_the_admission_reuses(monkeypatch):
     a, b = _FakePlan(400), _FakePlan(300)
     srv2.pool[(4, 47, 2)].append(a)
     srv2.pool[(5, 47, 2)].append(b)
-    srv2.plan_bytes = {id(a): 400, id(b): 300}
+    srv2._refresh(a)
+    srv2._refresh(b)
     assert not srv2._release_idle(500, [], keep=a)    # only `a` could make room
     assert not a.closed
 
@@ -65,10 +74,26 @@ def test_release_keeps_the_admissions_base_inputs(monkeypatch):
     assert dropped == [(9, 70)] and ok
 
 
+def test_shared_tables_count_once_and_go_with_their_last_plan():
+    srv = _server(1 << 40)
+    a, b = _FakePlan(10, (77, 1000)), _FakePlan(20, (77, 1000))
+    srv.pool[(6, 70, 2)].append(a)
+    srv.pool[(6, 70, 3)].append(b)
+    srv._refresh(a)
+    srv._refresh(b)
+    assert srv.held_bytes([]) == 10 + 20 + 1000
+    srv.budget = 1000 + 20 + 5
+    assert srv._release_idle(5, [], keep=b)        # closing `a` frees its 10 bytes only
+    assert a.closed and srv.held_bytes([]) == 20 + 1000
+    srv.budget = 10
+    srv._release_idle(5, [])
+    assert b.closed and srv.held_bytes([]) == 0 and not srv.table_sets
+
+
 def test_held_bytes_sums_every_part():
     srv = _server(1 << 40)
     p = _FakePlan(7)
-    srv.plan_bytes = {id(p): 7}
+    srv._refresh(p)
     srv.base_bytes = {(3, 47): 11}
     srv._finishing = [(_Active(3, 47, 13), None, None, None)]
     assert srv.held_bytes([_Active(3, 47, 17)]) == 7 + 11 + 13 + 17
@@ -81,7 +106,7 @@ def test_adjoint_plan_stays_while_its_mesh_has_an_adjoint_in_flight():
     srv = _server(1000)
     adj = _FakePlan(600)
     srv.adj_plans[(5, 47)] = adj
-    srv.plan_bytes = {id(adj): 600}
+    srv._refresh(adj)
     a = _Active(5, 47, 100)
     a.spec.adjoint = True
     assert not srv._release_idle(500, [a])
