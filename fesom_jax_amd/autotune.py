@@ -1,11 +1,16 @@
 """Autotuner of the fused path's launch shapes.
 
-Knobs (``FCT_TUNE_*`` in ``include/fesom_fct.h``): the XCD remap of workgroups, the
-wavefronts per workgroup, and the edge tiles of the two edge passes (edges per
-wavefront). Lanes always map to levels (the level block is one 64-lane wavefront per
-64 levels). Results never depend on the knobs; only time does. The tuner times every
-configuration of the space on the caller's own workload with HIP events on the launch
-stream and keeps the fastest. No reference counterpart (XLA picks its own shapes).
+Knobs (``FCT_TUNE_*`` in ``include/fesom_fct.h``), one axis each: the XCD remap of
+workgroups and its range length (which XCD's L2 sees which part of the mesh), the
+wavefronts per workgroup, the edge tiles of the two edge passes (edges per wavefront),
+and, for the adf_h pass of a single plan, the level block of its edge-patch kernel
+(levels staged per chunk; 0 = one wavefront per edge column) and the threads per edge.
+The node passes' level block is fixed by the hardware: one 64-lane wavefront per 64
+levels. Results never depend on the knobs; only time does. The tuner times
+configurations on the caller's own workload with HIP events on the launch stream:
+every one of a small space, or (``search="descent"``) one axis at a time from the
+library's defaults, keeping each axis's fastest value, until a sweep changes nothing.
+No reference counterpart (XLA picks its own shapes).
 """
 
 from __future__ import annotations
@@ -18,14 +23,23 @@ from . import _lib
 
 SPACE = {
     _lib.FCT_TUNE_XCD: (1, 8),
+    _lib.FCT_TUNE_XCD_CHUNK: (0, 8, 32),
     _lib.FCT_TUNE_WAVES_PER_GROUP: (1, 2, 4),
     _lib.FCT_TUNE_VFLUX_EDGES: (2, 4),
     _lib.FCT_TUNE_ADFH_EDGES: (1, 2),
+    _lib.FCT_TUNE_ADFH_CHUNK: (0, 4, 8),
+    _lib.FCT_TUNE_ADFH_THREADS: (1, 2),
 }
-DEFAULT = {_lib.FCT_TUNE_XCD: 1, _lib.FCT_TUNE_WAVES_PER_GROUP: 4,
-           _lib.FCT_TUNE_VFLUX_EDGES: 4, _lib.FCT_TUNE_ADFH_EDGES: 2}
-NAMES = {_lib.FCT_TUNE_XCD: "xcd", _lib.FCT_TUNE_WAVES_PER_GROUP: "waves_per_group",
-         _lib.FCT_TUNE_VFLUX_EDGES: "vflux_edges", _lib.FCT_TUNE_ADFH_EDGES: "adfh_edges"}
+# the axes a batch uses (its adf_h pass runs one wavefront per edge column)
+BATCH_SPACE = {k: v for k, v in SPACE.items()
+               if k not in (_lib.FCT_TUNE_ADFH_CHUNK, _lib.FCT_TUNE_ADFH_THREADS)}
+DEFAULT = {_lib.FCT_TUNE_XCD: 8, _lib.FCT_TUNE_XCD_CHUNK: 8, _lib.FCT_TUNE_WAVES_PER_GROUP: 4,
+           _lib.FCT_TUNE_VFLUX_EDGES: 4, _lib.FCT_TUNE_ADFH_EDGES: 2,
+           _lib.FCT_TUNE_ADFH_CHUNK: 8, _lib.FCT_TUNE_ADFH_THREADS: 1}
+NAMES = {_lib.FCT_TUNE_XCD: "xcd", _lib.FCT_TUNE_XCD_CHUNK: "xcd_chunk",
+         _lib.FCT_TUNE_WAVES_PER_GROUP: "waves_per_group",
+         _lib.FCT_TUNE_VFLUX_EDGES: "vflux_edges", _lib.FCT_TUNE_ADFH_EDGES: "adfh_edges",
+         _lib.FCT_TUNE_ADFH_CHUNK: "adfh_level_chunk", _lib.FCT_TUNE_ADFH_THREADS: "adfh_threads"}
 
 _CACHE: dict = {}
 
@@ -45,12 +59,16 @@ def time_ms(run, stream, *, reps=3, warmup=1) -> float:
     return e0.elapsed_time(e1) / reps
 
 
-def tune(set_knob, run, stream, *, space=None, reps=3, warmup=1, key=None, cache_path=None):
-    """Time every configuration of ``space`` (default ``SPACE``) and leave the fastest
+def tune(set_knob, run, stream, *, space=None, reps=3, warmup=1, key=None, cache_path=None,
+         search="grid", max_sweeps=3):
+    """Find the fastest configuration of ``space`` (default ``SPACE``) and leave it
     applied. ``set_knob(knob, value)`` applies one knob; ``run()`` launches one step on
-    ``stream``. Returns ``(best, table)`` with ``best`` a {knob: value} dict and
-    ``table`` a list of ({knob: value}, ms). A ``key`` reuses an earlier result from
-    this process (or from the JSON file ``cache_path``)."""
+    ``stream``. ``search="grid"`` times every configuration; ``"descent"`` starts from
+    the library's defaults and times one axis at a time, keeping each axis's fastest
+    value, for at most ``max_sweeps`` sweeps or until a sweep changes nothing. Returns
+    ``(best, table)`` with ``best`` a {knob: value} dict and ``table`` a list of
+    ({knob: value}, ms). A ``key`` reuses an earlier result from this process (or from
+    the JSON file ``cache_path``)."""
     space = SPACE if space is None else space
     if key is not None:
         hit = _lookup(key, cache_path)
@@ -60,12 +78,36 @@ def tune(set_knob, run, stream, *, space=None, reps=3, warmup=1, key=None, cache
             return hit, []
     knobs = sorted(space)
     table = []
-    for values in itertools.product(*(space[k] for k in knobs)):
-        cfg = dict(zip(knobs, values))
+
+    def timed(cfg):
         for k, v in cfg.items():
             set_knob(k, v)
-        table.append((cfg, time_ms(run, stream, reps=reps, warmup=warmup)))
-    best = min(table, key=lambda r: r[1])[0]
+        ms = time_ms(run, stream, reps=reps, warmup=warmup)
+        table.append((dict(cfg), ms))
+        return ms
+
+    if search == "grid":
+        for values in itertools.product(*(space[k] for k in knobs)):
+            timed(dict(zip(knobs, values)))
+        best = min(table, key=lambda r: r[1])[0]
+    elif search == "descent":
+        best = {k: DEFAULT.get(k, space[k][0]) for k in knobs}
+        best_ms = timed(best)
+        for _ in range(max_sweeps):
+            changed = False
+            for k in knobs:
+                for v in space[k]:
+                    if v == best[k]:
+                        continue
+                    cfg = dict(best)
+                    cfg[k] = v
+                    ms = timed(cfg)
+                    if ms < best_ms:
+                        best, best_ms, changed = cfg, ms, True
+            if not changed:
+                break
+    else:
+        raise ValueError(f"unknown search {search!r}")
     for k, v in best.items():
         set_knob(k, v)
     if key is not None:
@@ -89,7 +131,8 @@ def tune_plan(plan, args, stream, **kw):
 
 
 def tune_batch(batch, stream, *, graph=True, **kw):
-    """Tune a batch (the timed runs advance its instances)."""
+    """Tune a batch (the timed runs advance its instances) over ``BATCH_SPACE``."""
+    kw.setdefault("space", BATCH_SPACE)
     return tune(batch.set_tuning, lambda: batch.step(graph=graph, stream=stream), stream, **kw)
 
 

@@ -206,9 +206,9 @@ int launch_backward(const DevMesh& m, const AdjointWork& w, const fct_step_args*
 // Launch-shape knobs of the fused path (fct_plan_set_tuning / fct_batch_set_tuning);
 // results do not depend on them.
 struct FusedTuning {
-    int xcd = 1;        // remap blocks so each of `xcd` XCDs sweeps its own tile ranges
+    int xcd = 8;        // remap blocks so each of `xcd` XCDs sweeps its own tile ranges
                         // (1 = hardware order)
-    int xchunk = 0;     // tiles per contiguous range of one XCD (0 = one range per XCD)
+    int xchunk = 8;     // tiles per contiguous range of one XCD (0 = one range per XCD)
     int wpb = 4;        // wavefronts per workgroup (1, 2 or 4)
     int vflux_epw = 4;  // edges per wavefront in kf_vflux (2 or 4)
     int adfh_epw = 2;   // edges per wavefront in kf_adfh (1 or 2)

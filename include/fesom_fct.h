@@ -111,12 +111,12 @@ enum {
 
 enum {
     FCT_TUNE_XCD = 1,               /* blocks remapped so each of N XCDs sweeps its own
-                                       ranges of nodes (1 = hardware order); default 1    */
+                                       ranges of nodes (1 = hardware order); default 8    */
     FCT_TUNE_WAVES_PER_GROUP = 2,   /* wavefronts per workgroup: 1, 2 or 4 (default 4)     */
     FCT_TUNE_VFLUX_EDGES = 3,       /* edges per wavefront in the vflux pass: 2 or 4 (4)   */
     FCT_TUNE_ADFH_EDGES = 4,        /* edges per wavefront in the adf_h pass: 1 or 2 (2)   */
     FCT_TUNE_XCD_CHUNK = 5,         /* workgroups per range of one XCD under FCT_TUNE_XCD:
-                                       0 = one range per XCD, else 1..2^20 (default 0)    */
+                                       0 = one range per XCD, else 1..2^20 (default 8)    */
     FCT_TUNE_W_EDGE_FLUX = 6,       /* fct_compute_w: 1 = edge terms once per edge (the
                                        step's edge flux pass), 0 = per node from the
                                        triangles; same bits either way (default 1)        */

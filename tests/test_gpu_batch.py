@@ -142,8 +142,46 @@ def test_autotuner_picks_a_valid_config(torch_dev):
     batch = FctBatch(insts)
     s = torch.cuda.current_stream(torch_dev)
     best, table = autotune.tune_batch(batch, s, reps=1, warmup=1, key="test-batch")
-    assert len(table) == 24 and all(ms > 0 for _, ms in table)
-    assert set(best) == set(autotune.SPACE)
+    n = 1
+    for v in autotune.BATCH_SPACE.values():
+        n *= len(v)
+    assert len(table) == n and all(ms > 0 for _, ms in table)
+    assert set(best) == set(autotune.BATCH_SPACE)
     again, table2 = autotune.tune_batch(batch, s, key="test-batch")
     assert again == best and table2 == []
     batch.close()
+
+
+def test_autotuner_descent_on_a_plan(icosa4_mesh, icosa4_case, torch_dev):
+    """Axis-by-axis descent over every axis (edge tiles, the adf_h level block, XCD
+    ranges) on a single plan: the picked shape gives the default shape's bits."""
+    import ctypes
+    import torch
+    from ctypes import c_void_p
+    from fesom_jax_amd import _lib, autotune
+    from fesom_jax_amd.tracer_adv import FctPlan
+    m, c = icosa4_mesh, icosa4_case
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(torch_dev)
+    plan = FctPlan(m, torch_dev, max_tracers=2)
+    T = [t(x) for x in c["T"][:2]]
+    To = [t(x) for x in c["T_old"][:2]]
+    out = [torch.empty_like(x) for x in T]
+    arr = lambda xs: (c_void_p * len(xs))(*[x.data_ptr() for x in xs])
+    keep = (arr(T), arr(To), arr(out))
+    P = lambda a: ctypes.cast(a, ctypes.POINTER(c_void_p))
+    f = [t(c[k]) for k in ("uv", "w_e", "helem", "hnode", "hnode_new")]
+    args = _lib.FctStepArgs(ntracers=2, dt=c["dt"], uv=f[0].data_ptr(), w_e=f[1].data_ptr(),
+                            helem=f[2].data_ptr(), hnode=f[3].data_ptr(),
+                            hnode_new=f[4].data_ptr(), T=P(keep[0]), T_old=P(keep[1]),
+                            T_new=P(keep[2]), w_i=None, Z3d=None, zbar3=None)
+    s = torch.cuda.current_stream(torch_dev)
+    sp = c_void_p(s.cuda_stream)
+    _lib.check(plan.lib.fct_forward(plan.ptr, ctypes.byref(args), sp), "fct_forward")
+    ref = [x.clone() for x in out]
+    best, table = autotune.tune_plan(plan, args, s, reps=1, warmup=1, search="descent")
+    assert set(best) == set(autotune.SPACE) and len(table) >= 1 + sum(
+        len(v) - 1 for v in autotune.SPACE.values())
+    _lib.check(plan.lib.fct_forward(plan.ptr, ctypes.byref(args), sp), "fct_forward")
+    torch.cuda.synchronize()
+    assert all(torch.equal(x, y) for x, y in zip(out, ref))
+    plan.close()
