@@ -401,7 +401,9 @@ def run_batched(args, dev):
             inst.step_alone(stream)
 
     batch = FctBatch(insts)
-    best, table = autotune.tune_batch(batch, stream, reps=2, warmup=1)
+    best, table = autotune.tune_batch(
+        batch, stream, reps=2, warmup=1, search="descent",
+        progress=lambda cfg, ms: log(f"tune {autotune.describe(cfg)}: {ms:.2f} ms"))
     log(f"autotuner: {autotune.describe(best)} "
         f"({min(ms for _, ms in table):.3f} .. {max(ms for _, ms in table):.3f} ms/step)")
     batch.prepare()

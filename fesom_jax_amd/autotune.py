@@ -60,7 +60,7 @@ def time_ms(run, stream, *, reps=3, warmup=1) -> float:
 
 
 def tune(set_knob, run, stream, *, space=None, reps=3, warmup=1, key=None, cache_path=None,
-         search="grid", max_sweeps=3):
+         search="grid", max_sweeps=3, progress=None):
     """Find the fastest configuration of ``space`` (default ``SPACE``) and leave it
     applied. ``set_knob(knob, value)`` applies one knob; ``run()`` launches one step on
     ``stream``. ``search="grid"`` times every configuration; ``"descent"`` starts from
@@ -68,7 +68,7 @@ def tune(set_knob, run, stream, *, space=None, reps=3, warmup=1, key=None, cache
     value, for at most ``max_sweeps`` sweeps or until a sweep changes nothing. Returns
     ``(best, table)`` with ``best`` a {knob: value} dict and ``table`` a list of
     ({knob: value}, ms). A ``key`` reuses an earlier result from this process (or from
-    the JSON file ``cache_path``)."""
+    the JSON file ``cache_path``). ``progress(cfg, ms)`` is called after each timing."""
     space = SPACE if space is None else space
     if key is not None:
         hit = _lookup(key, cache_path)
@@ -84,6 +84,8 @@ def tune(set_knob, run, stream, *, space=None, reps=3, warmup=1, key=None, cache
             set_knob(k, v)
         ms = time_ms(run, stream, reps=reps, warmup=warmup)
         table.append((dict(cfg), ms))
+        if progress is not None:
+            progress(cfg, ms)
         return ms
 
     if search == "grid":

@@ -679,6 +679,11 @@ int fct_plan_create(const fct_mesh_desc* d, fct_plan** out) {
         p->tab.nshallow = (int)h_nlist.size();
         for (int n = 0; n < N; ++n)
             if (d->nlevels_nod2D[n] - 1 >= 64) h_nlist.push_back(n);
+    } else if (dm == 6) {
+        // nl <= 64: every column is shallow; the list lets two tracers share a wavefront
+        h_nlist.resize(N);
+        for (int n = 0; n < N; ++n) h_nlist[n] = n;
+        p->tab.nshallow = N;
     }
     if (dm && D && ((rc = dev_upload(p, &p->tab.ep.ptab, loc.ptab.data(), loc.ptab.size())) ||
                     (rc = dev_upload(p, &p->tab.ep.pnodes, loc.pnodes.data(), loc.pnodes.size())) ||

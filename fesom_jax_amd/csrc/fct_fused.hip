@@ -1597,7 +1597,11 @@ static void launch_single(const FusedArgs& fa, const FusedTuning& t, hipStream_t
                 hipLaunchKernelGGL((kf_vflux<NS, 4>), dim3(blocks_for(vflux_waves(m.D, t), t.wpb)), blk, 0, s, fa);
             break;
         case 1:
-            if (NS == 2 && fa.tab.nlist) {
+            if (NS == 1 && DM == 6 && fa.K == 2 && t.pair && fa.tab.nlist) {
+                // nl <= 64: every column is shallow; both tracers per wavefront
+                hipLaunchKernelGGL((kl_pass1_all<DM, 2>), dim3(blocks_for(fa.tab.nshallow, t.wpb)),
+                                   blk, 0, s, fa, fa.tab.nshallow);
+            } else if (NS == 2 && fa.tab.nlist) {
                 const int ns = fa.tab.nshallow, nd = m.N - fa.tab.nshallow;
                 // two tracers (temperature and salinity) share the tracer-independent
                 // loads of a shallow node in one wavefront
@@ -1642,7 +1646,11 @@ static void launch_single(const FusedArgs& fa, const FusedTuning& t, hipStream_t
         case 3:
         case 4: {
             const NodeTab& tb = fa.tab;
-            if (NS == 2 && tb.nlist) {
+            if (NS == 1 && DM == 6 && fa.K == 2 && t.pair && tb.nlist) {
+                const dim3 g(blocks_for(tb.nshallow, t.wpb));
+                if (stage == 3) hipLaunchKernelGGL((kl_pass2_all<DM, 2>), g, blk, 0, s, fa, tb.nshallow);
+                else hipLaunchKernelGGL((kl_pass3_all<DM, 2>), g, blk, 0, s, fa, tb.nshallow);
+            } else if (NS == 2 && tb.nlist) {
                 const int ns = tb.nshallow, nd = m.N - tb.nshallow;
                 const unsigned g1 = blocks_for(node_waves(ns, fa.K), t.wpb);
                 const unsigned g2 = blocks_for(node_waves(nd, fa.K), t.wpb);

@@ -292,7 +292,7 @@ def test_launch_shapes_do_not_change_results(knobs, torch_dev):
                        "tuning")
         got = _gpu_step(m, case, torch_dev, path="fused")
     finally:
-        for k, v in (("XCD", 1), ("XCD_CHUNK", 0), ("WAVES_PER_GROUP", 4), ("VFLUX_EDGES", 4),
+        for k, v in (("XCD", 8), ("XCD_CHUNK", 8), ("WAVES_PER_GROUP", 4), ("VFLUX_EDGES", 4),
                      ("ADFH_EDGES", 2), ("PAIR_TRACERS", 1), ("ADFH_CHUNK", 8),
                      ("ADFH_THREADS", 1)):
             plan.lib.fct_plan_set_tuning(plan.ptr, getattr(_lib, "FCT_TUNE_" + k), v)
@@ -424,3 +424,28 @@ def test_cloned_plan_shares_tables(icosa4_mesh, icosa4_case, torch_dev):
 def _pi_like():
     from fesom_jax_amd.meshgen import icosahedral_mesh
     return icosahedral_mesh(2, 24, seed=3)
+
+
+def test_two_tracers_per_wavefront_nl48(torch_dev):
+    """nl <= 64 with two tracers: every column takes the two-tracer node passes; bit for
+    bit with the oracle and with one tracer per wavefront (limiter-active state)."""
+    from fesom_jax_amd import _lib
+    from fesom_jax_amd.inputs import make_case
+    from fesom_jax_amd.meshgen import icosahedral_mesh
+    from fesom_jax_amd.tracer_adv import get_plan
+    m = icosahedral_mesh(4, 48, seed=61)
+    base = make_case(m, seed=62, dt=2700.0)
+    sharp = sharp_case(m, base, 2.0)
+    # the sharp tracer (limiter active) and the smooth one of the base case
+    case = dict(sharp, T=[sharp["T"][0], base["T"][0]], T_old=[sharp["T_old"][0], base["T_old"][0]])
+    paired = _gpu_step(m, case, torch_dev, path="fused")
+    for j, g in enumerate(paired):
+        _check(f"nl48 paired tracer {j}", g, _oracle(m, case, j), m)
+    plan = get_plan(m, torch_dev)
+    try:
+        _lib.check(plan.lib.fct_plan_set_tuning(plan.ptr, _lib.FCT_TUNE_PAIR_TRACERS, 0), "tuning")
+        single = _gpu_step(m, case, torch_dev, path="fused")
+    finally:
+        plan.lib.fct_plan_set_tuning(plan.ptr, _lib.FCT_TUNE_PAIR_TRACERS, 1)
+    for a, b in zip(paired, single):
+        assert np.array_equal(a, b)
