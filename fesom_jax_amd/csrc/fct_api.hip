@@ -1141,24 +1141,38 @@ int build_slot(fct_batch* b, BatchSlot& s, const fct_step_args* args) {
         g.ns = keys[c].first;
         g.dm = keys[c].second;
         std::vector<unsigned char> desc;
-        std::vector<int2> bv, bn, ba, bns, bnd;
+        std::vector<int2> bv, bn, ba, bns, bnd, bpa, be2, be1;
         const bool split = g.ns == 2;     // every instance of such a group has node lists
         for (int i = 0; i < n; ++i) {
             if (cls[i] != (int)c) continue;
             fct_plan* p = b->plans[i];
             const int inst = g.count++;
+            const int K = args[i].ntracers;
             desc.resize(desc.size() + dsz);
             int rc = fill_fused_args(desc.data() + inst * dsz, p->m, p->tab, p->ekr, &args[i], t,
                                      p->vflux, p->fLO, p->fgnode, p->fadfh, p->fpf, p->fadfv);
             if (rc) return rc;
             block_table(bv, inst, vflux_waves(p->m.D, t), t.wpb);
-            block_table(bn, inst, node_waves(p->m.N, args[i].ntracers), t.wpb);
-            block_table(ba, inst, adfh_waves(p->m.D, args[i].ntracers, t), t.wpb);
-            if (split) {
-                if (!p->tab.nlist) return set_error(FCT_E_UNSUPPORTED, "batch: plan without node lists");
-                block_table(bns, inst, node_waves(p->tab.nshallow, args[i].ntracers), t.wpb);
-                block_table(bnd, inst, node_waves(p->m.N - p->tab.nshallow, args[i].ntracers),
-                            t.wpb);
+            if (split && !p->tab.nlist)
+                return set_error(FCT_E_UNSUPPORTED, "batch: plan without node lists");
+            // the single-instance launcher's choices, per instance
+            const bool pair = p->tab.nlist && g.dm == 6 && K == 2 && t.pair;
+            if (pair) {
+                block_table(bpa, inst, p->tab.nshallow, t.wpb);
+                if (split) block_table(bnd, inst, node_waves(p->m.N - p->tab.nshallow, K), t.wpb);
+            } else if (split) {
+                block_table(bns, inst, node_waves(p->tab.nshallow, K), t.wpb);
+                block_table(bnd, inst, node_waves(p->m.N - p->tab.nshallow, K), t.wpb);
+            } else {
+                block_table(bn, inst, node_waves(p->m.N, K), t.wpb);
+            }
+            if (t.adfh_kc && p->tab.ep.np) {
+                for (int j0 = 0; j0 < K; j0 += 2) {
+                    std::vector<int2>& be = K - j0 >= 2 ? be2 : be1;
+                    for (int q = 0; q < p->tab.ep.np; ++q) be.push_back(make_int2(inst, q | (j0 << 24)));
+                }
+            } else {
+                block_table(ba, inst, adfh_waves(p->m.D, K, t), t.wpb);
             }
         }
         std::vector<unsigned char> dcopy(desc);
@@ -1168,12 +1182,16 @@ int build_slot(fct_batch* b, BatchSlot& s, const fct_step_args* args) {
             (rc = slot_upload(s, b->capture, &g.blk_v, bv)) ||
             (rc = slot_upload(s, b->capture, &g.blk_n, bn)) ||
             (rc = slot_upload(s, b->capture, &g.blk_a, ba)) ||
+            (rc = slot_upload(s, b->capture, &g.blk_pa, bpa)) ||
+            (rc = slot_upload(s, b->capture, &g.blk_e2, be2)) ||
+            (rc = slot_upload(s, b->capture, &g.blk_e1, be1)) ||
             (split && ((rc = slot_upload(s, b->capture, &g.blk_ns, bns)) ||
                        (rc = slot_upload(s, b->capture, &g.blk_nd, bnd)))))
             return rc;
         g.desc = reinterpret_cast<const FusedArgs*>(dd);
         g.nb_v = (unsigned)bv.size(); g.nb_n = (unsigned)bn.size(); g.nb_a = (unsigned)ba.size();
         g.nb_ns = (unsigned)bns.size(); g.nb_nd = (unsigned)bnd.size();
+        g.nb_pa = (unsigned)bpa.size(); g.nb_e2 = (unsigned)be2.size(); g.nb_e1 = (unsigned)be1.size();
         s.groups.push_back(g);
     }
     s.bound = true;

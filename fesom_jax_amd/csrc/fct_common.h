@@ -237,7 +237,14 @@ struct BatchGroup {
     // and deep node lists (NodeTab::nlist)
     const int2* blk_ns = nullptr;
     const int2* blk_nd = nullptr;
-    unsigned nb_v = 0, nb_n = 0, nb_a = 0, nb_ns = 0, nb_nd = 0;
+    // two tracers per wavefront (instances with K = 2 on 6-slot meshes): waves over each
+    // such instance's shallow node list; its deep nodes (nl > 64) stay in blk_nd
+    const int2* blk_pa = nullptr;
+    // adf_h by edge patches: per workgroup (instance, patch | first tracer << 24), two
+    // tracers per thread (e2) or one (e1); instances without patches stay in blk_a
+    const int2* blk_e2 = nullptr;
+    const int2* blk_e1 = nullptr;
+    unsigned nb_v = 0, nb_n = 0, nb_a = 0, nb_ns = 0, nb_nd = 0, nb_pa = 0, nb_e2 = 0, nb_e1 = 0;
 };
 
 // Fused launcher (fct_fused.hip): kw = tracers per chunk (workspace slots).

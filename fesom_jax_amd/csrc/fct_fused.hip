@@ -1478,6 +1478,15 @@ __global__ void __launch_bounds__(FCT_EP * TPE) __attribute__((amdgpu_waves_per_
 ke_adfh(FusedArgs a, int j0) {
     adfh_patch<KT, KC, TPE>(a, (int)block_tile(a.remap), j0);
 }
+// batched form: per workgroup (instance, patch | first tracer << 24)
+template <int KT, int KC, int TPE>
+__global__ void __launch_bounds__(FCT_EP * TPE) __attribute__((amdgpu_waves_per_eu(4)))
+kbe_adfh(const FusedArgs* __restrict__ d, const int2* __restrict__ blk, int remap) {
+    const int2 e = blk[block_tile(remap)];
+    const int inst = __builtin_amdgcn_readfirstlane(e.x);
+    const int pj = __builtin_amdgcn_readfirstlane(e.y);
+    adfh_patch<KT, KC, TPE>(d[inst], pj & 0xffffff, pj >> 24);
+}
 
 #define FCT_NODE_KERNELS(PASS, OCC)                                                     \
     template <int NS, int DM>                                                           \
@@ -1565,6 +1574,40 @@ __global__ void FCT_OCC(DM == 6 ? 5 : 4) kl_pass1_all(FusedArgs a, int count) {
     if (!single_wave((int64_t)count, a.remap, w)) return;
     const int n = ldc(a.tab.nlist, (size_t)w);
     pass1_shallow_all<DM, KT>(a, n, ldc(a.tab.hdr, n));
+}
+
+// batched two-tracer passes: waves over each instance's shallow node list
+template <int DM>
+__global__ void FCT_OCC(DM == 6 ? 5 : 4) kbp_pass1(const FusedArgs* __restrict__ d,
+                                                     const int2* __restrict__ blk, int remap) {
+    int inst;
+    const int w = batch_wave(blk, remap, inst);
+    const FusedArgs& a = d[inst];
+    if (w >= a.tab.nshallow) return;
+    const int n = ldc(a.tab.nlist, (size_t)w);
+    pass1_shallow_all<DM, 2>(a, n, ldc(a.tab.hdr, n));
+}
+template <int DM>
+__global__ void FCT_OCC(DM == 6 ? 6 : 4) kbp_pass2(const FusedArgs* __restrict__ d,
+                                                     const int2* __restrict__ blk, int remap) {
+    int inst;
+    const int w = batch_wave(blk, remap, inst);
+    const FusedArgs& a = d[inst];
+    if (w >= a.tab.nshallow) return;
+    const int n = ldc(a.tab.nlist, (size_t)w);
+    const int4 hd = ldc(a.tab.hdr, n);
+    pass2_shallow_all<DM, 2>(a, n, make_int2(hd.x, hd.y), make_int2(hd.z & 0xff, hd.w));
+}
+template <int DM>
+__global__ void FCT_OCC(DM == 6 ? 7 : 4) kbp_pass3(const FusedArgs* __restrict__ d,
+                                                     const int2* __restrict__ blk, int remap) {
+    int inst;
+    const int w = batch_wave(blk, remap, inst);
+    const FusedArgs& a = d[inst];
+    if (w >= a.tab.nshallow) return;
+    const int n = ldc(a.tab.nlist, (size_t)w);
+    const int4 hd = ldc(a.tab.hdr, n);
+    pass3_shallow_all<DM, 2>(a, n, make_int2(hd.x, hd.y), make_int2(hd.z & 0xff, hd.w));
 }
 
 template <int NSB, int DM>
@@ -1813,11 +1856,31 @@ static void launch_batch_t(const BatchGroup& g, const FusedTuning& t, hipStream_
             hipLaunchKernelGGL((kb_vflux<NS, 4>), dim3(g.nb_v), blk, 0, s, g.desc, g.blk_v, t.remap());
     }
     const bool split = NS == 2 && g.blk_ns;
+    if (g.nb_pa) hipLaunchKernelGGL((kbp_pass1<DM>), dim3(g.nb_pa), blk, 0, s, g.desc, g.blk_pa, t.remap());
     if (split) {
         if (g.nb_ns) hipLaunchKernelGGL((kbl_pass1<1, DM>), dim3(g.nb_ns), blk, 0, s, g.desc, g.blk_ns, t.remap(), 0);
         if (g.nb_nd) hipLaunchKernelGGL((kbl_pass1<2, DM>), dim3(g.nb_nd), blk, 0, s, g.desc, g.blk_nd, t.remap(), 1);
     } else if (g.nb_n) {
         hipLaunchKernelGGL((kb_pass1<NS, DM>), dim3(g.nb_n), blk, 0, s, g.desc, g.blk_n, t.remap());
+    }
+    if (g.nb_e2 || g.nb_e1) {
+#define FCT_KBE(KC, TPE)                                                                            \
+    do {                                                                                            \
+        if (g.nb_e2) hipLaunchKernelGGL((kbe_adfh<2, KC, TPE>), dim3(g.nb_e2), dim3(FCT_EP * TPE), 0, s, \
+                                        g.desc, g.blk_e2, t.remap());                               \
+        if (g.nb_e1) hipLaunchKernelGGL((kbe_adfh<1, KC, TPE>), dim3(g.nb_e1), dim3(FCT_EP * TPE), 0, s, \
+                                        g.desc, g.blk_e1, t.remap());                               \
+    } while (0)
+        if (t.adfh_kc == 4) {
+            if (t.adfh_tpe == 1) FCT_KBE(4, 1);
+            else if (t.adfh_tpe == 2) FCT_KBE(4, 2);
+            else FCT_KBE(4, 4);
+        } else {
+            if (t.adfh_tpe == 1) FCT_KBE(8, 1);
+            else if (t.adfh_tpe == 2) FCT_KBE(8, 2);
+            else FCT_KBE(8, 4);
+        }
+#undef FCT_KBE
     }
     if (g.nb_a) {
         if (t.adfh_epw == 1)
@@ -1825,13 +1888,18 @@ static void launch_batch_t(const BatchGroup& g, const FusedTuning& t, hipStream_
         else
             hipLaunchKernelGGL((kb_adfh<NS, 2>), dim3(g.nb_a), blk, 0, s, g.desc, g.blk_a, t.remap());
     }
+    if (g.nb_pa) hipLaunchKernelGGL((kbp_pass2<DM>), dim3(g.nb_pa), blk, 0, s, g.desc, g.blk_pa, t.remap());
     if (split) {
         if (g.nb_ns) hipLaunchKernelGGL((kbl_pass2<1, DM>), dim3(g.nb_ns), blk, 0, s, g.desc, g.blk_ns, t.remap(), 0);
         if (g.nb_nd) hipLaunchKernelGGL((kbl_pass2<2, DM>), dim3(g.nb_nd), blk, 0, s, g.desc, g.blk_nd, t.remap(), 1);
+    } else if (g.nb_n) {
+        hipLaunchKernelGGL((kb_pass2<NS, DM>), dim3(g.nb_n), blk, 0, s, g.desc, g.blk_n, t.remap());
+    }
+    if (g.nb_pa) hipLaunchKernelGGL((kbp_pass3<DM>), dim3(g.nb_pa), blk, 0, s, g.desc, g.blk_pa, t.remap());
+    if (split) {
         if (g.nb_ns) hipLaunchKernelGGL((kbl_pass3<1, DM>), dim3(g.nb_ns), blk, 0, s, g.desc, g.blk_ns, t.remap(), 0);
         if (g.nb_nd) hipLaunchKernelGGL((kbl_pass3<2, DM>), dim3(g.nb_nd), blk, 0, s, g.desc, g.blk_nd, t.remap(), 1);
     } else if (g.nb_n) {
-        hipLaunchKernelGGL((kb_pass2<NS, DM>), dim3(g.nb_n), blk, 0, s, g.desc, g.blk_n, t.remap());
         hipLaunchKernelGGL((kb_pass3<NS, DM>), dim3(g.nb_n), blk, 0, s, g.desc, g.blk_n, t.remap());
     }
 }
