@@ -661,17 +661,22 @@ _BASE = {}
 
 
 def _base_case(mesh, device, dt):
-    """Seeded base inputs of a mesh (the configs' recipe), built once per mesh."""
+    """Seeded base inputs of a mesh (the configs' recipe), built once per mesh. An
+    entry goes with its mesh (so a later mesh that reuses the id never finds it)."""
+    import weakref
     from .inputs import make_case_device
     key = (id(mesh), str(device), dt)
-    if key not in _BASE:
-        _BASE[key] = make_case_device(mesh, device, seed=1234, dt=dt,
-                                      tracers=("T", "S", "T", "S"))
-    return _BASE[key]
+    hit = _BASE.get(key)
+    if hit is None or hit[0]() is not mesh:
+        _BASE[key] = (weakref.ref(mesh), make_case_device(mesh, device, seed=1234, dt=dt,
+                                                          tracers=("T", "S", "T", "S")))
+        weakref.finalize(mesh, _BASE.pop, key, None)
+    return _BASE[key][1]
 
 
 def _has_base_case(mesh, device, dt):
-    return (id(mesh), str(device), dt) in _BASE
+    hit = _BASE.get((id(mesh), str(device), dt))
+    return hit is not None and hit[0]() is mesh
 
 
 def _drop_base_case(mesh, device, dt):

@@ -113,3 +113,27 @@ def test_adjoint_plan_stays_while_its_mesh_has_an_adjoint_in_flight():
     assert not adj.closed
     assert srv._release_idle(500, [_Active(5, 47, 100)])
     assert adj.closed and (5, 47) not in srv.adj_plans
+
+
+def test_base_inputs_go_with_their_mesh(monkeypatch):
+    """The base-input cache never hands a mesh another mesh's inputs, even when the
+    other mesh's id is reused, and drops an entry when its mesh is collected."""
+    import gc
+    from fesom_jax_amd import inputs
+
+    class M:
+        pass
+    built = []
+    monkeypatch.setattr(inputs, "make_case_device",
+                        lambda mesh, device, **kw: built.append(id(mesh)) or {"mesh": id(mesh)})
+    a = M()
+    assert serve._base_case(a, "cpu", 1.0) == {"mesh": id(a)}
+    assert serve._has_base_case(a, "cpu", 1.0)
+    key = (id(a), "cpu", 1.0)
+    del a
+    gc.collect()
+    assert key not in serve._BASE
+    b = M()
+    assert not serve._has_base_case(b, "cpu", 1.0) or serve._BASE[(id(b), "cpu", 1.0)][0]() is b
+    serve._base_case(b, "cpu", 1.0)
+    assert serve._base_case(b, "cpu", 1.0) == {"mesh": id(b)} and len(built) == 2
