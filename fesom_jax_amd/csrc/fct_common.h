@@ -94,13 +94,18 @@ struct NodeTab {
     // slots past a node's counts repeat its first slot, so a kernel can load every slot
     // without waiting for the counts
     EdgePatches ep;           // edge patches of the adf_h patch kernel
+    // order in which the vertical-flux kernel visits the edges (NULL = id order); the
+    // patches' edge order when the caller's numbering is not local
+    const int* vorder = nullptr;
 };
 
-// Host side of the edge patches (fct_locality.hip), tables as in EdgePatches. `udv` and
+// Host side of the edge patches (fct_locality.hip), tables as in EdgePatches, and the
+// breadth-first node order of the patches' balls. `udv` and
 // `has_updn` are the up/down triangle vertex table and whether the edge has both
 // triangles, `utri` the level range in which an edge uses its up/down triangles'
 // gradients.
 struct LocalityHost {
+    std::vector<int> node_order;    // nodes ball by ball (breadth-first), balls in order
     std::vector<int4> ptab;
     std::vector<int> pnodes, elist;
     std::vector<int2> eslot;

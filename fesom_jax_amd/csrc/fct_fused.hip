@@ -224,7 +224,7 @@ __device__ __forceinline__ void vflux_group(const FusedArgs& a, int g) {
 #pragma unroll
     for (int q = 0; q < EPW; ++q) {
         const int eq = g * EPW + q;
-        e[q] = eq < m.D ? eq : m.D - 1;
+        e[q] = eq < m.D ? (a.tab.vorder ? ldc(a.tab.vorder, eq) : eq) : m.D - 1;
         et[q] = ldc(m.edge_tri, e[q]);
         kr[q] = eq < m.D ? ldc(a.ekr, e[q]) : make_int2(0, 0);
         c[q] = ldc(m.cross, e[q]);
@@ -1644,6 +1644,9 @@ static void launch_single(const FusedArgs& fa, const FusedTuning& t, hipStream_t
                 // nl <= 64: every column is shallow; both tracers per wavefront
                 hipLaunchKernelGGL((kl_pass1_all<DM, 2>), dim3(blocks_for(fa.tab.nshallow, t.wpb)),
                                    blk, 0, s, fa, fa.tab.nshallow);
+            } else if (NS == 1 && fa.tab.nlist) {
+                // nl <= 64 in the list's (visit) order
+                hipLaunchKernelGGL((kl_pass1<1, DM>), dim3(gn), blk, 0, s, fa, 0, m.N);
             } else if (NS == 2 && fa.tab.nlist) {
                 const int ns = fa.tab.nshallow, nd = m.N - fa.tab.nshallow;
                 // two tracers (temperature and salinity) share the tracer-independent
@@ -1693,6 +1696,9 @@ static void launch_single(const FusedArgs& fa, const FusedTuning& t, hipStream_t
                 const dim3 g(blocks_for(tb.nshallow, t.wpb));
                 if (stage == 3) hipLaunchKernelGGL((kl_pass2_all<DM, 2>), g, blk, 0, s, fa, tb.nshallow);
                 else hipLaunchKernelGGL((kl_pass3_all<DM, 2>), g, blk, 0, s, fa, tb.nshallow);
+            } else if (NS == 1 && tb.nlist) {
+                if (stage == 3) hipLaunchKernelGGL((kl_pass2<1, DM>), dim3(gn), blk, 0, s, fa, 0, m.N);
+                else hipLaunchKernelGGL((kl_pass3<1, DM>), dim3(gn), blk, 0, s, fa, 0, m.N);
             } else if (NS == 2 && tb.nlist) {
                 const int ns = tb.nshallow, nd = m.N - tb.nshallow;
                 const unsigned g1 = blocks_for(node_waves(ns, fa.K), t.wpb);

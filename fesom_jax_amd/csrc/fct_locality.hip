@@ -130,6 +130,11 @@ int build_locality(int N, int D, const int2* edges, const int* eoff, const int* 
     }
     std::vector<int> balls;   // balls in the final order
     for (const auto& g : members) balls.insert(balls.end(), g.begin(), g.end());
+    out.node_order.clear();
+    out.node_order.reserve(N);
+    for (int b : balls)
+        out.node_order.insert(out.node_order.end(), order.begin() + ball_start[b],
+                              order.begin() + ball_start[b + 1]);
 
     // 2. edge patches: the owned edges of each node patch, split where the node list
     //    (edge ends and up/down triangle vertices) would pass FCT_PU
