@@ -116,7 +116,8 @@ class FctPhaseBufs(ctypes.Structure):
 class FctPlanInfo(ctypes.Structure):
     _fields_ = [("nod2D", c_int32), ("elem2D", c_int32), ("edge2D", c_int32), ("nl", c_int32),
                 ("workspace_bytes", c_int64), ("device", c_int32), ("paths", c_int32),
-                ("table_bytes", c_int64), ("table_id", ctypes.c_uint64)]
+                ("table_bytes", c_int64), ("table_id", ctypes.c_uint64),
+                ("patch_order", c_int32), ("reserved", c_int32)]
 
 
 _LIB = None

@@ -770,6 +770,8 @@ int fct_plan_get_info(const fct_plan* p, fct_plan_info* info) {
     info->workspace_bytes = p->bytes + (p->tables ? p->tables->bytes : 0);
     info->table_bytes = p->tables ? p->tables->bytes : 0;
     info->table_id = (uint64_t)(uintptr_t)p->tables;
+    info->patch_order = p->tab.vorder != nullptr;
+    info->reserved = 0;
     info->device = p->device;
     info->paths = FCT_PATH_BIT_PHASED | (p->tab.dm > 0 ? FCT_PATH_BIT_FUSED : 0);
     return FCT_OK;

@@ -121,6 +121,12 @@ class FctPlan:
         info = self._info()
         return int(info.table_id), int(info.table_bytes)
 
+    @property
+    def patch_order(self) -> bool:
+        """Whether the mesh's numbering was found scattered, so that the node and
+        vertical-flux passes visit it in the edge patches' breadth-first order."""
+        return bool(self._info().patch_order)
+
     def close(self):
         self._finalizer()
 

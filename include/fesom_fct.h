@@ -142,6 +142,10 @@ typedef struct fct_plan_info {
     int32_t paths;             /* FCT_PATH_BIT_* of the paths this mesh can take */
     int64_t table_bytes;       /* the mesh tables' share of workspace_bytes */
     uint64_t table_id;         /* equal for plans that share their mesh tables */
+    int32_t patch_order;       /* 1: the caller's numbering is not local, so the node
+                                  passes and the vertical-flux pass visit nodes and edges
+                                  in the edge patches' breadth-first order; 0: id order */
+    int32_t reserved;
 } fct_plan_info;
 
 #define FCT_PATH_BIT_PHASED 1

@@ -354,7 +354,10 @@ def test_shuffled_numbering(nl, torch_dev):
     per-node sum still runs in the numbering's (edge, side) order)."""
     from fesom_jax_amd.inputs import make_case
     from fesom_jax_amd.meshgen import icosahedral_mesh, shuffled
-    m = shuffled(icosahedral_mesh(4, nl, seed=31, degree8=6), seed=32)
+    from fesom_jax_amd.tracer_adv import get_plan
+    base = icosahedral_mesh(4, nl, seed=31, degree8=6)
+    m = shuffled(base, seed=32)
+    assert get_plan(m, torch_dev).patch_order and not get_plan(base, torch_dev).patch_order
     case = sharp_case(m, make_case(m, seed=33, dt=2700.0), 2.0)
     outs = _gpu_step(m, case, torch_dev, path="fused")
     for j, g in enumerate(outs):
