@@ -1157,7 +1157,7 @@ int build_slot(fct_batch* b, BatchSlot& s, const fct_step_args* args) {
         g.ns = keys[c].first;
         g.dm = keys[c].second;
         std::vector<unsigned char> desc;
-        std::vector<int2> bv, bn, ba, bns, bnd, bpa, be2, be1;
+        std::vector<int2> bv, bn, ba, bns, bnd, bpa, bp1, be2, be1;
         const bool split = g.ns == 2;     // every instance of such a group has node lists
         for (int i = 0; i < n; ++i) {
             if (cls[i] != (int)c) continue;
@@ -1172,9 +1172,10 @@ int build_slot(fct_batch* b, BatchSlot& s, const fct_step_args* args) {
             if (split && !p->tab.nlist)
                 return set_error(FCT_E_UNSUPPORTED, "batch: plan without node lists");
             // the single-instance launcher's choices, per instance
-            const bool pair = p->tab.nlist && g.dm == 6 && K == 2 && t.pair;
+            const bool pair = p->tab.nlist && g.dm == 6 && K >= 2 && t.pair;
             if (pair) {
-                block_table(bpa, inst, p->tab.nshallow, t.wpb);
+                for (int j0 = 0; j0 < K; j0 += 2)
+                    block_table(K - j0 >= 2 ? bpa : bp1, inst | (j0 << 16), p->tab.nshallow, t.wpb);
                 if (split) block_table(bnd, inst, node_waves(p->m.N - p->tab.nshallow, K), t.wpb);
             } else if (split) {
                 block_table(bns, inst, node_waves(p->tab.nshallow, K), t.wpb);
@@ -1199,6 +1200,7 @@ int build_slot(fct_batch* b, BatchSlot& s, const fct_step_args* args) {
             (rc = slot_upload(s, b->capture, &g.blk_n, bn)) ||
             (rc = slot_upload(s, b->capture, &g.blk_a, ba)) ||
             (rc = slot_upload(s, b->capture, &g.blk_pa, bpa)) ||
+            (rc = slot_upload(s, b->capture, &g.blk_p1, bp1)) ||
             (rc = slot_upload(s, b->capture, &g.blk_e2, be2)) ||
             (rc = slot_upload(s, b->capture, &g.blk_e1, be1)) ||
             (split && ((rc = slot_upload(s, b->capture, &g.blk_ns, bns)) ||
@@ -1207,7 +1209,8 @@ int build_slot(fct_batch* b, BatchSlot& s, const fct_step_args* args) {
         g.desc = reinterpret_cast<const FusedArgs*>(dd);
         g.nb_v = (unsigned)bv.size(); g.nb_n = (unsigned)bn.size(); g.nb_a = (unsigned)ba.size();
         g.nb_ns = (unsigned)bns.size(); g.nb_nd = (unsigned)bnd.size();
-        g.nb_pa = (unsigned)bpa.size(); g.nb_e2 = (unsigned)be2.size(); g.nb_e1 = (unsigned)be1.size();
+        g.nb_pa = (unsigned)bpa.size(); g.nb_p1 = (unsigned)bp1.size();
+        g.nb_e2 = (unsigned)be2.size(); g.nb_e1 = (unsigned)be1.size();
         s.groups.push_back(g);
     }
     s.bound = true;

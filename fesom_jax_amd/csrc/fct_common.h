@@ -242,14 +242,18 @@ struct BatchGroup {
     // and deep node lists (NodeTab::nlist)
     const int2* blk_ns = nullptr;
     const int2* blk_nd = nullptr;
-    // two tracers per wavefront (instances with K = 2 on 6-slot meshes): waves over each
-    // such instance's shallow node list; its deep nodes (nl > 64) stay in blk_nd
+    // several tracers per wavefront (instances with K >= 2 on 6-slot meshes): waves over
+    // each such instance's shallow node list, per pair of tracer slots (blk_pa) and for
+    // the last slot of an odd K (blk_p1), the instance field carrying the first slot in
+    // its high 16 bits; the deep nodes (nl > 64) stay in blk_nd
     const int2* blk_pa = nullptr;
+    const int2* blk_p1 = nullptr;
     // adf_h by edge patches: per workgroup (instance, patch | first tracer << 24), two
     // tracers per thread (e2) or one (e1); instances without patches stay in blk_a
     const int2* blk_e2 = nullptr;
     const int2* blk_e1 = nullptr;
-    unsigned nb_v = 0, nb_n = 0, nb_a = 0, nb_ns = 0, nb_nd = 0, nb_pa = 0, nb_e2 = 0, nb_e1 = 0;
+    unsigned nb_v = 0, nb_n = 0, nb_a = 0, nb_ns = 0, nb_nd = 0, nb_pa = 0, nb_p1 = 0, nb_e2 = 0,
+             nb_e1 = 0;
 };
 
 // Fused launcher (fct_fused.hip): kw = tracers per chunk (workspace slots).

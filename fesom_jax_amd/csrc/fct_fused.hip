@@ -392,7 +392,7 @@ __device__ __forceinline__ void pass1_body(const FusedArgs& a, int n, int j, int
 // areasvol, hnode, hnode_new) are made once; per tracer the arithmetic is pass1_body's.
 // KT is a compile-time count so that the tracer pointers are read at constant offsets.
 template <int DM, int KT>
-__device__ __forceinline__ void pass1_shallow_all(const FusedArgs& a, int n, int4 hd) {
+__device__ __forceinline__ void pass1_shallow_all(const FusedArgs& a, int n, int4 hd, int j0) {
     const DevMesh& m = a.m;
     const int L = m.L;
     const int2 nr = make_int2(hd.x, hd.y), cnt = make_int2(hd.z & 0xff, hd.w);
@@ -424,7 +424,8 @@ __device__ __forceinline__ void pass1_shallow_all(const FusedArgs& a, int n, int
     }
     const double w1 = wave_next(w0, 0.0), a1 = wave_next(a0, 0.0);
 #pragma unroll
-    for (int j = 0; j < KT; ++j) {
+    for (int jj = 0; jj < KT; ++jj) {
+        const int j = j0 + jj;   // tracer slot
         const auto T = G(a.T[j]);
         const size_t jo = (size_t)j * m.N * L;
         const double Tk = rd ? T[b0 + k] : 0.0;
@@ -998,7 +999,7 @@ __device__ __forceinline__ void pass2_body(const FusedArgs& a, int n, int j, int
 // area, areasvol, the stencil depths and hnode_new are loaded once; per tracer the
 // arithmetic is pass2_body's.
 template <int DM, int KT>
-__device__ __forceinline__ void pass2_shallow_all(const FusedArgs& a, int n, int2 nr, int2 cnt) {
+__device__ __forceinline__ void pass2_shallow_all(const FusedArgs& a, int n, int2 nr, int2 cnt, int j0) {
     constexpr int NS = 1;
     const DevMesh& m = a.m;
     const int L = m.L;
@@ -1010,7 +1011,8 @@ __device__ __forceinline__ void pass2_shallow_all(const FusedArgs& a, int n, int
         hns[0] = in_layer(nr, kl) ? G(a.hnn)[(size_t)n * L + kl] : 0.0;
     }
 #pragma unroll
-    for (int j = 0; j < KT; ++j) {
+    for (int jj = 0; jj < KT; ++jj) {
+        const int j = j0 + jj;   // tracer slot
         const size_t jo = (size_t)j * m.N * L;
         const size_t b0 = (size_t)n * L;
         const auto Tb = G(a.T[j]);
@@ -1240,7 +1242,7 @@ __device__ __forceinline__ void pass3_body(const FusedArgs& a, int n, int j, int
 // areasvol, hnode_new and hnode are loaded once; per tracer the arithmetic is
 // pass3_body's.
 template <int DM, int KT>
-__device__ __forceinline__ void pass3_shallow_all(const FusedArgs& a, int n, int2 nr, int2 cnt) {
+__device__ __forceinline__ void pass3_shallow_all(const FusedArgs& a, int n, int2 nr, int2 cnt, int j0) {
     constexpr int NS = 1;
     const DevMesh& m = a.m;
     const int L = m.L;
@@ -1256,7 +1258,8 @@ __device__ __forceinline__ void pass3_shallow_all(const FusedArgs& a, int n, int
         hnd_s = G(a.hnode)[(size_t)n * L + kl];
     }
 #pragma unroll
-    for (int j = 0; j < KT; ++j) {
+    for (int jj = 0; jj < KT; ++jj) {
+        const int j = j0 + jj;   // tracer slot
         const auto T = G(a.T[j]);
         const size_t jo = (size_t)j * m.N * L;
         const double2* pm = reinterpret_cast<const double2*>(a.pf) + jo;   // (plus, minus) pairs
@@ -1549,65 +1552,66 @@ FCT_BATCH_LIST_KERNEL(pass3, 8, 6, (pass3_body<NSB, DM>(a, n, w - t * a.K, make_
                                                         make_int2(hd.z & 0xff, hd.w))))
 #undef FCT_BATCH_LIST_KERNEL
 
+// one wavefront per shallow node for KT tracers from slot j0
 template <int DM, int KT>
-__global__ void FCT_OCC(DM == 6 ? 6 : 4) kl_pass2_all(FusedArgs a, int count) {
+__global__ void FCT_OCC(DM == 6 ? 6 : 4) kl_pass2_all(FusedArgs a, int count, int j0) {
     int w;
     if (!single_wave((int64_t)count, a.remap, w)) return;
     const int n = ldc(a.tab.nlist, (size_t)w);
     const int4 hd = ldc(a.tab.hdr, n);
-    pass2_shallow_all<DM, KT>(a, n, make_int2(hd.x, hd.y), make_int2(hd.z & 0xff, hd.w));
+    pass2_shallow_all<DM, KT>(a, n, make_int2(hd.x, hd.y), make_int2(hd.z & 0xff, hd.w), j0);
 }
 
 template <int DM, int KT>
-__global__ void FCT_OCC(DM == 6 ? 7 : 4) kl_pass3_all(FusedArgs a, int count) {
+__global__ void FCT_OCC(DM == 6 ? 7 : 4) kl_pass3_all(FusedArgs a, int count, int j0) {
     int w;
     if (!single_wave((int64_t)count, a.remap, w)) return;
     const int n = ldc(a.tab.nlist, (size_t)w);
     const int4 hd = ldc(a.tab.hdr, n);
-    pass3_shallow_all<DM, KT>(a, n, make_int2(hd.x, hd.y), make_int2(hd.z & 0xff, hd.w));
+    pass3_shallow_all<DM, KT>(a, n, make_int2(hd.x, hd.y), make_int2(hd.z & 0xff, hd.w), j0);
 }
 
-// one wavefront per shallow node, KT = K tracers
 template <int DM, int KT>
-__global__ void FCT_OCC(DM == 6 ? 5 : 4) kl_pass1_all(FusedArgs a, int count) {
+__global__ void FCT_OCC(DM == 6 ? 5 : 4) kl_pass1_all(FusedArgs a, int count, int j0) {
     int w;
     if (!single_wave((int64_t)count, a.remap, w)) return;
     const int n = ldc(a.tab.nlist, (size_t)w);
-    pass1_shallow_all<DM, KT>(a, n, ldc(a.tab.hdr, n));
+    pass1_shallow_all<DM, KT>(a, n, ldc(a.tab.hdr, n), j0);
 }
 
-// batched two-tracer passes: waves over each instance's shallow node list
-template <int DM>
+// batched multi-tracer passes: waves over each instance's shallow node list; the block
+// table's instance field carries the first tracer slot in its high 16 bits
+template <int DM, int KT>
 __global__ void FCT_OCC(DM == 6 ? 5 : 4) kbp_pass1(const FusedArgs* __restrict__ d,
                                                      const int2* __restrict__ blk, int remap) {
-    int inst;
-    const int w = batch_wave(blk, remap, inst);
-    const FusedArgs& a = d[inst];
+    int ij;
+    const int w = batch_wave(blk, remap, ij);
+    const FusedArgs& a = d[ij & 0xffff];
     if (w >= a.tab.nshallow) return;
     const int n = ldc(a.tab.nlist, (size_t)w);
-    pass1_shallow_all<DM, 2>(a, n, ldc(a.tab.hdr, n));
+    pass1_shallow_all<DM, KT>(a, n, ldc(a.tab.hdr, n), ij >> 16);
 }
-template <int DM>
+template <int DM, int KT>
 __global__ void FCT_OCC(DM == 6 ? 6 : 4) kbp_pass2(const FusedArgs* __restrict__ d,
                                                      const int2* __restrict__ blk, int remap) {
-    int inst;
-    const int w = batch_wave(blk, remap, inst);
-    const FusedArgs& a = d[inst];
+    int ij;
+    const int w = batch_wave(blk, remap, ij);
+    const FusedArgs& a = d[ij & 0xffff];
     if (w >= a.tab.nshallow) return;
     const int n = ldc(a.tab.nlist, (size_t)w);
     const int4 hd = ldc(a.tab.hdr, n);
-    pass2_shallow_all<DM, 2>(a, n, make_int2(hd.x, hd.y), make_int2(hd.z & 0xff, hd.w));
+    pass2_shallow_all<DM, KT>(a, n, make_int2(hd.x, hd.y), make_int2(hd.z & 0xff, hd.w), ij >> 16);
 }
-template <int DM>
+template <int DM, int KT>
 __global__ void FCT_OCC(DM == 6 ? 7 : 4) kbp_pass3(const FusedArgs* __restrict__ d,
                                                      const int2* __restrict__ blk, int remap) {
-    int inst;
-    const int w = batch_wave(blk, remap, inst);
-    const FusedArgs& a = d[inst];
+    int ij;
+    const int w = batch_wave(blk, remap, ij);
+    const FusedArgs& a = d[ij & 0xffff];
     if (w >= a.tab.nshallow) return;
     const int n = ldc(a.tab.nlist, (size_t)w);
     const int4 hd = ldc(a.tab.hdr, n);
-    pass3_shallow_all<DM, 2>(a, n, make_int2(hd.x, hd.y), make_int2(hd.z & 0xff, hd.w));
+    pass3_shallow_all<DM, KT>(a, n, make_int2(hd.x, hd.y), make_int2(hd.z & 0xff, hd.w), ij >> 16);
 }
 
 template <int NSB, int DM>
@@ -1627,11 +1631,35 @@ static inline unsigned blocks_for(int64_t waves, int wpb) {
     return (unsigned)((waves + wpb - 1) / wpb);
 }
 
+// node pass `stage` (1, 3 or 4) over the first `count` nodes of the node list with the
+// tracers taken two per wavefront (and a last one alone when K is odd)
+template <int DM>
+static void launch_shallow_multi(const FusedArgs& fa, const FusedTuning& t, hipStream_t s,
+                                 int stage, int count) {
+    const dim3 blk(64 * t.wpb), g(blocks_for(count, t.wpb));
+    for (int j0 = 0; j0 < fa.K; j0 += 2) {
+        const bool two = fa.K - j0 >= 2;
+        if (stage == 1) {
+            if (two) hipLaunchKernelGGL((kl_pass1_all<DM, 2>), g, blk, 0, s, fa, count, j0);
+            else hipLaunchKernelGGL((kl_pass1_all<DM, 1>), g, blk, 0, s, fa, count, j0);
+        } else if (stage == 3) {
+            if (two) hipLaunchKernelGGL((kl_pass2_all<DM, 2>), g, blk, 0, s, fa, count, j0);
+            else hipLaunchKernelGGL((kl_pass2_all<DM, 1>), g, blk, 0, s, fa, count, j0);
+        } else {
+            if (two) hipLaunchKernelGGL((kl_pass3_all<DM, 2>), g, blk, 0, s, fa, count, j0);
+            else hipLaunchKernelGGL((kl_pass3_all<DM, 1>), g, blk, 0, s, fa, count, j0);
+        }
+    }
+}
+
 template <int NS, int DM>
 static void launch_single(const FusedArgs& fa, const FusedTuning& t, hipStream_t s, int stage) {
     const DevMesh& m = fa.m;
     const dim3 blk(64 * t.wpb);
     const unsigned gn = blocks_for(node_waves(m.N, fa.K), t.wpb);
+    // several tracers share the tracer-independent loads of a shallow node in one
+    // wavefront (two at a time)
+    const bool multi = DM == 6 && fa.K >= 2 && t.pair && fa.tab.nlist;
     switch (stage) {
         case 0:
             if (t.vflux_epw == 2)
@@ -1640,20 +1668,16 @@ static void launch_single(const FusedArgs& fa, const FusedTuning& t, hipStream_t
                 hipLaunchKernelGGL((kf_vflux<NS, 4>), dim3(blocks_for(vflux_waves(m.D, t), t.wpb)), blk, 0, s, fa);
             break;
         case 1:
-            if (NS == 1 && DM == 6 && fa.K == 2 && t.pair && fa.tab.nlist) {
-                // nl <= 64: every column is shallow; both tracers per wavefront
-                hipLaunchKernelGGL((kl_pass1_all<DM, 2>), dim3(blocks_for(fa.tab.nshallow, t.wpb)),
-                                   blk, 0, s, fa, fa.tab.nshallow);
+            if (NS == 1 && multi) {
+                // nl <= 64: every column is shallow
+                launch_shallow_multi<DM>(fa, t, s, 1, fa.tab.nshallow);
             } else if (NS == 1 && fa.tab.nlist) {
                 // nl <= 64 in the list's (visit) order
                 hipLaunchKernelGGL((kl_pass1<1, DM>), dim3(gn), blk, 0, s, fa, 0, m.N);
             } else if (NS == 2 && fa.tab.nlist) {
                 const int ns = fa.tab.nshallow, nd = m.N - fa.tab.nshallow;
-                // two tracers (temperature and salinity) share the tracer-independent
-                // loads of a shallow node in one wavefront
-                if (ns && DM == 6 && fa.K == 2 && t.pair)
-                    hipLaunchKernelGGL((kl_pass1_all<DM, 2>), dim3(blocks_for(ns, t.wpb)),
-                                       blk, 0, s, fa, ns);
+                if (ns && multi)
+                    launch_shallow_multi<DM>(fa, t, s, 1, ns);
                 else if (ns)
                     hipLaunchKernelGGL((kl_pass1<1, DM>), dim3(blocks_for(node_waves(ns, fa.K), t.wpb)),
                                        blk, 0, s, fa, 0, ns);
@@ -1692,10 +1716,8 @@ static void launch_single(const FusedArgs& fa, const FusedTuning& t, hipStream_t
         case 3:
         case 4: {
             const NodeTab& tb = fa.tab;
-            if (NS == 1 && DM == 6 && fa.K == 2 && t.pair && tb.nlist) {
-                const dim3 g(blocks_for(tb.nshallow, t.wpb));
-                if (stage == 3) hipLaunchKernelGGL((kl_pass2_all<DM, 2>), g, blk, 0, s, fa, tb.nshallow);
-                else hipLaunchKernelGGL((kl_pass3_all<DM, 2>), g, blk, 0, s, fa, tb.nshallow);
+            if (NS == 1 && multi) {
+                launch_shallow_multi<DM>(fa, t, s, stage, tb.nshallow);
             } else if (NS == 1 && tb.nlist) {
                 if (stage == 3) hipLaunchKernelGGL((kl_pass2<1, DM>), dim3(gn), blk, 0, s, fa, 0, m.N);
                 else hipLaunchKernelGGL((kl_pass3<1, DM>), dim3(gn), blk, 0, s, fa, 0, m.N);
@@ -1704,16 +1726,14 @@ static void launch_single(const FusedArgs& fa, const FusedTuning& t, hipStream_t
                 const unsigned g1 = blocks_for(node_waves(ns, fa.K), t.wpb);
                 const unsigned g2 = blocks_for(node_waves(nd, fa.K), t.wpb);
                 if (stage == 3) {
-                    if (ns && DM == 6 && fa.K == 2 && t.pair)
-                        hipLaunchKernelGGL((kl_pass2_all<DM, 2>), dim3(blocks_for(ns, t.wpb)), blk, 0,
-                                           s, fa, ns);
+                    if (ns && multi)
+                        launch_shallow_multi<DM>(fa, t, s, 3, ns);
                     else if (ns)
                         hipLaunchKernelGGL((kl_pass2<1, DM>), dim3(g1), blk, 0, s, fa, 0, ns);
                     if (nd) hipLaunchKernelGGL((kl_pass2<2, DM>), dim3(g2), blk, 0, s, fa, ns, nd);
                 } else {
-                    if (ns && DM == 6 && fa.K == 2 && t.pair)
-                        hipLaunchKernelGGL((kl_pass3_all<DM, 2>), dim3(blocks_for(ns, t.wpb)), blk, 0,
-                                           s, fa, ns);
+                    if (ns && multi)
+                        launch_shallow_multi<DM>(fa, t, s, 4, ns);
                     else if (ns)
                         hipLaunchKernelGGL((kl_pass3<1, DM>), dim3(g1), blk, 0, s, fa, 0, ns);
                     if (nd) hipLaunchKernelGGL((kl_pass3<2, DM>), dim3(g2), blk, 0, s, fa, ns, nd);
@@ -1862,7 +1882,8 @@ static void launch_batch_t(const BatchGroup& g, const FusedTuning& t, hipStream_
             hipLaunchKernelGGL((kb_vflux<NS, 4>), dim3(g.nb_v), blk, 0, s, g.desc, g.blk_v, t.remap());
     }
     const bool split = NS == 2 && g.blk_ns;
-    if (g.nb_pa) hipLaunchKernelGGL((kbp_pass1<DM>), dim3(g.nb_pa), blk, 0, s, g.desc, g.blk_pa, t.remap());
+    if (g.nb_pa) hipLaunchKernelGGL((kbp_pass1<DM, 2>), dim3(g.nb_pa), blk, 0, s, g.desc, g.blk_pa, t.remap());
+    if (g.nb_p1) hipLaunchKernelGGL((kbp_pass1<DM, 1>), dim3(g.nb_p1), blk, 0, s, g.desc, g.blk_p1, t.remap());
     if (split) {
         if (g.nb_ns) hipLaunchKernelGGL((kbl_pass1<1, DM>), dim3(g.nb_ns), blk, 0, s, g.desc, g.blk_ns, t.remap(), 0);
         if (g.nb_nd) hipLaunchKernelGGL((kbl_pass1<2, DM>), dim3(g.nb_nd), blk, 0, s, g.desc, g.blk_nd, t.remap(), 1);
@@ -1894,14 +1915,16 @@ static void launch_batch_t(const BatchGroup& g, const FusedTuning& t, hipStream_
         else
             hipLaunchKernelGGL((kb_adfh<NS, 2>), dim3(g.nb_a), blk, 0, s, g.desc, g.blk_a, t.remap());
     }
-    if (g.nb_pa) hipLaunchKernelGGL((kbp_pass2<DM>), dim3(g.nb_pa), blk, 0, s, g.desc, g.blk_pa, t.remap());
+    if (g.nb_pa) hipLaunchKernelGGL((kbp_pass2<DM, 2>), dim3(g.nb_pa), blk, 0, s, g.desc, g.blk_pa, t.remap());
+    if (g.nb_p1) hipLaunchKernelGGL((kbp_pass2<DM, 1>), dim3(g.nb_p1), blk, 0, s, g.desc, g.blk_p1, t.remap());
     if (split) {
         if (g.nb_ns) hipLaunchKernelGGL((kbl_pass2<1, DM>), dim3(g.nb_ns), blk, 0, s, g.desc, g.blk_ns, t.remap(), 0);
         if (g.nb_nd) hipLaunchKernelGGL((kbl_pass2<2, DM>), dim3(g.nb_nd), blk, 0, s, g.desc, g.blk_nd, t.remap(), 1);
     } else if (g.nb_n) {
         hipLaunchKernelGGL((kb_pass2<NS, DM>), dim3(g.nb_n), blk, 0, s, g.desc, g.blk_n, t.remap());
     }
-    if (g.nb_pa) hipLaunchKernelGGL((kbp_pass3<DM>), dim3(g.nb_pa), blk, 0, s, g.desc, g.blk_pa, t.remap());
+    if (g.nb_pa) hipLaunchKernelGGL((kbp_pass3<DM, 2>), dim3(g.nb_pa), blk, 0, s, g.desc, g.blk_pa, t.remap());
+    if (g.nb_p1) hipLaunchKernelGGL((kbp_pass3<DM, 1>), dim3(g.nb_p1), blk, 0, s, g.desc, g.blk_p1, t.remap());
     if (split) {
         if (g.nb_ns) hipLaunchKernelGGL((kbl_pass3<1, DM>), dim3(g.nb_ns), blk, 0, s, g.desc, g.blk_ns, t.remap(), 0);
         if (g.nb_nd) hipLaunchKernelGGL((kbl_pass3<2, DM>), dim3(g.nb_nd), blk, 0, s, g.desc, g.blk_nd, t.remap(), 1);

@@ -429,21 +429,26 @@ def _pi_like():
     return icosahedral_mesh(2, 24, seed=3)
 
 
-def test_two_tracers_per_wavefront_nl48(torch_dev):
-    """nl <= 64 with two tracers: every column takes the two-tracer node passes; bit for
-    bit with the oracle and with one tracer per wavefront (limiter-active state)."""
+@pytest.mark.parametrize("nl, K", [(48, 2), (70, 3), (48, 4)])
+def test_tracers_share_wavefronts(nl, K, torch_dev):
+    """Shallow columns take the multi-tracer node passes (two tracer slots per
+    wavefront, the last alone for odd K), for nl <= 64 and nl > 64; bit for bit with
+    the oracle and with one tracer per wavefront (limiter-active state)."""
     from fesom_jax_amd import _lib
     from fesom_jax_amd.inputs import make_case
     from fesom_jax_amd.meshgen import icosahedral_mesh
     from fesom_jax_amd.tracer_adv import get_plan
-    m = icosahedral_mesh(4, 48, seed=61)
+    m = icosahedral_mesh(4, nl, seed=61)
     base = make_case(m, seed=62, dt=2700.0)
     sharp = sharp_case(m, base, 2.0)
-    # the sharp tracer (limiter active) and the smooth one of the base case
-    case = dict(sharp, T=[sharp["T"][0], base["T"][0]], T_old=[sharp["T_old"][0], base["T_old"][0]])
+    # the sharp tracer (limiter active) and smooth ones of the base case
+    Ts = [sharp["T"][0]] + [base["T"][j % len(base["T"])] * (1.0 + 0.1 * j) for j in range(K - 1)]
+    Tos = [sharp["T_old"][0]] + [base["T_old"][j % len(base["T"])] * (1.0 + 0.1 * j)
+                                for j in range(K - 1)]
+    case = dict(sharp, T=Ts, T_old=Tos)
     paired = _gpu_step(m, case, torch_dev, path="fused")
     for j, g in enumerate(paired):
-        _check(f"nl48 paired tracer {j}", g, _oracle(m, case, j), m)
+        _check(f"nl{nl} K{K} shared tracer {j}", g, _oracle(m, case, j), m)
     plan = get_plan(m, torch_dev)
     try:
         _lib.check(plan.lib.fct_plan_set_tuning(plan.ptr, _lib.FCT_TUNE_PAIR_TRACERS, 0), "tuning")
