@@ -269,23 +269,27 @@ class Server:
         if not users:
             del self.table_sets[tid]
 
-    def _release_idle(self, need, active, busy=(), keep=None, keep_base=None):
+    def _release_idle(self, need, active, busy=(), keep=None, keep_base=None, keep_adj=None):
         """Close idle plans (largest first, never ``keep``: the plan the admission that
         asks will reuse, whose bytes ``need`` leaves out), then drop the base inputs of
         meshes no active or finishing request uses, nor the asking admission
         (``keep_base``; they are rebuilt when needed), until ``need`` more bytes fit;
-        returns whether they do."""
+        returns whether they do. ``need`` may be a function of the server's state,
+        evaluated after every release: closing the last live plan of the asking
+        admission's mesh makes that admission build the mesh's tables again."""
+        need_now = need if callable(need) else (lambda: need)
         # an adjoint plan is in use while a request of its mesh that asks for the adjoint
-        # is active or finishing, and kept for the asking admission's mesh
+        # is active or finishing; it is kept for an asking admission that runs an adjoint
+        # (``keep_adj``, its mesh), while a forward-only one may release it
         adj_use = {(a.spec.level, a.spec.nl) for a in list(active) + list(busy) if a.spec.adjoint}
-        if keep_base is not None:
-            adj_use.add(keep_base)
+        if keep_adj is not None:
+            adj_use.add(keep_adj)
         idle = sorted([(self.plan_bytes[id(p)], k, p) for k, ps in self.pool.items() for p in ps
                        if p is not keep] +
                       [(self.plan_bytes[id(p)], k, p) for k, p in self.adj_plans.items()
                        if k not in adj_use], key=lambda x: -x[0])
         for nbytes, key, plan in idle:
-            if self.held_bytes(active) + need <= self.budget:
+            if self.held_bytes(active) + need_now() <= self.budget:
                 return True
             if key in self.adj_plans and self.adj_plans[key] is plan:
                 del self.adj_plans[key]
@@ -299,14 +303,14 @@ class Server:
         if keep_base is not None:
             in_use.add(keep_base)
         for key in sorted(self.base_bytes, key=lambda k: -self.base_bytes[k]):
-            if self.held_bytes(active) + need <= self.budget:
+            if self.held_bytes(active) + need_now() <= self.budget:
                 break
             if key in in_use:
                 continue
             _drop_base_case(self.meshes.get(*key), self.device, self.dt)
             del self.base_bytes[key]
             self.counts["base_dropped"] += 1
-        return self.held_bytes(active) + need <= self.budget
+        return self.held_bytes(active) + need_now() <= self.budget
 
     # -- request life cycle --------------------------------------------------------
     def _plan(self, mesh, spec):
@@ -472,12 +476,16 @@ class Server:
             now = time.perf_counter() - t0
             while (pending and pending[0].arrival <= now and len(active) < self.max_active):
                 s = pending[0]
-                need = self._need(s)
                 ps = self.pool[(s.level, s.nl, s.K)]
-                if (not self._release_idle(need, active, [f[0] for f in finishing],
-                                           keep=ps[-1] if ps else None,
-                                           keep_base=(s.level, s.nl))
-                        and (active or finishing)):
+                fits = self._release_idle(lambda s=s: self._need(s), active,
+                                          [f[0] for f in finishing],
+                                          keep=ps[-1] if ps else None,
+                                          keep_base=(s.level, s.nl),
+                                          keep_adj=(s.level, s.nl) if s.adjoint else None)
+                if not fits and not (active or finishing):
+                    # larger than the budget with everything else released: runs alone
+                    self.counts["admitted_alone_over_budget"] += 1
+                if not fits and (active or finishing):
                     # the oldest request waits for memory; until it has waited
                     # `backfill_s`, a later arrival that fits as things stand goes ahead
                     s = None
@@ -637,6 +645,12 @@ class Server:
                         self.setup, wall, busy, self.meshes)
         out["failed"] = len(self.errors)
         out["peak_bytes"] = int(self.peak_bytes)
+        out["budget_bytes"] = self.budget
+        if self.peak_detail is not None:
+            d = dict(self.peak_detail)
+            d["plans"] = {str(k): v for k, v in d["plans"].items()}
+            d["base"] = {str(k): v for k, v in d["base"].items()}
+            out["peak_detail"] = d
         out["device_peak_bytes"] = int(self.device_peak)
         out["timing_s"] = {k: round(v, 3) for k, v in self.timing.items()}
         out["counts"] = dict(self.counts)

@@ -115,6 +115,42 @@ def test_adjoint_plan_stays_while_its_mesh_has_an_adjoint_in_flight():
     assert adj.closed and (5, 47) not in srv.adj_plans
 
 
+def test_forward_admission_may_release_its_meshs_adjoint_plan():
+    """An idle adjoint plan is kept for an admission of its mesh only when that
+    admission runs an adjoint; a forward-only request of the mesh may release it (kept,
+    it would push the held bytes past the budget)."""
+    srv = _server(1000)
+    adj = _FakePlan(600)
+    srv.adj_plans[(9, 70)] = adj
+    srv._refresh(adj)
+    assert not srv._release_idle(500, [], keep_adj=(9, 70))
+    assert not adj.closed
+    assert srv._release_idle(500, [], keep_base=(9, 70))
+    assert adj.closed and (9, 70) not in srv.adj_plans
+    assert srv.held_bytes([]) + 500 <= srv.budget
+
+
+def test_need_counts_the_tables_a_release_takes_away():
+    """Closing the last live plan of the asking admission's mesh drops that mesh's
+    tables from the held bytes, and the admission will build them again: the need is
+    evaluated after each release, so the admission fits only if the tables fit too."""
+    srv = _server(650)
+    other_k = _FakePlan(300, (1, 200))          # the mesh's only live plan (K = 3)
+    srv.pool[(9, 70, 3)].append(other_k)
+    srv._refresh(other_k)
+
+    def need():                                 # fields + plan for K = 2 (+ tables)
+        return 500 + (200 if srv._live_plan(9, 70) is None else 0)
+    assert not srv._release_idle(need, [])
+    assert other_k.closed
+    srv2 = _server(700)
+    p = _FakePlan(300, (1, 200))
+    srv2.pool[(9, 70, 3)].append(p)
+    srv2._refresh(p)
+    assert srv2._release_idle(lambda: 500 + (200 if srv2._live_plan(9, 70) is None else 0), [])
+    assert srv2.held_bytes([]) == 0
+
+
 def test_base_inputs_go_with_their_mesh(monkeypatch):
     """The base-input cache never hands a mesh another mesh's inputs, even when the
     other mesh's id is reused, and drops an entry when its mesh is collected."""
