@@ -197,6 +197,8 @@ class Server:
         self.active = []
         self.peak_bytes = 0
         self.peak_detail = None
+        self.over_budget = None     # snapshot of the first moment held > budget, if any
+        self._admissions = collections.deque(maxlen=8)
         self.device_peak = 0        # device memory in use (all users), highest seen
         self.timing = collections.Counter()         # seconds per activity (host clock)
         self.counts = collections.Counter()         # plans built/closed, base inputs built/dropped
@@ -245,6 +247,8 @@ class Server:
         free, total = torch.cuda.mem_get_info(self.device)
         self.device_peak = max(self.device_peak, total - free)
         held = self.held_bytes(active)
+        if held > self.budget and self.over_budget is None:
+            self.over_budget = self._snapshot(active, held)
         if held > self.peak_bytes:
             self.peak_bytes = held
             self.peak_detail = {
@@ -253,6 +257,28 @@ class Server:
                           for k, ps in self.pool.items() if ps},
                 "active_plans": sum(self.plan_bytes.get(id(a.inst.plan), 0) for a in active),
                 "all_plans": sum(self.plan_bytes.values()), "base": dict(self.base_bytes)}
+
+    def _snapshot(self, active, held):
+        """What the server holds, part by part, and its last admissions (need, held
+        before and after): the record of the first moment the held bytes passed the
+        budget."""
+        label = {}
+        for key, ps in self.pool.items():
+            for p in ps:
+                label[id(p)] = f"idle {key}"
+        for key, p in self.adj_plans.items():
+            label[id(p)] = f"adjoint {key}"
+        for a in active:
+            label[id(a.inst.plan)] = f"active {a.spec.rid}"
+        for f in self._finishing:
+            label[id(f[0].inst.plan)] = f"finishing {f[0].spec.rid}"
+        return {"held": int(held),
+                "active": {a.spec.rid: a.bytes for a in active},
+                "finishing": {f[0].spec.rid: f[0].bytes for f in self._finishing},
+                "plans": {label.get(k, f"unlabelled {k}"): v for k, v in self.plan_bytes.items()},
+                "tables": {str(t): b for t, (b, _) in self.table_sets.items()},
+                "base": {str(k): v for k, v in self.base_bytes.items()},
+                "admissions": list(self._admissions)}
 
     def _refresh(self, plan):
         self.plan_bytes[id(plan)] = plan.own_bytes()
@@ -330,10 +356,12 @@ class Server:
         self.counts["plans_built" if share is None else "plans_cloned"] += 1
         return plan
 
-    def _adjoint_plan(self, mesh, spec):
+    def _adjoint_plan(self, mesh, spec, share=None):
         """The mesh's adjoint plan (one tracer of fused workspace: it only runs the
         backward, one request after another on the finishing stream); its adjoint
-        workspace is counted from admission on."""
+        workspace is counted from admission on. It shares the tables of ``share`` (the
+        admitted request's own plan, not yet among the active ones) or of another live
+        plan of the mesh."""
         from .tracer_adv import FctPlan
         import torch
         key = (spec.level, spec.nl)
@@ -341,7 +369,7 @@ class Server:
             torch.cuda.empty_cache()
             t = time.perf_counter()
             plan = FctPlan(mesh, self.device, max_tracers=1,
-                           share=self._live_plan(spec.level, spec.nl))
+                           share=share if share is not None else self._live_plan(spec.level, spec.nl))
             self.adj_plans[key] = plan
             self._refresh(plan)
             # the adjoint workspace is allocated by the first backward: count it now
@@ -374,7 +402,7 @@ class Server:
         torch.cuda.synchronize(self.device)
         nbytes = estimate_field_bytes(mesh, spec.K, spec.nsteps, spec.adjoint)
         if spec.adjoint:
-            self._adjoint_plan(mesh, spec)
+            self._adjoint_plan(mesh, spec, share=plan)
         a = _Active(spec=spec, inst=inst, mesh=mesh, case_extra=extra, remaining=spec.nsteps,
                     bytes=nbytes, t_admit=now, setup_s=time.perf_counter() - t)
         if spec.adjoint:
@@ -501,6 +529,7 @@ class Server:
                         break
                     self.counts["backfilled"] += 1
                 pending.remove(s)
+                need_s, held_s = self._need(s), self.held_bytes(active)
                 try:
                     a = self._start(s, now)
                 except Exception as exc:          # setup failure stays with its request
@@ -508,6 +537,8 @@ class Server:
                     continue
                 self.setup[s.rid] = a.setup_s
                 active.append(a)
+                self._admissions.append((s.rid, s.level, s.nl, s.K, bool(s.adjoint), need_s,
+                                         held_s, self.held_bytes(active)))
                 self._note_peak(active)
                 changed = True
             if not active:
@@ -646,6 +677,8 @@ class Server:
         out["failed"] = len(self.errors)
         out["peak_bytes"] = int(self.peak_bytes)
         out["budget_bytes"] = self.budget
+        if self.over_budget is not None:
+            out["over_budget"] = self.over_budget
         if self.peak_detail is not None:
             d = dict(self.peak_detail)
             d["plans"] = {str(k): v for k, v in d["plans"].items()}

@@ -151,6 +151,27 @@ def test_need_counts_the_tables_a_release_takes_away():
     assert srv2.held_bytes([]) == 0
 
 
+def test_adjoint_plan_shares_the_admitted_requests_tables(monkeypatch):
+    """The adjoint plan built for an admission shares the tables of that request's own
+    plan (which is not yet among the active requests); a plan of its own would hold a
+    second copy of the mesh's tables that the admission never counted."""
+    from fesom_jax_amd import tracer_adv
+    made = []
+
+    class Plan(_FakePlan):
+        def __init__(self, mesh, device, max_tracers=4, share=None):
+            super().__init__(1, share.tables() if share is not None else (object(), 50))
+            made.append(share)
+    monkeypatch.setattr(tracer_adv, "FctPlan", Plan)
+    mesh = type("Mesh", (), {"nod2D": 10, "elem2D": 16, "edge2D": 25, "nl": 5})()
+    srv = _server(1 << 40)
+    own = _FakePlan(7, (123, 50))
+    srv._refresh(own)
+    adj = srv._adjoint_plan(mesh, _Spec(9, 70, adjoint=True), share=own)
+    assert made == [own] and adj.tables() == own.tables()
+    assert len(srv.table_sets) == 1 and srv.held_bytes([]) == 7 + 50 + srv.plan_bytes[id(adj)]
+
+
 def test_base_inputs_go_with_their_mesh(monkeypatch):
     """The base-input cache never hands a mesh another mesh's inputs, even when the
     other mesh's id is reused, and drops an entry when its mesh is collected."""
