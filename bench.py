@@ -792,7 +792,7 @@ def run_ice(args, dev):
                    "nodes": N, "elements": E},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": None, "kernel": "k_ice_* (five launches)", "bytes_per_call": B},
+                     "traffic": None, "kernel": "k_ice_* (six launches: the packing of the inputs, then five passes)", "bytes_per_call": B},
         "cpu_baseline": None,
     }
     print(json.dumps(result), flush=True)

@@ -83,9 +83,9 @@ struct fct_plan {
     // implicit vertical diffusion scratch (allocated on the first call)
     double* vd_scratch = nullptr;
     int vd_grid = 0;
-    // sea-ice FCT workspace and element factors (allocated on the first call)
+    // sea-ice FCT workspace and element table (allocated on the first call)
     double* ice_work = nullptr;
-    double* ice_sq = nullptr;
+    double* ice_tab = nullptr;
 };
 
 namespace {
@@ -883,10 +883,19 @@ int fct_ice_fct_solve(fct_plan* p, const fct_ice_args* a, void* stream) {
         HIPCHK(hipMemcpy(h.data(), p->m.earea, E * sizeof(double), hipMemcpyDeviceToHost));
         for (int e = 0; e < E; ++e) h[e] = std::sqrt(h[e] / 2.0e8);
         int rc;
-        const double* sq = nullptr;
-        if ((rc = dev_upload(p, &sq, h.data(), (size_t)E))) return rc;
-        p->ice_sq = const_cast<double*>(sq);
-        if ((rc = dev_alloc(p, &p->ice_work, ice_workspace_doubles(p->m.N)))) return rc;
+        double* sq = nullptr;
+        double* tab = nullptr;
+        double* work = nullptr;
+        if ((rc = dev_alloc(p, &sq, (size_t)E)) ||
+            (rc = dev_alloc(p, &tab, ice_table_doubles(E))) ||
+            (rc = dev_alloc(p, &work, ice_workspace_doubles(p->m.N))))
+            return rc;
+        HIPCHK(hipMemcpy(sq, h.data(), (size_t)E * sizeof(double), hipMemcpyHostToDevice));
+        // built once, on the null stream, and waited for: later calls may use any stream
+        if ((rc = launch_ice_tables(p->m, sq, tab, nullptr))) return rc;
+        HIPCHK(hipDeviceSynchronize());
+        p->ice_tab = tab;
+        p->ice_work = work;
     }
     IceCall c;
     c.dt = a->ice_dt;
@@ -896,7 +905,7 @@ int fct_ice_fct_solve(fct_plan* p, const fct_ice_args* a, void* stream) {
     c.v = a->v_ice;
     c.X[0] = a->a_ice; c.X[1] = a->m_ice; c.X[2] = a->m_snow;
     c.out[0] = a->a_new; c.out[1] = a->m_new; c.out[2] = a->ms_new;
-    return launch_ice_fct(p->m, c, p->ice_work, p->ice_sq, (hipStream_t)stream);
+    return launch_ice_fct(p->m, c, p->ice_work, p->ice_tab, (hipStream_t)stream);
 }
 
 int fct_compute_w(fct_plan* p, const double* uv, const double* helem, double* w,

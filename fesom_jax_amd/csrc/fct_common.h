@@ -287,8 +287,12 @@ struct IceCall {
     double* out[3] = {nullptr, nullptr, nullptr};
 };
 size_t ice_workspace_doubles(int N);
-// sq[e] = sqrt(elem_area[e] / 2e8), computed on the host (correctly rounded)
-int launch_ice_fct(const DevMesh& m, const IceCall& c, double* work, const double* sq,
+size_t ice_table_doubles(int E);
+// the per-element table of the sea-ice passes (vertices, open flag, gradient
+// coefficients, area, sq[e] = sqrt(elem_area[e] / 2e8) computed on the host, correctly
+// rounded), built once per plan into `tab`
+int launch_ice_tables(const DevMesh& m, const double* sq, double* tab, hipStream_t s);
+int launch_ice_fct(const DevMesh& m, const IceCall& c, double* work, const double* tab,
                    hipStream_t s);
 
 // Vertical velocity from continuity (fct_ale.hip).

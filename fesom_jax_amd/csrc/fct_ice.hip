@@ -6,10 +6,18 @@
 // the tracers share every geometry load). Each node gathers over its elements in
 // increasing element id (the node -> element CSR), which is the order of the
 // reference's scatter-add over (element, vertex); sums over an element's three
-// vertices run ((x0 + x1) + x2). Five launches, each a gather pass over the elements
-// around every node: right-hand side + low-order solve, two high-order corrections (the
-// second also stores each node's gamma * X + d, the vertex value of the antidiffusive
-// fluxes), fluxes + bounds + ratios, limited fluxes + update.
+// vertices run ((x0 + x1) + x2). Six launches: the inputs packed per node, then five
+// gather passes over the elements around every node: right-hand side + low-order
+// solve, two high-order corrections (the second also stores each node's gamma * X + d,
+// the vertex value of the antidiffusive fluxes), fluxes + bounds + ratios, limited
+// fluxes + update.
+//
+// Layout: every gather is per-lane scattered, so the passes are bound by the number
+// of gather instructions. The three tracers of a node sit together (dv4, the fourth
+// slot carrying the node's inverse area where a pass needs it), (u, v) as a pair, and
+// an element's vertices, open flag, gradient coefficients, volume and diffusion factor
+// in one iv4 and four dv2: a vertex costs two loads for all three tracers instead
+// of three, an element five instead of twelve.
 
 #include <hip/hip_runtime.h>
 
@@ -23,45 +31,40 @@ using gp = __attribute__((address_space(1))) T*;
 template <class T>
 __device__ __forceinline__ gp<T> glb(T* p) { return (gp<T>)p; }
 
+// packed per-node and per-element values as clang vectors (HIP's vector types are
+// classes, which do not load through address-space pointers)
+typedef double dv2 __attribute__((ext_vector_type(2)));
+typedef double dv4 __attribute__((ext_vector_type(4)));
+typedef int iv4 __attribute__((ext_vector_type(4)));
+
 constexpr double ICE_FLUX_EPS = 1.0e-12;   // ice_adv.py:32
 constexpr int ICE_K = 3;                   // a_ice, m_ice, m_snow
 
 struct IceArgs {
     int N;
-    int L;
     double dt, diff, gamma;
     const int* ncsr_off;
     const int* ncsr;
-    const int* en;          // [E*3]
-    const double* gs;       // [E*6]
-    const double* vol;      // [E]
-    const double* sq;       // [E] sqrt(vol / 2e8)
-    const int* elev;        // [E*2] (ulevels-1, nlevels-1)
+    const iv4* el;          // [E] (v0, v1, v2, 1 if the element is open: ulevels <= 1)
+    const dv2* eg;          // [E][4] (dx0, dx1), (dx2, dy0), (dy1, dy2), (vol, sqrt(vol/2e8))
     const int* nlev;        // [N*2] (ulevels_nod2D-1, nlevels_nod2D-1)
-    const double* area;     // [N*L]; the surface column is area[n*L]
-    const double* u;
-    const double* v;
-    const double* X[ICE_K];     // tracers in
-    double* out[ICE_K];         // tracers out
-    double* rhs;            // [3][N]
-    double* xl;             // [3][N] low-order solution
-    double* d0;             // [3][N] high-order increment (ping)
-    double* d1;             // [3][N] (pong)
+    const dv4* X;           // [N] (a_ice, m_ice, m_snow, 0), packed from the inputs
+    const dv2* uv;          // [N] (u, v)
+    double* out[ICE_K];     // tracers out
+    dv4* rhs;               // [N] right-hand side
+    dv4* xl;                // [N] low-order solution; .w = 1 / surface area
+    dv4* d0;                // [N] high-order increment (ping)
+    dv4* d1;                // [N] (pong)
+    dv4* w;                 // [N] gamma * X + d with the final d; .w = 1 / surface area
     double* pp;             // [3][N] ratios
     double* pm;
-    double* inv;            // [N] 1 / surface area (k_ice_rhs), as inv_area computes it
-    double* w;              // [3][N] gamma * X + d with the final d (second k_ice_hi)
 };
 
-__device__ __forceinline__ double inv_area(const IceArgs& a, int n) {
-    const double ar = glb(a.area)[(size_t)n * a.L];
-    return 1.0 / (ar > 0 ? ar : 1.0);
+__device__ __forceinline__ double comp(const dv4& v, int t) {
+    return t == 0 ? v.x : (t == 1 ? v.y : v.z);
 }
 __device__ __forceinline__ bool node_open(const IceArgs& a, int n) {
     return glb(a.nlev)[2 * n] == 0;            // ulevels_nod2D <= 1
-}
-__device__ __forceinline__ bool elem_open(const IceArgs& a, int e) {
-    return glb(a.elev)[2 * e] == 0;            // ulevels <= 1
 }
 __device__ __forceinline__ int slot_of(int n, int v0, int v1) { return v0 == n ? 0 : (v1 == n ? 1 : 2); }
 __device__ __forceinline__ double sum3(double x0, double x1, double x2) { return (x0 + x1) + x2; }
@@ -72,31 +75,67 @@ __device__ __forceinline__ double mm_term(double vol, double xq, double x0, doub
     return vol / 12.0 * (xq + sum3(x0, x1, x2));
 }
 
+// element tables, once per plan: vertices and open flag; gradient coefficients, volume
+// and the diffusion factor sq (computed on the host)
+__global__ void __launch_bounds__(256) k_ice_tables(int E, const int* __restrict__ en,
+                                                    const int2* __restrict__ elev,
+                                                    const double* __restrict__ gs,
+                                                    const double* __restrict__ vol,
+                                                    const double* __restrict__ sq,
+                                                    iv4* __restrict__ el, dv2* __restrict__ eg) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    el[e] = iv4{en[3 * e], en[3 * e + 1], en[3 * e + 2], elev[e].x == 0 ? 1 : 0};
+    eg[4 * e] = dv2{gs[6 * e], gs[6 * e + 1]};
+    eg[4 * e + 1] = dv2{gs[6 * e + 2], gs[6 * e + 3]};
+    eg[4 * e + 2] = dv2{gs[6 * e + 4], gs[6 * e + 5]};
+    eg[4 * e + 3] = dv2{vol[e], sq[e]};
+}
+
+// the inputs packed per node
+__global__ void __launch_bounds__(256) k_ice_pack(int N, const double* __restrict__ a0,
+                                                  const double* __restrict__ a1,
+                                                  const double* __restrict__ a2,
+                                                  const double* __restrict__ u,
+                                                  const double* __restrict__ v,
+                                                  dv4* __restrict__ X, dv2* __restrict__ uv) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    X[n] = dv4{a0[n], a1[n], a2[n], 0.0};
+    uv[n] = dv2{u[n], v[n]};
+}
+
 // pass 1: right-hand side (ice_adv.py:47-73), low-order solution (:76-87) and the first
 // high-order approximation rhs / area (:101)
-__global__ void __launch_bounds__(256) k_ice_rhs(IceArgs a) {
+__global__ void __launch_bounds__(256) k_ice_rhs(IceArgs a, const double* __restrict__ area,
+                                                 int L) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= a.N) return;
-    const auto en = glb(a.en);
-    const auto gs = glb(a.gs);
-    const auto U = glb(a.u), V = glb(a.v);
-    double rhs[ICE_K] = {0.0, 0.0, 0.0}, mm[ICE_K] = {0.0, 0.0, 0.0}, xn[ICE_K];
-    for (int t = 0; t < ICE_K; ++t) xn[t] = glb(a.X[t])[n];
+    const auto UV = glb(a.uv);
+    const auto X = glb(a.X);
+    double rhs[ICE_K] = {0.0, 0.0, 0.0}, mm[ICE_K] = {0.0, 0.0, 0.0};
+    const dv4 xn = X[n];
     const int s1 = glb(a.ncsr_off)[n + 1];
     for (int s = glb(a.ncsr_off)[n]; s < s1; ++s) {
         const int e = glb(a.ncsr)[s];
-        const int v0 = en[3 * e], v1 = en[3 * e + 1], v2 = en[3 * e + 2];
-        const int q = slot_of(n, v0, v1);
-        const bool on = elem_open(a, e);
-        const double vol = glb(a.vol)[e];
-        double dx[3], dy[3];
+        const iv4 ev = glb(a.el)[e];
+        const int vv[3] = {ev.x, ev.y, ev.z};
+        const int q = slot_of(n, vv[0], vv[1]);
+        const bool on = ev.w != 0;
+        const dv2 g0 = glb(a.eg)[4 * e], g1 = glb(a.eg)[4 * e + 1];
+        const dv2 g2 = glb(a.eg)[4 * e + 2], g3 = glb(a.eg)[4 * e + 3];
+        const double dx[3] = {g0.x, g0.y, g1.x}, dy[3] = {g1.y, g2.x, g2.y};
+        const double vol = g3.x;
+        double Uv[3], Vv[3];
+        dv4 xv[3];
         for (int i = 0; i < 3; ++i) {
-            dx[i] = gs[6 * e + i];
-            dy[i] = gs[6 * e + 3 + i];
+            const dv2 w = UV[vv[i]];
+            Uv[i] = w.x;
+            Vv[i] = w.y;
+            xv[i] = X[vv[i]];
         }
-        const double Uv[3] = {U[v0], U[v1], U[v2]}, Vv[3] = {V[v0], V[v1], V[v2]};
         const double um = sum3(Uv[0], Uv[1], Uv[2]), vm = sum3(Vv[0], Vv[1], Vv[2]);
-        const double diff = a.diff * glb(a.sq)[e];
+        const double diff = a.diff * g3.y;
         const double udn_q = um * dx[q] + vm * dy[q];
         double ent[3];
         for (int j = 0; j < 3; ++j) {
@@ -107,56 +146,63 @@ __global__ void __launch_bounds__(256) k_ice_rhs(IceArgs a) {
             ent[j] = vol * a.dt * (ta - tb - tc);
         }
         for (int t = 0; t < ICE_K; ++t) {
-            const auto X = glb(a.X[t]);
-            const double x0 = X[v0], x1 = X[v1], x2 = X[v2];
+            const double x0 = comp(xv[0], t), x1 = comp(xv[1], t), x2 = comp(xv[2], t);
             const double c = (ent[0] * x0 + ent[1] * x1) + ent[2] * x2;
             const double xq = q == 0 ? x0 : (q == 1 ? x1 : x2);
             rhs[t] = rhs[t] + (on ? c : 0.0);
             mm[t] = mm[t] + (on ? mm_term(vol, xq, x0, x1, x2) : 0.0);
         }
     }
-    const double inv = inv_area(a, n);
-    glb(a.inv)[n] = inv;
+    const double ar = glb(area)[(size_t)n * L];
+    const double inv = 1.0 / (ar > 0 ? ar : 1.0);
     const bool nc = node_open(a, n);
+    double xl[ICE_K], d[ICE_K];
     for (int t = 0; t < ICE_K; ++t) {
-        const size_t i = (size_t)t * a.N + n;
-        glb(a.rhs)[i] = rhs[t];
-        glb(a.xl)[i] = nc ? (rhs[t] + a.gamma * mm[t]) * inv + (1.0 - a.gamma) * xn[t] : xn[t];
-        glb(a.d0)[i] = nc ? rhs[t] * inv : 0.0;
+        const double x = comp(xn, t);
+        xl[t] = nc ? (rhs[t] + a.gamma * mm[t]) * inv + (1.0 - a.gamma) * x : x;
+        d[t] = nc ? rhs[t] * inv : 0.0;
     }
+    glb(a.rhs)[n] = dv4{rhs[0], rhs[1], rhs[2], 0.0};
+    glb(a.xl)[n] = dv4{xl[0], xl[1], xl[2], inv};
+    glb(a.d0)[n] = dv4{d[0], d[1], d[2], 0.0};
 }
 
 // pass 2 (twice): d <- d + (rhs - mm d) / area (ice_adv.py:102-104); the second pass also
 // stores gamma * X + d (wout), the vertex value of the antidiffusive fluxes
-__global__ void __launch_bounds__(256) k_ice_hi(IceArgs a, const double* din, double* dout,
-                                                double* wout) {
+__global__ void __launch_bounds__(256) k_ice_hi(IceArgs a, const dv4* din, dv4* dout,
+                                                dv4* wout) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= a.N) return;
-    const auto en = glb(a.en);
     const auto D = glb(din);
     double mm[ICE_K] = {0.0, 0.0, 0.0};
     const int s1 = glb(a.ncsr_off)[n + 1];
     for (int s = glb(a.ncsr_off)[n]; s < s1; ++s) {
         const int e = glb(a.ncsr)[s];
-        const int v0 = en[3 * e], v1 = en[3 * e + 1], v2 = en[3 * e + 2];
-        const int q = slot_of(n, v0, v1);
-        const bool on = elem_open(a, e);
-        const double vol = glb(a.vol)[e];
+        const iv4 ev = glb(a.el)[e];
+        const int q = slot_of(n, ev.x, ev.y);
+        const bool on = ev.w != 0;
+        const double vol = glb(a.eg)[4 * e + 3].x;
+        const dv4 d0 = D[ev.x], d1 = D[ev.y], d2 = D[ev.z];
         for (int t = 0; t < ICE_K; ++t) {
-            const size_t o = (size_t)t * a.N;
-            const double x0 = D[o + v0], x1 = D[o + v1], x2 = D[o + v2];
+            const double x0 = comp(d0, t), x1 = comp(d1, t), x2 = comp(d2, t);
             const double xq = q == 0 ? x0 : (q == 1 ? x1 : x2);
             mm[t] = mm[t] + (on ? mm_term(vol, xq, x0, x1, x2) : 0.0);
         }
     }
-    const double inv = inv_area(a, n);
+    const dv4 xl = glb(a.xl)[n];
+    const double inv = xl.w;
     const bool nc = node_open(a, n);
+    const dv4 dv = D[n], r = glb(a.rhs)[n];
+    double dn[ICE_K];
     for (int t = 0; t < ICE_K; ++t) {
-        const size_t i = (size_t)t * a.N + n;
-        const double d = D[i];
-        const double dn = nc ? d + (glb(a.rhs)[i] - mm[t]) * inv : d;
-        glb(dout)[i] = dn;
-        if (wout) glb(wout)[i] = a.gamma * glb(a.X[t])[n] + dn;
+        const double d = comp(dv, t);
+        dn[t] = nc ? d + (comp(r, t) - mm[t]) * inv : d;
+    }
+    glb(dout)[n] = dv4{dn[0], dn[1], dn[2], 0.0};
+    if (wout) {
+        const dv4 x = glb(a.X)[n];
+        glb(wout)[n] = dv4{a.gamma * x.x + dn[0], a.gamma * x.y + dn[1],
+                           a.gamma * x.z + dn[2], inv};
     }
 }
 
@@ -180,27 +226,33 @@ __device__ __forceinline__ bool ice_flux_nonneg(double g) { return g >= -0x6p-10
 __global__ void __launch_bounds__(256) k_ice_ratio(IceArgs a) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= a.N) return;
-    const auto en = glb(a.en);
     const auto W = glb(a.w);
     const auto XL = glb(a.xl);
-    const double inv = inv_area(a, n);
+    const auto X = glb(a.X);
+    const dv4 xln = XL[n];
+    const double inv = xln.w;
     double fp[ICE_K] = {0.0, 0.0, 0.0}, fm[ICE_K] = {0.0, 0.0, 0.0};
     double hi[ICE_K], lo[ICE_K];
     for (int t = 0; t < ICE_K; ++t) { hi[t] = -INFINITY; lo[t] = INFINITY; }
     const int s1 = glb(a.ncsr_off)[n + 1];
     for (int s = glb(a.ncsr_off)[n]; s < s1; ++s) {
         const int e = glb(a.ncsr)[s];
-        const int vv[3] = {en[3 * e], en[3 * e + 1], en[3 * e + 2]};
+        const iv4 ev = glb(a.el)[e];
+        const int vv[3] = {ev.x, ev.y, ev.z};
         const int q = slot_of(n, vv[0], vv[1]);
-        const bool on = elem_open(a, e);
-        const double vol = glb(a.vol)[e];
+        const bool on = ev.w != 0;
+        const double vol = glb(a.eg)[4 * e + 3].x;
+        dv4 xv[3], lv[3], wv[3];
+        for (int j = 0; j < 3; ++j) {
+            xv[j] = X[vv[j]];
+            lv[j] = XL[vv[j]];
+            wv[j] = W[vv[j]];
+        }
         for (int t = 0; t < ICE_K; ++t) {
-            const size_t o = (size_t)t * a.N;
-            const auto X = glb(a.X[t]);
             double w[3], eh = -INFINITY, el = INFINITY;
             for (int j = 0; j < 3; ++j) {
-                const double x = X[vv[j]], l = XL[o + vv[j]];
-                w[j] = W[o + vv[j]];
+                const double x = comp(xv[j], t), l = comp(lv[j], t);
+                w[j] = comp(wv[j], t);
                 eh = fmax(eh, fmax(l, x));
                 el = fmin(el, fmin(l, x));
             }
@@ -217,7 +269,7 @@ __global__ void __launch_bounds__(256) k_ice_ratio(IceArgs a) {
     for (int t = 0; t < ICE_K; ++t) {
         const size_t i = (size_t)t * a.N + n;
         const double h = isfinite(hi[t]) ? hi[t] : 0.0, l = isfinite(lo[t]) ? lo[t] : 0.0;
-        const double xl = XL[i];
+        const double xl = comp(xln, t);
         const double tmax = h - xl, tmin = l - xl;
         double p = fp[t] > 0.0 ? min1(tmax / (fp[t] > ICE_FLUX_EPS ? fp[t] : ICE_FLUX_EPS)) : 0.0;
         double m = fm[t] < 0.0 ? min1(tmin / (fm[t] < -ICE_FLUX_EPS ? fm[t] : -ICE_FLUX_EPS)) : 0.0;
@@ -231,79 +283,84 @@ __global__ void __launch_bounds__(256) k_ice_ratio(IceArgs a) {
 __global__ void __launch_bounds__(256) k_ice_apply(IceArgs a) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= a.N) return;
-    const auto en = glb(a.en);
     const auto W = glb(a.w);
     const auto PP = glb(a.pp), PM = glb(a.pm);
     double acc[ICE_K] = {0.0, 0.0, 0.0};
     const int s1 = glb(a.ncsr_off)[n + 1];
     for (int s = glb(a.ncsr_off)[n]; s < s1; ++s) {
         const int e = glb(a.ncsr)[s];
-        const int vv[3] = {en[3 * e], en[3 * e + 1], en[3 * e + 2]};
+        const iv4 ev = glb(a.el)[e];
+        const int vv[3] = {ev.x, ev.y, ev.z};
         const int q = slot_of(n, vv[0], vv[1]);
-        const bool on = elem_open(a, e);
-        const double vol = glb(a.vol)[e];
-        double invj[3];
-        for (int j = 0; j < 3; ++j) invj[j] = glb(a.inv)[vv[j]];
+        const bool on = ev.w != 0;
+        const double vol = glb(a.eg)[4 * e + 3].x;
+        // the vertices' gamma * X + d, with their inverse areas in the fourth slot
+        dv4 wv[3];
+        for (int j = 0; j < 3; ++j) wv[j] = W[vv[j]];
         for (int t = 0; t < ICE_K; ++t) {
             const size_t o = (size_t)t * a.N;
             double w[3];
-            for (int j = 0; j < 3; ++j) w[j] = W[o + vv[j]];
+            for (int j = 0; j < 3; ++j) w[j] = comp(wv[j], t);
             // the element's factor takes each vertex's ratio by the sign of that vertex's
             // flux (0 when the element is closed); only the node's own flux is divided
             double ae = INFINITY, gq = 0.0;
             for (int j = 0; j < 3; ++j) {
-                const double g = ice_flux_num(w, j, vol, invj[j]);
+                const double g = ice_flux_num(w, j, vol, wv[j].w);
                 gq = j == q ? g : gq;
                 ae = fmin(ae, !on || ice_flux_nonneg(g) ? PP[o + vv[j]] : PM[o + vv[j]]);
             }
             acc[t] = acc[t] + (on ? (gq / 12.0) * ae : 0.0);
         }
     }
-    for (int t = 0; t < ICE_K; ++t) {
-        const size_t i = (size_t)t * a.N + n;
-        glb(a.out[t])[n] = glb(a.xl)[i] + acc[t];
-    }
+    const dv4 xl = glb(a.xl)[n];
+    for (int t = 0; t < ICE_K; ++t) glb(a.out[t])[n] = comp(xl, t) + acc[t];
 }
 
 }  // namespace
 
-size_t ice_workspace_doubles(int N) { return (size_t)22 * N; }
+// rhs, xl, d0, d1, w, X as dv4, (u, v) as dv2, pp and pm: 32 doubles per node
+size_t ice_workspace_doubles(int N) { return (size_t)32 * N; }
+// el (iv4) and eg (four dv2): 10 doubles per element
+size_t ice_table_doubles(int E) { return (size_t)10 * E; }
 
-int launch_ice_fct(const DevMesh& m, const IceCall& c, double* work, const double* sq,
+int launch_ice_tables(const DevMesh& m, const double* sq, double* tab, hipStream_t s) {
+    if (m.E == 0) return FCT_OK;
+    iv4* el = reinterpret_cast<iv4*>(tab);
+    dv2* eg = reinterpret_cast<dv2*>(tab + 2 * (size_t)m.E);
+    k_ice_tables<<<(m.E + 255) / 256, 256, 0, s>>>(m.E, m.en, m.elev, m.gs, m.earea, sq, el, eg);
+    return check_launch("k_ice_tables");
+}
+
+int launch_ice_fct(const DevMesh& m, const IceCall& c, double* work, const double* tab,
                    hipStream_t s) {
     if (m.N == 0) return FCT_OK;
     IceArgs a{};
     a.N = m.N;
-    a.L = m.L;
     a.dt = c.dt;
     a.diff = c.diff;
     a.gamma = c.gamma;
     a.ncsr_off = m.ncsr_off;
     a.ncsr = m.ncsr;
-    a.en = m.en;
-    a.gs = m.gs;
-    a.vol = m.earea;
-    a.sq = sq;
-    a.elev = reinterpret_cast<const int*>(m.elev);
+    a.el = reinterpret_cast<const iv4*>(tab);
+    a.eg = reinterpret_cast<const dv2*>(tab + 2 * (size_t)m.E);
     a.nlev = reinterpret_cast<const int*>(m.nlev);
-    a.area = m.area;
-    a.u = c.u;
-    a.v = c.v;
-    for (int t = 0; t < ICE_K; ++t) {
-        a.X[t] = c.X[t];
-        a.out[t] = c.out[t];
-    }
-    const size_t N3 = (size_t)3 * m.N;
-    a.rhs = work;
-    a.xl = work + N3;
-    a.d0 = work + 2 * N3;
-    a.d1 = work + 3 * N3;
-    a.pp = work + 4 * N3;
-    a.pm = work + 5 * N3;
-    a.inv = work + 6 * N3;
-    a.w = work + 6 * N3 + m.N;
+    for (int t = 0; t < ICE_K; ++t) a.out[t] = c.out[t];
+    const size_t N = (size_t)m.N;
+    dv4* w4 = reinterpret_cast<dv4*>(work);
+    a.rhs = w4;
+    a.xl = w4 + N;
+    a.d0 = w4 + 2 * N;
+    a.d1 = w4 + 3 * N;
+    a.w = w4 + 4 * N;
+    dv4* X = w4 + 5 * N;
+    dv2* uv = reinterpret_cast<dv2*>(work + 24 * N);
+    a.X = X;
+    a.uv = uv;
+    a.pp = work + 26 * N;
+    a.pm = work + 29 * N;
     const dim3 grid((m.N + 255) / 256), block(256);
-    k_ice_rhs<<<grid, block, 0, s>>>(a);
+    k_ice_pack<<<grid, block, 0, s>>>(m.N, c.X[0], c.X[1], c.X[2], c.u, c.v, X, uv);
+    k_ice_rhs<<<grid, block, 0, s>>>(a, m.area, m.L);
     k_ice_hi<<<grid, block, 0, s>>>(a, a.d0, a.d1, nullptr);
     k_ice_hi<<<grid, block, 0, s>>>(a, a.d1, a.d0, a.w);
     k_ice_ratio<<<grid, block, 0, s>>>(a);
