@@ -211,8 +211,9 @@ int fct_forward_upwind(fct_plan* plan, const fct_step_args* args, void* stream);
  * (fesom_jax/ice_adv.py:177-194, exch=None): Taylor-Galerkin right-hand side,
  * consistent-mass low- and high-order solves and the element Zalesak limiter of the
  * three ice tracers on the surface nodes. All fields [nod2D], device; outputs must not
- * alias inputs. The first call allocates 22 * nod2D doubles of workspace and one factor
- * per element; later calls allocate nothing. */
+ * alias inputs. The first call allocates 32 * nod2D doubles of workspace and 11 doubles
+ * per element (the packed element table and its diffusion factor), builds the table and
+ * waits for the device to finish it; later calls allocate nothing and do not wait. */
 typedef struct fct_ice_args {
     double ice_dt;                    /* IceConfig.ice_dt (fesom_jax/ice.py:62) */
     double ice_diff;                  /* IceConfig.ice_diff (:59) */
