@@ -323,7 +323,9 @@ class Server:
             else:
                 self.pool[key].remove(plan)
             self._forget(plan)
-            plan.close()
+            tt = time.perf_counter()
+            plan.close()          # frees device memory: waits for the device's queued work
+            self.timing["release"] += time.perf_counter() - tt
             self.counts["plans_closed"] += 1
         in_use = {(a.spec.level, a.spec.nl) for a in list(active) + list(busy)}
         if keep_base is not None:
@@ -333,7 +335,9 @@ class Server:
                 break
             if key in in_use:
                 continue
+            tt = time.perf_counter()
             _drop_base_case(self.meshes.get(*key), self.device, self.dt)
+            self.timing["release"] += time.perf_counter() - tt
             del self.base_bytes[key]
             self.counts["base_dropped"] += 1
         return self.held_bytes(active) + need_now() <= self.budget
@@ -399,7 +403,9 @@ class Server:
             raise
         extra = {"uv": case["uv"], "target": case.get("target")}
         import torch
-        torch.cuda.synchronize(self.device)
+        # the request's inputs are ready once the main stream's queued work is; the
+        # finishing stream's adjoints keep running meanwhile
+        torch.cuda.current_stream(self.device).synchronize()
         nbytes = estimate_field_bytes(mesh, spec.K, spec.nsteps, spec.adjoint)
         if spec.adjoint:
             self._adjoint_plan(mesh, spec, share=plan)
