@@ -70,6 +70,31 @@ __device__ __forceinline__ int slot_of(int n, int v0, int v1) { return v0 == n ?
 __device__ __forceinline__ double sum3(double x0, double x1, double x2) { return (x0 + x1) + x2; }
 __device__ __forceinline__ double min1(double x) { return x > 1.0 ? 1.0 : x; }   // jnp.minimum(x, 1)
 
+// The elements of node n in CSR order (increasing id). Each element's table entry is
+// fetched one element ahead and its CSR entry two ahead, so that an element's vertex
+// gathers need not wait behind the chain CSR entry -> element entry.
+struct ElemWalk {
+    int s, s1, e_next, e_after;
+    iv4 ev_next;
+    __device__ __forceinline__ ElemWalk(const IceArgs& a, int n) {
+        s = glb(a.ncsr_off)[n];
+        s1 = glb(a.ncsr_off)[n + 1];
+        e_next = s < s1 ? glb(a.ncsr)[s] : 0;
+        e_after = s + 1 < s1 ? glb(a.ncsr)[s + 1] : 0;
+        ev_next = s < s1 ? glb(a.el)[e_next] : iv4{0, 0, 0, 0};
+    }
+    __device__ __forceinline__ bool next(const IceArgs& a, int& e, iv4& ev) {
+        if (s >= s1) return false;
+        e = e_next;
+        ev = ev_next;
+        e_next = e_after;
+        if (s + 1 < s1) ev_next = glb(a.el)[e_next];
+        if (s + 2 < s1) e_after = glb(a.ncsr)[s + 2];
+        ++s;
+        return true;
+    }
+};
+
 // (mm X)[n]'s term of element e: vol/12 (X[n] + sum of the vertices)
 __device__ __forceinline__ double mm_term(double vol, double xq, double x0, double x1, double x2) {
     return vol / 12.0 * (xq + sum3(x0, x1, x2));
@@ -115,10 +140,10 @@ __global__ void __launch_bounds__(256) k_ice_rhs(IceArgs a, const double* __rest
     const auto X = glb(a.X);
     double rhs[ICE_K] = {0.0, 0.0, 0.0}, mm[ICE_K] = {0.0, 0.0, 0.0};
     const dv4 xn = X[n];
-    const int s1 = glb(a.ncsr_off)[n + 1];
-    for (int s = glb(a.ncsr_off)[n]; s < s1; ++s) {
-        const int e = glb(a.ncsr)[s];
-        const iv4 ev = glb(a.el)[e];
+    ElemWalk wk(a, n);
+    int e;
+    iv4 ev;
+    while (wk.next(a, e, ev)) {
         const int vv[3] = {ev.x, ev.y, ev.z};
         const int q = slot_of(n, vv[0], vv[1]);
         const bool on = ev.w != 0;
@@ -175,10 +200,10 @@ __global__ void __launch_bounds__(256) k_ice_hi(IceArgs a, const dv4* din, dv4* 
     if (n >= a.N) return;
     const auto D = glb(din);
     double mm[ICE_K] = {0.0, 0.0, 0.0};
-    const int s1 = glb(a.ncsr_off)[n + 1];
-    for (int s = glb(a.ncsr_off)[n]; s < s1; ++s) {
-        const int e = glb(a.ncsr)[s];
-        const iv4 ev = glb(a.el)[e];
+    ElemWalk wk(a, n);
+    int e;
+    iv4 ev;
+    while (wk.next(a, e, ev)) {
         const int q = slot_of(n, ev.x, ev.y);
         const bool on = ev.w != 0;
         const double vol = glb(a.eg)[4 * e + 3].x;
@@ -234,10 +259,10 @@ __global__ void __launch_bounds__(256) k_ice_ratio(IceArgs a) {
     double fp[ICE_K] = {0.0, 0.0, 0.0}, fm[ICE_K] = {0.0, 0.0, 0.0};
     double hi[ICE_K], lo[ICE_K];
     for (int t = 0; t < ICE_K; ++t) { hi[t] = -INFINITY; lo[t] = INFINITY; }
-    const int s1 = glb(a.ncsr_off)[n + 1];
-    for (int s = glb(a.ncsr_off)[n]; s < s1; ++s) {
-        const int e = glb(a.ncsr)[s];
-        const iv4 ev = glb(a.el)[e];
+    ElemWalk wk(a, n);
+    int e;
+    iv4 ev;
+    while (wk.next(a, e, ev)) {
         const int vv[3] = {ev.x, ev.y, ev.z};
         const int q = slot_of(n, vv[0], vv[1]);
         const bool on = ev.w != 0;
@@ -286,10 +311,10 @@ __global__ void __launch_bounds__(256) k_ice_apply(IceArgs a) {
     const auto W = glb(a.w);
     const auto PP = glb(a.pp), PM = glb(a.pm);
     double acc[ICE_K] = {0.0, 0.0, 0.0};
-    const int s1 = glb(a.ncsr_off)[n + 1];
-    for (int s = glb(a.ncsr_off)[n]; s < s1; ++s) {
-        const int e = glb(a.ncsr)[s];
-        const iv4 ev = glb(a.el)[e];
+    ElemWalk wk(a, n);
+    int e;
+    iv4 ev;
+    while (wk.next(a, e, ev)) {
         const int vv[3] = {ev.x, ev.y, ev.z};
         const int q = slot_of(n, vv[0], vv[1]);
         const bool on = ev.w != 0;
