@@ -200,14 +200,27 @@ __global__ void __launch_bounds__(256) k_ice_hi(IceArgs a, const dv4* din, dv4* 
     if (n >= a.N) return;
     const auto D = glb(din);
     double mm[ICE_K] = {0.0, 0.0, 0.0};
+    // one element ahead: its area and vertex values load while this one is summed
     ElemWalk wk(a, n);
     int e;
     iv4 ev;
-    while (wk.next(a, e, ev)) {
+    bool more = wk.next(a, e, ev);
+    double vol_n = 0.0;
+    dv4 dn0 = {}, dn1 = {}, dn2 = {};
+    if (more) {
+        vol_n = glb(a.eg)[4 * e + 3].x;
+        dn0 = D[ev.x]; dn1 = D[ev.y]; dn2 = D[ev.z];
+    }
+    while (more) {
         const int q = slot_of(n, ev.x, ev.y);
         const bool on = ev.w != 0;
-        const double vol = glb(a.eg)[4 * e + 3].x;
-        const dv4 d0 = D[ev.x], d1 = D[ev.y], d2 = D[ev.z];
+        const double vol = vol_n;
+        const dv4 d0 = dn0, d1 = dn1, d2 = dn2;
+        more = wk.next(a, e, ev);
+        if (more) {
+            vol_n = glb(a.eg)[4 * e + 3].x;
+            dn0 = D[ev.x]; dn1 = D[ev.y]; dn2 = D[ev.z];
+        }
         for (int t = 0; t < ICE_K; ++t) {
             const double x0 = comp(d0, t), x1 = comp(d1, t), x2 = comp(d2, t);
             const double xq = q == 0 ? x0 : (q == 1 ? x1 : x2);
@@ -311,17 +324,29 @@ __global__ void __launch_bounds__(256) k_ice_apply(IceArgs a) {
     const auto W = glb(a.w);
     const auto PP = glb(a.pp), PM = glb(a.pm);
     double acc[ICE_K] = {0.0, 0.0, 0.0};
+    // one element ahead: its area and vertex values load while this one is summed
     ElemWalk wk(a, n);
     int e;
     iv4 ev;
-    while (wk.next(a, e, ev)) {
+    bool more = wk.next(a, e, ev);
+    double vol_n = 0.0;
+    dv4 wn[3] = {};
+    if (more) {
+        vol_n = glb(a.eg)[4 * e + 3].x;
+        wn[0] = W[ev.x]; wn[1] = W[ev.y]; wn[2] = W[ev.z];
+    }
+    while (more) {
         const int vv[3] = {ev.x, ev.y, ev.z};
         const int q = slot_of(n, vv[0], vv[1]);
         const bool on = ev.w != 0;
-        const double vol = glb(a.eg)[4 * e + 3].x;
+        const double vol = vol_n;
         // the vertices' gamma * X + d, with their inverse areas in the fourth slot
-        dv4 wv[3];
-        for (int j = 0; j < 3; ++j) wv[j] = W[vv[j]];
+        const dv4 wv[3] = {wn[0], wn[1], wn[2]};
+        more = wk.next(a, e, ev);
+        if (more) {
+            vol_n = glb(a.eg)[4 * e + 3].x;
+            wn[0] = W[ev.x]; wn[1] = W[ev.y]; wn[2] = W[ev.z];
+        }
         for (int t = 0; t < ICE_K; ++t) {
             const size_t o = (size_t)t * a.N;
             double w[3];
