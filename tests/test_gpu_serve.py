@@ -90,19 +90,22 @@ def test_budget_counts_every_live_plan(torch_dev):
     big = max(estimate_bytes(meshes.get(s.level, s.nl), s.K, s.nsteps, s.adjoint) for s in specs)
     budget = int(2.5 * big)
     srv = Server(torch_dev, budget_bytes=budget, max_active=8, meshes=meshes)
-    srv.run(specs)
+    summary = srv.run(specs)
     assert not srv.errors and len(srv.latency) == len(specs)
+    assert "over_budget" not in summary, summary.get("over_budget")
     pooled = [p for ps in srv.pool.values() for p in ps] + list(srv.adj_plans.values())
     assert pooled
     tables = dict(p.tables() for p in pooled)      # each mesh's tables once
     assert srv.held_bytes([]) == (sum(p.own_bytes() for p in pooled) + sum(tables.values())
                                   + sum(srv.base_bytes.values()))
-    # plans of one mesh share one table set
+    # plans of one mesh share one table set, the mesh's adjoint plan included
     per_mesh = {}
     for key, ps in srv.pool.items():
         for p in ps:
             per_mesh.setdefault(key[:2], set()).add(p.tables()[0])
-    assert all(len(ids) == 1 for ids in per_mesh.values())
+    for key, p in srv.adj_plans.items():
+        per_mesh.setdefault(key, set()).add(p.tables()[0])
+    assert all(len(ids) == 1 for ids in per_mesh.values()), per_mesh
     assert any(s.adjoint for s in specs)
     assert srv.peak_bytes <= budget
     srv.close()
