@@ -172,6 +172,30 @@ def test_adjoint_plan_shares_the_admitted_requests_tables(monkeypatch):
     assert len(srv.table_sets) == 1 and srv.held_bytes([]) == 7 + 50 + srv.plan_bytes[id(adj)]
 
 
+def test_over_budget_snapshot_is_plain_data():
+    """The snapshot of the first over-budget moment names every held part and the last
+    admissions, and goes into the JSON summary as it is."""
+    import json
+    srv = _server(100)
+    idle, adj = _FakePlan(30, (1, 20)), _FakePlan(40, (1, 20))
+    srv.pool[(5, 47, 2)].append(idle)
+    srv.adj_plans[(5, 47)] = adj
+    srv._refresh(idle)
+    srv._refresh(adj)
+    srv.base_bytes = {(5, 47): 15}
+    a = _Active(5, 47, 10)
+    a.spec.rid = 7
+    own = _FakePlan(25, (1, 20))
+    srv._refresh(own)
+    a.inst = type("Inst", (), {"plan": own})()
+    srv._admissions.append((7, 5, 47, 2, False, 10, 105, 115))
+    snap = srv._snapshot([a], srv.held_bytes([a]))
+    assert snap["held"] == 30 + 40 + 25 + 20 + 15 + 10
+    assert snap["plans"] == {"idle (5, 47, 2)": 30, "adjoint (5, 47)": 40, "active 7": 25}
+    assert snap["active"] == {7: 10} and snap["admissions"][-1][0] == 7
+    json.loads(json.dumps(snap))
+
+
 def test_base_inputs_go_with_their_mesh(monkeypatch):
     """The base-input cache never hands a mesh another mesh's inputs, even when the
     other mesh's id is reused, and drops an entry when its mesh is collected."""
