@@ -646,8 +646,10 @@ class Server:
                 if ev.query():
                     bad = bool(host.pop("_bad"))
                     if host.pop("_deferred", False):
+                        tt = time.perf_counter()
                         host = {k: ([x.cpu() for x in v] if isinstance(v, list) else v.cpu())
                                 for k, v in keep.items()}
+                        self.timing["download"] += time.perf_counter() - tt
                     if bad:
                         self._fail(a.spec, t0, "non-finite result", log)
                     else:
