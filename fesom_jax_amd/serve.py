@@ -153,6 +153,22 @@ class _Active:
     done_steps: int = 0
 
 
+def staged_to_host(src, stage, wait):
+    """A copy of the 1-D tensor ``src`` in host memory, made through ``stage`` chunk by
+    chunk: each chunk goes into the staging tensor (pinned host memory on a GPU, where
+    the copy engine fills it at full speed), ``wait()`` lets that copy finish, and the
+    host copies the chunk out. Returns a CPU tensor of ``src``'s dtype."""
+    import torch
+    out = torch.empty(src.numel(), dtype=src.dtype)
+    cs = stage.numel()
+    for off in range(0, src.numel(), cs):
+        m = min(cs, src.numel() - off)
+        stage[:m].copy_(src[off:off + m], non_blocking=True)
+        wait()
+        out[off:off + m].copy_(stage[:m])
+    return out
+
+
 class Server:
     """Continuous-batching loop; see module docstring.
 
@@ -179,6 +195,9 @@ class Server:
         self.backfill_s = float(backfill_s)   # longest wait of the oldest request that
         self.lookahead = int(lookahead)       # later arrivals (this many) may overtake
         self.pinned_bytes = 1 << 30           # results up to this size return pinned
+        self.stage_elems = 1 << 25            # larger ones through a 256 MiB pinned stage
+        self._stage = None
+        self._dl_stream = None
         self.budget = int(budget_bytes)
         self.max_active = int(max_active)
         self.meshes = meshes or MeshCache()
@@ -647,8 +666,8 @@ class Server:
                     bad = bool(host.pop("_bad"))
                     if host.pop("_deferred", False):
                         tt = time.perf_counter()
-                        host = {k: ([x.cpu() for x in v] if isinstance(v, list) else v.cpu())
-                                for k, v in keep.items()}
+                        host = {k: ([self._to_host(x) for x in v] if isinstance(v, list)
+                                    else self._to_host(v)) for k, v in keep.items()}
                         self.timing["download"] += time.perf_counter() - tt
                     if bad:
                         self._fail(a.spec, t0, "non-finite result", log)
@@ -697,6 +716,21 @@ class Server:
         out["counts"] = dict(self.counts)
         return out
 
+    def _to_host(self, x):
+        """A finished large result in host memory, through the server's pinned staging
+        buffer on a stream of its own (the steps queued on the main stream and the
+        adjoints on the finishing stream are not waited for)."""
+        import torch
+        if self._stage is None:
+            self._stage = torch.empty(self.stage_elems, dtype=torch.float64, pin_memory=True)
+            self._dl_stream = torch.cuda.Stream(self.device)      # default priority
+        src = x.reshape(-1)
+        if src.dtype != torch.float64:
+            return x.cpu()
+        with torch.cuda.stream(self._dl_stream):
+            out = staged_to_host(src, self._stage, self._dl_stream.synchronize)
+        return out.reshape(x.shape)
+
     def close(self):
         """Release every pooled plan."""
         for ps in self.pool.values():
@@ -707,6 +741,7 @@ class Server:
         self.pool.clear()
         self.adj_plans.clear()
         self.adj_ready.clear()
+        self._stage = None
         self.plan_bytes.clear()
         self.table_sets.clear()
         self.plan_table.clear()

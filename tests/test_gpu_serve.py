@@ -62,6 +62,10 @@ def test_failures_stay_with_their_request(torch_dev):
     specs[4] = dataclasses.replace(specs[4], inject="fail")
     meshes = MeshCache(seed=12)
     srv = Server(torch_dev, max_active=6, meshes=meshes)
+    # every result comes back through the staged download (large results do), in
+    # chunks smaller than the fields
+    srv.pinned_bytes = 0
+    srv.stage_elems = 1 << 16
     summary = srv.run(specs)
     assert set(srv.errors) == {specs[1].rid, specs[4].rid}, srv.errors
     assert "non-finite" in srv.errors[specs[1].rid]

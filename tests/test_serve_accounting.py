@@ -218,3 +218,16 @@ def test_base_inputs_go_with_their_mesh(monkeypatch):
     assert not serve._has_base_case(b, "cpu", 1.0) or serve._BASE[(id(b), "cpu", 1.0)][0]() is b
     serve._base_case(b, "cpu", 1.0)
     assert serve._base_case(b, "cpu", 1.0) == {"mesh": id(b)} and len(built) == 2
+
+
+def test_staged_copy_covers_every_chunk():
+    """The staged download copies every element, whatever the size relation between the
+    field and the staging buffer (CPU tensors stand for device and pinned memory)."""
+    import torch
+    waits = []
+    for n, cs in ((1, 7), (7, 7), (20, 7), (21, 7), (5, 64)):
+        src = torch.arange(n, dtype=torch.float64) * 0.5 - 3.0
+        stage = torch.full((cs,), float("nan"), dtype=torch.float64)
+        out = serve.staged_to_host(src, stage, lambda: waits.append(1))
+        assert torch.equal(out, src) and out.data_ptr() != src.data_ptr()
+    assert len(waits) == 1 + 1 + 3 + 3 + 1
