@@ -724,10 +724,11 @@ class Server:
         if self._stage is None:
             self._stage = torch.empty(self.stage_elems, dtype=torch.float64, pin_memory=True)
             self._dl_stream = torch.cuda.Stream(self.device)      # default priority
-        src = x.reshape(-1)
-        if src.dtype != torch.float64:
+        if x.dtype != torch.float64:
             return x.cpu()
         with torch.cuda.stream(self._dl_stream):
+            # a view for the contiguous results; any copy is made on this stream too
+            src = x.reshape(-1)
             out = staged_to_host(src, self._stage, self._dl_stream.synchronize)
         return out.reshape(x.shape)
 
